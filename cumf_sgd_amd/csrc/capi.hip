@@ -1,0 +1,867 @@
+// C ABI of the device path (include/cumf_sgd_amd.h): a context owns the device
+// copies of the ratings and factors and drives the gfx950 kernels on its own
+// stream.  Replaces the device half of sgd_update_k128
+// (singleGPU/sgd_kernel.cu:125-720) and the device kernels of sgd.cu.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cumf_sgd_amd.h"
+#include "errors.h"
+#include "kernels.h"
+
+using namespace cumf;
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int cumf::fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+namespace {
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return fail(CUMF_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                      \
+    } while (0)
+
+template <typename T>
+int ensure(T** p, size_t* cap, size_t count)
+{
+    if (*cap >= count && *p) return CUMF_OK;
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (count == 0) return CUMF_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(CUMF_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    *cap = count;
+    return CUMF_OK;
+}
+
+template <typename T>
+void release(T** p, size_t* cap)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+}
+
+// Device scratch that lives for one call.
+struct DevTmp {
+    void* p = nullptr;
+    ~DevTmp() { if (p) (void)hipFree(p); }
+};
+
+int prec_bytes(int prec) { return prec == CUMF_FP32 ? 4 : (prec == CUMF_FP8_E4M3 ? 1 : 2); }
+
+}  // namespace
+
+struct cumf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+
+    int32_t* u = nullptr; size_t u_cap = 0;
+    int32_t* v = nullptr; size_t v_cap = 0;
+    float* r = nullptr; size_t r_cap = 0;
+    int64_t nnz = 0;
+    int32_t m = 0, n = 0;
+    bool have_ratings = false;
+
+    int32_t* tu = nullptr; size_t tu_cap = 0;
+    int32_t* tv = nullptr; size_t tv_cap = 0;
+    float* tr = nullptr; size_t tr_cap = 0;
+    int64_t tnnz = 0;
+    bool have_test = false;
+
+    int32_t ux = 1, vy = 1;
+    std::vector<int64_t> offsets{0, 0};
+
+    int32_t k = 0, prec = -1;
+    void* P = nullptr; size_t P_cap = 0;
+    void* Q = nullptr; size_t Q_cap = 0;
+    float* ps = nullptr; size_t ps_cap = 0;
+    float* qs = nullptr; size_t qs_cap = 0;
+    bool have_factors = false;
+
+    float* lr = nullptr; size_t lr_cap = 0;
+    SgdTask* tasks = nullptr; size_t tasks_cap = 0;
+    double* partials = nullptr; size_t partials_cap = 0;
+
+    std::vector<hipEvent_t> ev;   // start/stop pairs of training launches
+    size_t ev_used = 0;
+    double kernel_ms = 0.0;
+    int64_t launches = 0, updates = 0;
+    cumf_launch_info last{};
+};
+
+namespace {
+
+int check_ctx(cumf_ctx* c)
+{
+    if (!c) return fail(CUMF_ERR_ARG, "null context");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return fail(CUMF_ERR_HIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e));
+    return CUMF_OK;
+}
+
+// Copy `bytes` from a host or device pointer into device memory.
+int copy_in(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return CUMF_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+    return CUMF_OK;
+}
+
+int upload_soa(cumf_ctx* c, const int32_t* u, const int32_t* v, const float* r, int64_t nnz, bool test)
+{
+    int32_t** du = test ? &c->tu : &c->u;
+    int32_t** dv = test ? &c->tv : &c->v;
+    float** dr = test ? &c->tr : &c->r;
+    size_t* cu = test ? &c->tu_cap : &c->u_cap;
+    size_t* cv = test ? &c->tv_cap : &c->v_cap;
+    size_t* cr = test ? &c->tr_cap : &c->r_cap;
+    int rc;
+    if ((rc = ensure(du, cu, (size_t)nnz)) || (rc = ensure(dv, cv, (size_t)nnz)) || (rc = ensure(dr, cr, (size_t)nnz)))
+        return rc;
+    if ((rc = copy_in(*du, u, sizeof(int32_t) * nnz, c->stream))) return rc;
+    if ((rc = copy_in(*dv, v, sizeof(int32_t) * nnz, c->stream))) return rc;
+    if ((rc = copy_in(*dr, r, sizeof(float) * nnz, c->stream))) return rc;
+    return CUMF_OK;
+}
+
+int upload_aos(cumf_ctx* c, const cumf_node* R, int64_t nnz, bool test)
+{
+    int32_t** du = test ? &c->tu : &c->u;
+    int32_t** dv = test ? &c->tv : &c->v;
+    float** dr = test ? &c->tr : &c->r;
+    size_t* cu = test ? &c->tu_cap : &c->u_cap;
+    size_t* cv = test ? &c->tv_cap : &c->v_cap;
+    size_t* cr = test ? &c->tr_cap : &c->r_cap;
+    int rc;
+    if ((rc = ensure(du, cu, (size_t)nnz)) || (rc = ensure(dv, cv, (size_t)nnz)) || (rc = ensure(dr, cr, (size_t)nnz)))
+        return rc;
+    if (nnz == 0) return CUMF_OK;
+    DevTmp tmp;
+    HIP_TRY(hipMalloc(&tmp.p, sizeof(cumf_node) * (size_t)nnz));
+    if ((rc = copy_in(tmp.p, R, sizeof(cumf_node) * (size_t)nnz, c->stream))) return rc;
+    HIP_TRY(launch_split(tmp.p, nnz, *du, *dv, *dr, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int validate_ids(cumf_ctx* c)
+{
+    DevTmp bad;
+    HIP_TRY(hipMalloc(&bad.p, sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), c->stream));
+    HIP_TRY(launch_check_ids(c->u, c->v, c->nnz, c->m, c->n, (unsigned long long*)bad.p, c->stream));
+    unsigned long long hb = 0;
+    HIP_TRY(hipMemcpyAsync(&hb, bad.p, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hb) return fail(CUMF_ERR_ARG, "%llu training ratings have ids outside [0,%d) x [0,%d)", hb, c->m, c->n);
+    return CUMF_OK;
+}
+
+int after_ratings(cumf_ctx* c, int64_t nnz, int32_t m, int32_t n)
+{
+    c->nnz = nnz;
+    c->m = m;
+    c->n = n;
+    c->ux = c->vy = 1;
+    c->offsets.assign({0, nnz});
+    int rc = validate_ids(c);
+    c->have_ratings = rc == CUMF_OK;
+    return rc;
+}
+
+hipEvent_t next_event(cumf_ctx* c)
+{
+    if (c->ev_used == c->ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev.push_back(e);
+    }
+    return c->ev[c->ev_used++];
+}
+
+// Sum the start/stop event pairs recorded since the last harvest.
+int harvest_events(cumf_ctx* c)
+{
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        c->kernel_ms += ms;
+    }
+    c->ev_used = 0;
+    return CUMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumf_abi_version(void) { return CUMF_ABI_VERSION; }
+
+const char* cumf_last_error(void) { return g_last_error.c_str(); }
+
+int cumf_create(int device, cumf_ctx** out)
+{
+    if (!out) return fail(CUMF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(CUMF_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0) device = 0;
+    device %= count;   // the reference selects gpu % deviceCount (singleGPU/main.cpp:292-294)
+    HIP_TRY(hipSetDevice(device));
+    cumf_ctx* c = new cumf_ctx;
+    c->device = device;
+    hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (se != hipSuccess) {
+        delete c;
+        return fail(CUMF_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(se));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cu_count = prop.multiProcessorCount;
+    *out = c;
+    return CUMF_OK;
+}
+
+int cumf_destroy(cumf_ctx* c)
+{
+    if (!c) return CUMF_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(&c->u, &c->u_cap); release(&c->v, &c->v_cap); release(&c->r, &c->r_cap);
+    release(&c->tu, &c->tu_cap); release(&c->tv, &c->tv_cap); release(&c->tr, &c->tr_cap);
+    release(&c->P, &c->P_cap); release(&c->Q, &c->Q_cap);
+    release(&c->ps, &c->ps_cap); release(&c->qs, &c->qs_cap);
+    release(&c->lr, &c->lr_cap); release(&c->tasks, &c->tasks_cap); release(&c->partials, &c->partials_cap);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CUMF_OK;
+}
+
+int cumf_sync(cumf_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int cumf_set_ratings(cumf_ctx* c, const cumf_node* R, int64_t nnz, int32_t m, int32_t n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 0 || m <= 0 || n <= 0 || (nnz > 0 && !R)) return fail(CUMF_ERR_ARG, "bad ratings (nnz=%lld m=%d n=%d)", (long long)nnz, m, n);
+    c->have_ratings = false;
+    if ((rc = upload_aos(c, R, nnz, false))) return rc;
+    return after_ratings(c, nnz, m, n);
+}
+
+int cumf_set_ratings_soa(cumf_ctx* c, const int32_t* u, const int32_t* v, const float* r, int64_t nnz, int32_t m,
+                         int32_t n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 0 || m <= 0 || n <= 0 || (nnz > 0 && (!u || !v || !r)))
+        return fail(CUMF_ERR_ARG, "bad ratings (nnz=%lld m=%d n=%d)", (long long)nnz, m, n);
+    c->have_ratings = false;
+    if ((rc = upload_soa(c, u, v, r, nnz, false))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return after_ratings(c, nnz, m, n);
+}
+
+int cumf_set_test(cumf_ctx* c, const cumf_node* R, int64_t nnz)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 0 || (nnz > 0 && !R)) return fail(CUMF_ERR_ARG, "bad test ratings");
+    if ((rc = upload_aos(c, R, nnz, true))) return rc;
+    c->tnnz = nnz;
+    c->have_test = true;
+    return CUMF_OK;
+}
+
+int cumf_set_test_soa(cumf_ctx* c, const int32_t* u, const int32_t* v, const float* r, int64_t nnz)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 0 || (nnz > 0 && (!u || !v || !r))) return fail(CUMF_ERR_ARG, "bad test ratings");
+    if ((rc = upload_soa(c, u, v, r, nnz, true))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->tnnz = nnz;
+    c->have_test = true;
+    return CUMF_OK;
+}
+
+int cumf_get_ratings(cumf_ctx* c, int32_t* u, int32_t* v, float* r)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (c->nnz == 0) return CUMF_OK;
+    if (u) HIP_TRY(hipMemcpyAsync(u, c->u, sizeof(int32_t) * c->nnz, hipMemcpyDefault, c->stream));
+    if (v) HIP_TRY(hipMemcpyAsync(v, c->v, sizeof(int32_t) * c->nnz, hipMemcpyDefault, c->stream));
+    if (r) HIP_TRY(hipMemcpyAsync(r, c->r, sizeof(float) * c->nnz, hipMemcpyDefault, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int cumf_rating_moments(cumf_ctx* c, double* sum, double* sum_sq)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    const size_t need = 2 * (size_t)moments_blocks(c->nnz) + 2;
+    if ((rc = ensure(&c->partials, &c->partials_cap, need))) return rc;
+    double* out2 = c->partials + need - 2;
+    HIP_TRY(launch_moments(c->r, c->nnz, c->partials, out2, c->stream));
+    double h[2];
+    HIP_TRY(hipMemcpyAsync(h, out2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (sum) *sum = h[0];
+    if (sum_sq) *sum_sq = h[1];
+    return CUMF_OK;
+}
+
+int cumf_remap(cumf_ctx* c, const int32_t* p_map, const int32_t* q_map)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (!p_map || !q_map) return fail(CUMF_ERR_ARG, "null map");
+    DevTmp dp, dq;
+    HIP_TRY(hipMalloc(&dp.p, sizeof(int32_t) * (size_t)c->m));
+    HIP_TRY(hipMalloc(&dq.p, sizeof(int32_t) * (size_t)c->n));
+    if ((rc = copy_in(dp.p, p_map, sizeof(int32_t) * c->m, c->stream))) return rc;
+    if ((rc = copy_in(dq.p, q_map, sizeof(int32_t) * c->n, c->stream))) return rc;
+    // maps must be permutations of [0,m) / [0,n): check the range on the host copy
+    std::vector<int32_t> hp(c->m), hq(c->n);
+    HIP_TRY(hipMemcpyAsync(hp.data(), dp.p, sizeof(int32_t) * c->m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hq.data(), dq.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    {
+        std::vector<char> seen(std::max(c->m, c->n), 0);
+        for (int32_t x : hp) {
+            if (x < 0 || x >= c->m || seen[x]) return fail(CUMF_ERR_ARG, "p_map is not a permutation of [0,%d)", c->m);
+            seen[x] = 1;
+        }
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int32_t x : hq) {
+            if (x < 0 || x >= c->n || seen[x]) return fail(CUMF_ERR_ARG, "q_map is not a permutation of [0,%d)", c->n);
+            seen[x] = 1;
+        }
+    }
+    HIP_TRY(launch_remap(c->u, c->v, c->nnz, (const int32_t*)dp.p, (const int32_t*)dq.p, c->m, c->n, c->stream));
+    if (c->have_test)
+        HIP_TRY(launch_remap(c->tu, c->tv, c->tnnz, (const int32_t*)dp.p, (const int32_t*)dq.p, c->m, c->n, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ux = c->vy = 1;
+    c->offsets.assign({0, c->nnz});
+    return CUMF_OK;
+}
+
+int cumf_scale_ratings(cumf_ctx* c, float scale)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    HIP_TRY(launch_scale(c->r, c->nnz, scale, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Partition with per-axis segment boundaries bu[0..ux], bv[0..vy].
+int partition_with(cumf_ctx* c, const std::vector<int64_t>& bu, const std::vector<int64_t>& bv, int64_t* offsets)
+{
+    const int ux = (int)bu.size() - 1, vy = (int)bv.size() - 1;
+    const int nb = ux * vy;
+    if (nb < 1 || nb > 4096) return fail(CUMF_ERR_ARG, "grid of %d blocks unsupported (1..4096)", nb);
+    const int64_t nnz = c->nnz;
+    DevTmp useg, vseg, bnd, hist, tot, base, off, uo, vo, ro;
+    HIP_TRY(hipMalloc(&useg.p, sizeof(int32_t) * (size_t)c->m));
+    HIP_TRY(hipMalloc(&vseg.p, sizeof(int32_t) * (size_t)c->n));
+    HIP_TRY(hipMalloc(&bnd.p, sizeof(int64_t) * (size_t)(ux + vy + 2)));
+    HIP_TRY(hipMemcpyAsync(bnd.p, bu.data(), sizeof(int64_t) * (ux + 1), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync((int64_t*)bnd.p + ux + 1, bv.data(), sizeof(int64_t) * (vy + 1), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_seg_table((int32_t*)useg.p, c->m, (const int64_t*)bnd.p, ux, c->stream));
+    HIP_TRY(launch_seg_table((int32_t*)vseg.p, c->n, (const int64_t*)bnd.p + ux + 1, vy, c->stream));
+    std::vector<int64_t> host(2 * (size_t)nb);
+    if (nnz > 0) {
+        const int64_t he = part_hist_entries(nnz, nb);
+        HIP_TRY(hipMalloc(&hist.p, sizeof(int32_t) * (size_t)he));
+        HIP_TRY(hipMalloc(&off.p, sizeof(int64_t) * (size_t)he));
+        HIP_TRY(hipMalloc(&tot.p, sizeof(int64_t) * (size_t)nb));
+        HIP_TRY(hipMalloc(&base.p, sizeof(int64_t) * (size_t)nb));
+        HIP_TRY(hipMalloc(&uo.p, sizeof(int32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&vo.p, sizeof(int32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&ro.p, sizeof(float) * (size_t)nnz));
+        HIP_TRY(launch_partition(c->u, c->v, c->r, nnz, (const int32_t*)useg.p, (const int32_t*)vseg.p, vy, nb,
+                                 (int32_t*)hist.p, (int64_t*)tot.p, (int64_t*)base.p, (int64_t*)off.p,
+                                 (int32_t*)uo.p, (int32_t*)vo.p, (float*)ro.p, c->stream, host.data()));
+        // adopt the partitioned arrays
+        std::swap(*(void**)&c->u, uo.p);
+        std::swap(*(void**)&c->v, vo.p);
+        std::swap(*(void**)&c->r, ro.p);
+        c->u_cap = c->v_cap = c->r_cap = (size_t)nnz;
+    } else {
+        std::fill(host.begin(), host.end(), 0);
+    }
+    c->ux = ux;
+    c->vy = vy;
+    c->offsets.assign((size_t)nb + 1, 0);
+    for (int b = 0; b < nb; ++b) c->offsets[b + 1] = c->offsets[b] + host[b];
+    if (offsets) std::copy(c->offsets.begin(), c->offsets.end(), offsets);
+    return CUMF_OK;
+}
+
+// Boundaries splitting `rows` ids into s contiguous ranges of about equal
+// rating count, from the degree histogram.
+std::vector<int64_t> balanced_bounds(const std::vector<int32_t>& deg, int s, int64_t nnz)
+{
+    const int64_t rows = (int64_t)deg.size();
+    std::vector<int64_t> b(s + 1, rows);
+    b[0] = 0;
+    int64_t run = 0;
+    int next = 1;
+    for (int64_t i = 0; i < rows && next < s; ++i) {
+        run += deg[i];
+        while (next < s && run * s >= nnz * next) b[next++] = i + 1;
+    }
+    for (int j = 1; j <= s; ++j) b[j] = std::max(b[j], b[j - 1]);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumf_partition(cumf_ctx* c, int32_t ux, int32_t vy, int64_t* offsets)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (ux < 1 || vy < 1) return fail(CUMF_ERR_ARG, "bad grid %d x %d", ux, vy);
+    // segment lengths of grid_problem (singleGPU/sgd.cu:419-423)
+    const int64_t useg = ux == 1 ? c->m : (int64_t)std::ceil((double)c->m / ux);
+    const int64_t vseg = vy == 1 ? c->n : (int64_t)std::ceil((double)c->n / vy);
+    std::vector<int64_t> bu(ux + 1), bv(vy + 1);
+    for (int i = 0; i <= ux; ++i) bu[i] = std::min<int64_t>((int64_t)i * useg, c->m);
+    for (int i = 0; i <= vy; ++i) bv[i] = std::min<int64_t>((int64_t)i * vseg, c->n);
+    bu[ux] = c->m;
+    bv[vy] = c->n;
+    return partition_with(c, bu, bv, offsets);
+}
+
+int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (s < 1 || s > 64) return fail(CUMF_ERR_ARG, "strata %d out of range 1..64", s);
+    DevTmp du, dv;
+    HIP_TRY(hipMalloc(&du.p, sizeof(int32_t) * (size_t)c->m));
+    HIP_TRY(hipMalloc(&dv.p, sizeof(int32_t) * (size_t)c->n));
+    HIP_TRY(hipMemsetAsync(du.p, 0, sizeof(int32_t) * (size_t)c->m, c->stream));
+    HIP_TRY(hipMemsetAsync(dv.p, 0, sizeof(int32_t) * (size_t)c->n, c->stream));
+    HIP_TRY(launch_degree(c->u, c->nnz, (int32_t*)du.p, c->stream));
+    HIP_TRY(launch_degree(c->v, c->nnz, (int32_t*)dv.p, c->stream));
+    std::vector<int32_t> hu(c->m), hv(c->n);
+    HIP_TRY(hipMemcpyAsync(hu.data(), du.p, sizeof(int32_t) * c->m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hv.data(), dv.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return partition_with(c, balanced_bounds(hu, s, c->nnz), balanced_bounds(hv, s, c->nnz), offsets);
+}
+
+int cumf_set_blocks(cumf_ctx* c, int32_t ux, int32_t vy, const int64_t* offsets)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (ux < 1 || vy < 1 || (int64_t)ux * vy > 4096 || !offsets) return fail(CUMF_ERR_ARG, "bad block grid");
+    const int nb = ux * vy;
+    if (offsets[0] != 0 || offsets[nb] != c->nnz) return fail(CUMF_ERR_ARG, "block offsets must span [0, nnz]");
+    for (int b = 0; b < nb; ++b)
+        if (offsets[b + 1] < offsets[b]) return fail(CUMF_ERR_ARG, "block offsets must be ascending");
+    c->ux = ux;
+    c->vy = vy;
+    c->offsets.assign(offsets, offsets + nb + 1);
+    return CUMF_OK;
+}
+
+int cumf_init_factors(cumf_ctx* c, int32_t k, int32_t precision, uint64_t seed)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "set ratings (m, n) before factors");
+    if (k != 32 && k != 64 && k != 128) return fail(CUMF_ERR_ARG, "k=%d unsupported (32, 64 or 128)", k);
+    if (precision < CUMF_FP32 || precision > CUMF_FP8_E4M3) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
+    const size_t eb = (size_t)prec_bytes(precision);
+    char** P = (char**)&c->P;
+    char** Q = (char**)&c->Q;
+    if ((rc = ensure(P, &c->P_cap, (size_t)c->m * k * eb))) return rc;
+    if ((rc = ensure(Q, &c->Q_cap, (size_t)c->n * k * eb))) return rc;
+    if (precision == CUMF_FP8_E4M3) {
+        if ((rc = ensure(&c->ps, &c->ps_cap, (size_t)c->m))) return rc;
+        if ((rc = ensure(&c->qs, &c->qs_cap, (size_t)c->n))) return rc;
+    }
+    c->k = k;
+    c->prec = precision;
+    const float s = (float)std::sqrt(1.0 / k);   // singleGPU/sgd.cu:519
+    HIP_TRY(launch_init(c->P, c->ps, c->m, k, precision, s, seed, 0, kStreamInitP, c->stream));
+    HIP_TRY(launch_init(c->Q, c->qs, c->n, k, precision, s, seed, 0, kStreamInitQ, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_factors = true;
+    return CUMF_OK;
+}
+
+static int factor_ref(cumf_ctx* c, int32_t which, void** F, float** S, int64_t* rows)
+{
+    if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
+    if (which != 0 && which != 1) return fail(CUMF_ERR_ARG, "which must be 0 (P) or 1 (Q)");
+    *F = which ? c->Q : c->P;
+    *S = which ? c->qs : c->ps;
+    *rows = which ? c->n : c->m;
+    return CUMF_OK;
+}
+
+int cumf_set_factors(cumf_ctx* c, int32_t which, const void* data, const float* scales)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    void* F; float* S; int64_t rows;
+    if ((rc = factor_ref(c, which, &F, &S, &rows))) return rc;
+    if (!data) return fail(CUMF_ERR_ARG, "null data");
+    if ((rc = copy_in(F, data, (size_t)rows * c->k * prec_bytes(c->prec), c->stream))) return rc;
+    if (c->prec == CUMF_FP8_E4M3) {
+        if (!scales) return fail(CUMF_ERR_ARG, "FP8 factors need row scales");
+        if ((rc = copy_in(S, scales, sizeof(float) * rows, c->stream))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int cumf_get_factors(cumf_ctx* c, int32_t which, void* data, float* scales)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    void* F; float* S; int64_t rows;
+    if ((rc = factor_ref(c, which, &F, &S, &rows))) return rc;
+    if (data) HIP_TRY(hipMemcpyAsync(data, F, (size_t)rows * c->k * prec_bytes(c->prec), hipMemcpyDefault, c->stream));
+    if (scales && c->prec == CUMF_FP8_E4M3) HIP_TRY(hipMemcpyAsync(scales, S, sizeof(float) * rows, hipMemcpyDefault, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int cumf_export_factors(cumf_ctx* c, int32_t which, float mult, const int32_t* map, float* out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    void* F; float* S; int64_t rows;
+    if ((rc = factor_ref(c, which, &F, &S, &rows))) return rc;
+    if (!out) return fail(CUMF_ERR_ARG, "null output");
+    DevTmp dmap, dout;
+    const int32_t* mp = nullptr;
+    if (map) {
+        std::vector<int32_t> hm(rows);
+        HIP_TRY(hipMemcpy(hm.data(), map, sizeof(int32_t) * rows, hipMemcpyDefault));
+        for (int32_t x : hm)
+            if (x < 0 || x >= rows) return fail(CUMF_ERR_ARG, "map entry %d outside [0,%lld)", x, (long long)rows);
+        HIP_TRY(hipMalloc(&dmap.p, sizeof(int32_t) * rows));
+        HIP_TRY(hipMemcpyAsync(dmap.p, hm.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice, c->stream));
+        mp = (const int32_t*)dmap.p;
+    }
+    HIP_TRY(hipMalloc(&dout.p, sizeof(float) * rows * c->k));
+    HIP_TRY(launch_export(F, S, rows, c->k, c->prec, mult, mp, (float*)dout.p, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, sizeof(float) * rows * c->k, hipMemcpyDefault, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+int cumf_test_rmse(cumf_ctx* c, float mult, float b, double* rmse)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
+    if (!c->have_test) return fail(CUMF_ERR_STATE, "no test ratings");
+    if (!rmse) return fail(CUMF_ERR_ARG, "null output");
+    if (c->tnnz == 0) {
+        *rmse = 0.0;   // test/mf.cpp:3818-3819
+        return CUMF_OK;
+    }
+    const size_t need = (size_t)rmse_blocks(c->tnnz) + 1;
+    if ((rc = ensure(&c->partials, &c->partials_cap, need))) return rc;
+    double* out = c->partials + need - 1;
+    HIP_TRY(launch_rmse(c->tu, c->tv, c->tr, c->tnnz, c->P, c->ps, c->Q, c->qs, c->m, c->n, c->k, c->prec, mult, b,
+                        c->partials, out, c->stream));
+    double h = 0.0;
+    HIP_TRY(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *rmse = std::sqrt(h / (double)c->tnnz);
+    return CUMF_OK;
+}
+
+int cumf_kernel_time(cumf_ctx* c, double* ms, int64_t* launches, int64_t* updates)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = harvest_events(c))) return rc;
+    if (ms) *ms = c->kernel_ms;
+    if (launches) *launches = c->launches;
+    if (updates) *updates = c->updates;
+    return CUMF_OK;
+}
+
+int cumf_reset_kernel_time(cumf_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = harvest_events(c))) return rc;
+    c->kernel_ms = 0.0;
+    c->launches = 0;
+    c->updates = 0;
+    return CUMF_OK;
+}
+
+int cumf_last_launch(cumf_ctx* c, cumf_launch_info* info)
+{
+    if (!c || !info) return fail(CUMF_ERR_ARG, "null argument");
+    *info = c->last;
+    return CUMF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Training.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Shape {
+    int G, S, L, W, blocks;
+};
+
+int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
+{
+    sh->G = o->lanes_per_sample ? o->lanes_per_sample : sgd_default_group(c->k, c->prec);
+    if (!sgd_shape_supported(c->k, c->prec, sh->G))
+        return fail(CUMF_ERR_ARG, "lanes_per_sample=%d unsupported for k=%d precision=%d", sh->G, c->k, c->prec);
+    if (o->coherent && (c->k / sh->G) * prec_bytes(c->prec) < 4)
+        return fail(CUMF_ERR_ARG, "coherent mode needs >= 4 bytes per lane");
+    sh->S = 64 / sh->G;
+    sh->L = o->batch_len ? o->batch_len : 128;
+    if (sh->L < 1 || sh->L > kSgdMaxBatch) return fail(CUMF_ERR_ARG, "batch_len=%d out of range 1..%d", sh->L, kSgdMaxBatch);
+    int W;
+    if (o->schedule == CUMF_SCHED_SERIAL) {
+        W = 1;
+    } else if (o->num_workers > 0) {
+        W = o->num_workers;
+    } else {
+        // default: 8 waves per CU
+        W = c->cu_count * 8 * sh->S;
+    }
+    const int per_block = kSgdWavesPerBlock * sh->S;
+    int blocks = (W + per_block - 1) / per_block;
+    if (xcd_multiple > 1) blocks = (blocks + xcd_multiple - 1) / xcd_multiple * xcd_multiple;
+    sh->W = W;
+    sh->blocks = blocks;
+    return CUMF_OK;
+}
+
+int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
+{
+    // singleGPU/sgd_kernel.cu:137-154: lr[i] = float(alpha/(1+beta*i^1.5) + lrate)
+    std::vector<float> h((size_t)n);
+    const float alpha = o->alpha, beta = o->beta, lrate = o->lrate;
+    for (int i = 0; i < n; ++i) {
+        double t = alpha / (1 + beta * std::pow(i, 1.5)) + lrate;
+        h[i] = (float)t;
+    }
+    int rc = ensure(&c->lr, &c->lr_cap, (size_t)n);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->lr, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    return CUMF_OK;
+}
+
+int run_launch(cumf_ctx* c, const std::vector<SgdTask>& tasks, const Shape& sh, int ep, const cumf_train_opts* o,
+               int num_xcd)
+{
+    int rc = ensure(&c->tasks, &c->tasks_cap, tasks.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
+    SgdLaunch a{};
+    a.tasks = c->tasks;
+    a.num_tasks = (int32_t)tasks.size();
+    a.epoch_begin = ep;
+    a.epoch_end = ep + 1;
+    a.batch_len = sh.L;
+    a.seed = o->seed;
+    a.num_xcd = num_xcd;
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    if (!e0 || !e1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, c->stream));
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    c->launches += 1;
+    for (const SgdTask& t : tasks) c->updates += (int64_t)t.worker_count * t.update_count * sh.L;
+    // the tasks buffer is reused by the next launch: wait for this one
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+SgdTask base_task(cumf_ctx* c, const cumf_train_opts* o, int64_t beg, int64_t end, int W, int L, uint32_t tag)
+{
+    SgdTask t{};
+    t.u = c->u + beg;
+    t.v = c->v + beg;
+    t.r = c->r + beg;
+    t.nnz = end - beg;
+    t.P = c->P;
+    t.Q = c->Q;
+    t.ps = c->ps;
+    t.qs = c->qs;
+    t.lr = c->lr;
+    t.lambda_p = o->lambda_p;
+    t.lambda_q = o->lambda_q;
+    t.worker_begin = 0;
+    t.worker_count = W;
+    // update_count = ceil(size / (W * L)) (singleGPU/sgd_kernel.cu:203)
+    t.update_count = (int32_t)((t.nnz + (int64_t)W * L - 1) / ((int64_t)W * L));
+    t.problem = 0;
+    t.tag = tag;
+    return t;
+}
+
+}  // namespace
+
+extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int32_t e1, const int32_t* block_order,
+                          int64_t order_len)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!o) return fail(CUMF_ERR_ARG, "null options");
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
+    if (o->k != c->k) return fail(CUMF_ERR_ARG, "opts k=%d but factors have k=%d", o->k, c->k);
+    if (e0 < 0 || e1 < e0 || e1 > 1 << 20) return fail(CUMF_ERR_ARG, "bad epoch range [%d,%d)", e0, e1);
+    if (o->schedule < CUMF_SCHED_HOGWILD || o->schedule > CUMF_SCHED_STRATUM) return fail(CUMF_ERR_ARG, "bad schedule");
+    if (c->nnz == 0 || e1 == e0) return CUMF_OK;
+
+    const int nb = c->ux * c->vy;
+    const bool stratum = o->schedule == CUMF_SCHED_STRATUM;
+    if (stratum && (c->ux != c->vy || c->ux < 2))
+        return fail(CUMF_ERR_STATE, "stratum schedule needs an s x s partition (have %d x %d)", c->ux, c->vy);
+    const int num_xcd = stratum ? 8 : 1;
+    Shape sh;
+    if ((rc = resolve_shape(c, o, &sh, num_xcd))) return rc;
+    if (stratum) sh.W = sh.blocks * kSgdWavesPerBlock * sh.S;   // every wave of every XCD works
+    if ((rc = upload_lr(c, o, e1 + 4))) return rc;
+
+    c->last.num_workers = sh.W;
+    c->last.lanes_per_sample = sh.G;
+    c->last.waves_per_block = kSgdWavesPerBlock;
+    c->last.blocks = sh.blocks;
+    c->last.batch_len = sh.L;
+    c->last.updates_per_epoch = 0;
+
+    if (o->schedule == CUMF_SCHED_GRID && block_order) {
+        if (order_len != nb && order_len != (int64_t)nb * (e1 - e0))
+            return fail(CUMF_ERR_ARG, "block_order has %lld entries (need %d or %lld)", (long long)order_len, nb,
+                        (long long)nb * (e1 - e0));
+        for (int64_t i = 0; i < order_len; ++i)
+            if (block_order[i] < 0 || block_order[i] >= nb) return fail(CUMF_ERR_ARG, "block_order entry out of range");
+    }
+
+    for (int ep = e0; ep < e1; ++ep) {
+        int64_t upd_epoch = 0;
+        if (o->schedule == CUMF_SCHED_HOGWILD || o->schedule == CUMF_SCHED_SERIAL) {
+            std::vector<SgdTask> t{base_task(c, o, 0, c->nnz, sh.W, sh.L, 0)};
+            upd_epoch += (int64_t)sh.W * t[0].update_count * sh.L;
+            if ((rc = run_launch(c, t, sh, ep, o, 1))) return rc;
+        } else if (o->schedule == CUMF_SCHED_GRID) {
+            for (int i = 0; i < nb; ++i) {
+                int blk = i;
+                if (block_order) blk = block_order[order_len == nb ? i : (int64_t)(ep - e0) * nb + i];
+                const int64_t beg = c->offsets[blk], end = c->offsets[blk + 1];
+                if (end == beg) continue;
+                std::vector<SgdTask> t{base_task(c, o, beg, end, sh.W, sh.L, (uint32_t)blk)};
+                upd_epoch += (int64_t)sh.W * t[0].update_count * sh.L;
+                if ((rc = run_launch(c, t, sh, ep, o, 1))) return rc;
+            }
+        } else {
+            // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
+            // which share no P and no Q rows; block i runs on XCD i mod 8.
+            const int s = c->ux;
+            const int per_xcd = sh.W / num_xcd;
+            for (int d = 0; d < s; ++d) {
+                std::vector<SgdTask> t;
+                for (int x = 0; x < num_xcd; ++x) {
+                    // blocks of this diagonal assigned to XCD x, workers split by size
+                    std::vector<int> mine;
+                    int64_t tot = 0;
+                    for (int i = x; i < s; i += num_xcd) {
+                        const int blk = i * s + (i + d) % s;
+                        mine.push_back(blk);
+                        tot += c->offsets[blk + 1] - c->offsets[blk];
+                    }
+                    int wb = x * per_xcd, left = per_xcd;
+                    for (size_t j = 0; j < mine.size(); ++j) {
+                        const int blk = mine[j];
+                        const int64_t sz = c->offsets[blk + 1] - c->offsets[blk];
+                        int w = j + 1 == mine.size() ? left
+                                                      : (int)std::max<int64_t>(1, tot ? (int64_t)per_xcd * sz / tot : 1);
+                        w = std::min(w, left - (int)(mine.size() - 1 - j));
+                        if (w < 1 || sz == 0) continue;
+                        SgdTask tk = base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk);
+                        tk.worker_begin = wb;
+                        wb += w;
+                        left -= w;
+                        upd_epoch += (int64_t)w * tk.update_count * sh.L;
+                        t.push_back(tk);
+                    }
+                }
+                if (t.empty()) continue;
+                if ((rc = run_launch(c, t, sh, ep, o, num_xcd))) return rc;
+            }
+        }
+        if (c->prec == CUMF_FP8_E4M3) {
+            HIP_TRY(launch_fp8_rescale((uint8_t*)c->P, c->ps, c->m, c->k, c->stream));
+            HIP_TRY(launch_fp8_rescale((uint8_t*)c->Q, c->qs, c->n, c->k, c->stream));
+        }
+        c->last.updates_per_epoch = upd_epoch;
+    }
+    return harvest_events(c);
+}

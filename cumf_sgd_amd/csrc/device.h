@@ -1,0 +1,247 @@
+// Device helpers for the gfx950 SGD kernels: Philox4x32-10, factor storage
+// formats (FP32 / FP16 / BF16 / OCP FP8-E4M3) and lane-group row I/O.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cumf {
+
+enum Prec : int { kF32 = 0, kF16 = 1, kBF16 = 2, kE4M3 = 3 };
+
+template <int PREC> struct PrecBytes { static constexpr int value = PREC == kF32 ? 4 : (PREC == kE4M3 ? 1 : 2); };
+
+// Stream tags (low byte of Philox counter word 3).
+constexpr uint32_t kStreamInitP = 1u;
+constexpr uint32_t kStreamInitQ = 2u;
+constexpr uint32_t kStreamWindow = 3u;
+constexpr uint32_t kStreamRound = 4u;
+
+// FP8 row-scale targets: largest |x| of a row maps to at most this value
+// (headroom below 448 for growth between rescales).
+constexpr float kFp8InitTarget = 14.0f;
+constexpr float kFp8EpochTarget = 56.0f;
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+#ifdef __HIP_DEVICE_COMPILE__
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+#else
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+#endif
+        U4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+struct Key { uint32_t k0, k1; };
+
+__host__ __device__ inline Key make_key(uint64_t seed, uint32_t problem)
+{
+    return Key{(uint32_t)seed, (uint32_t)(seed >> 32) ^ problem};
+}
+
+// Window start in [0, nnz): high 64 bits of a 64x64 product.
+__device__ inline int64_t window_start(Key key, uint32_t tag, uint32_t worker, uint32_t epoch,
+                                       uint32_t batch, int64_t nnz)
+{
+    U4 r = philox(U4{batch, epoch, worker, kStreamWindow | (tag << 8)}, key.k0, key.k1);
+    uint64_t x = ((uint64_t)r.y << 32) | r.x;
+    return (int64_t)__umul64hi(x, (uint64_t)nnz);
+}
+
+__host__ __device__ inline float u01_of(uint32_t x)
+{
+    return (float)((x >> 8) + 1u) * 0x1.0p-24f;
+}
+
+// ---------------------------------------------------------------------------
+// Formats.  Conversions are spelled out in integer arithmetic so that they are
+// bit-for-bit the same as the CPU definitions.
+// ---------------------------------------------------------------------------
+__device__ inline float f16_to_f32(uint32_t h)
+{
+    _Float16 x = __builtin_bit_cast(_Float16, (uint16_t)h);
+    return (float)x;
+}
+__device__ inline uint32_t f32_to_f16(float f)
+{
+    _Float16 x = (_Float16)f;   // v_cvt_f16_f32, round to nearest even
+    return __builtin_bit_cast(uint16_t, x);
+}
+__device__ inline float bf16_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+__device__ inline uint32_t f32_to_bf16(float f)
+{
+    uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return ((x >> 16) | 0x40u) & 0xffffu;
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return x >> 16;
+}
+__device__ inline float e4m3_to_f32(uint32_t b)
+{
+    uint32_t sign = (b >> 7) & 1u, exp = (b >> 3) & 0xfu, man = b & 7u;
+    float v;
+    if (exp == 0xfu && man == 7u) return __uint_as_float(0x7fc00000u | (sign << 31));
+    if (exp == 0u)
+        v = (float)man * 0x1.0p-9f;
+    else
+        v = __uint_as_float(((exp + 120u) << 23) | (man << 20));
+    return sign ? -v : v;
+}
+__device__ inline uint32_t e4m3_pack_normal(uint32_t ax)
+{
+    return (((ax >> 23) - 120u) << 3) | ((ax >> 20) & 7u);
+}
+__device__ inline uint32_t f32_to_e4m3_rne(float f)
+{
+    uint32_t x = __float_as_uint(f);
+    uint32_t sign = (x >> 24) & 0x80u, ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return sign | 0x7fu;
+    if (ax >= 0x43e00000u) return sign | 0x7eu;
+    if (ax < 0x3c800000u) return sign | (uint32_t)rintf(__uint_as_float(ax) * 0x1.0p9f);
+    uint32_t r = (ax + 0x7ffffu + ((ax >> 20) & 1u)) & 0xfff00000u;
+    r = r > 0x43e00000u ? 0x43e00000u : r;
+    return sign | e4m3_pack_normal(r);
+}
+// Stochastic rounding with 16 random bits.
+__device__ inline uint32_t f32_to_e4m3_sr(float f, uint32_t r16)
+{
+    uint32_t x = __float_as_uint(f);
+    uint32_t sign = (x >> 24) & 0x80u, ax = x & 0x7fffffffu;
+    r16 &= 0xffffu;
+    if (ax > 0x7f800000u) return sign | 0x7fu;
+    if (ax >= 0x43e00000u) return sign | 0x7eu;
+    if (ax < 0x3c800000u) {
+        float t = __uint_as_float(ax) * 0x1.0p9f;
+        float lo = floorf(t);
+        float frac = t - lo;
+        uint32_t q = (uint32_t)lo + (((float)r16 < frac * 0x1.0p16f) ? 1u : 0u);
+        return sign | q;
+    }
+    uint32_t r = (ax + (r16 << 4)) & 0xfff00000u;
+    r = r > 0x43e00000u ? 0x43e00000u : r;
+    return sign | e4m3_pack_normal(r);
+}
+
+// Smallest power of two s with amax <= target * s.
+__device__ inline float pow2_scale(float amax, float target)
+{
+    if (!(amax > 0.0f) || __builtin_isinf(amax)) return 1.0f;
+    int ea, et;
+    float fa = frexpf(amax, &ea);
+    float ft = frexpf(target, &et);
+    return ldexpf(1.0f, ea - et + (fa > ft ? 1 : 0));
+}
+
+// ---------------------------------------------------------------------------
+// Row I/O: a lane owns E consecutive elements of a row (TB = E * bytes).
+// COH = true uses device-scope relaxed atomics so every access is coherent
+// across XCDs (bypasses the per-XCD L2 copies).
+// ---------------------------------------------------------------------------
+template <int NW> struct Words { uint32_t w[NW]; };
+
+template <int TB, bool COH>
+__device__ inline Words<(TB + 3) / 4> load_raw(const void* p)
+{
+    Words<(TB + 3) / 4> r;
+    if constexpr (TB == 2) {
+        static_assert(!COH, "2-byte coherent loads are not supported");
+        r.w[0] = *(const uint16_t*)p;
+    } else if constexpr (TB == 4) {
+        if constexpr (COH)
+            r.w[0] = __hip_atomic_load((uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            r.w[0] = *(const uint32_t*)p;
+    } else if constexpr (TB == 8) {
+        uint64_t x;
+        if constexpr (COH)
+            x = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            x = *(const uint64_t*)p;
+        r.w[0] = (uint32_t)x;
+        r.w[1] = (uint32_t)(x >> 32);
+    } else {
+        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        if constexpr (COH) {
+            uint64_t a = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t b = __hip_atomic_load((uint64_t*)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            r.w[0] = (uint32_t)a; r.w[1] = (uint32_t)(a >> 32);
+            r.w[2] = (uint32_t)b; r.w[3] = (uint32_t)(b >> 32);
+        } else {
+            uint4 x = *(const uint4*)p;
+            r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+        }
+    }
+    return r;
+}
+
+template <int TB, bool COH>
+__device__ inline void store_raw(void* p, const Words<(TB + 3) / 4>& r)
+{
+    if constexpr (TB == 2) {
+        static_assert(!COH, "2-byte coherent stores are not supported");
+        *(uint16_t*)p = (uint16_t)r.w[0];
+    } else if constexpr (TB == 4) {
+        if constexpr (COH)
+            __hip_atomic_store((uint32_t*)p, r.w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            *(uint32_t*)p = r.w[0];
+    } else if constexpr (TB == 8) {
+        uint64_t x = ((uint64_t)r.w[1] << 32) | r.w[0];
+        if constexpr (COH)
+            __hip_atomic_store((uint64_t*)p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            *(uint64_t*)p = x;
+    } else {
+        if constexpr (COH) {
+            __hip_atomic_store((uint64_t*)p, ((uint64_t)r.w[1] << 32) | r.w[0], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((uint64_t*)p + 1, ((uint64_t)r.w[3] << 32) | r.w[2], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            *(uint4*)p = make_uint4(r.w[0], r.w[1], r.w[2], r.w[3]);
+        }
+    }
+}
+
+// Element i of a packed chunk, widened to FP32 (row scale applied by caller).
+template <int PREC, int NW>
+__device__ inline float unpack(const Words<NW>& r, int i)
+{
+    if constexpr (PREC == kF32) return __uint_as_float(r.w[i]);
+    else if constexpr (PREC == kF16) return f16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
+    else if constexpr (PREC == kBF16) return bf16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
+    else return e4m3_to_f32((r.w[i >> 2] >> ((i & 3) * 8)) & 0xffu);
+}
+
+template <int PREC, int NW>
+__device__ inline void pack(Words<NW>& r, int i, uint32_t bits)
+{
+    if constexpr (PREC == kF32) r.w[i] = bits;
+    else if constexpr (PREC == kF16 || PREC == kBF16) {
+        if ((i & 1) == 0) r.w[i >> 1] = bits;
+        else r.w[i >> 1] |= bits << 16;
+    } else {
+        if ((i & 3) == 0) r.w[i >> 2] = bits;
+        else r.w[i >> 2] |= bits << ((i & 3) * 8);
+    }
+}
+
+}  // namespace cumf

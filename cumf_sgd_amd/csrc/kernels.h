@@ -1,0 +1,72 @@
+// Launch interface between the C-ABI layer and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.h"
+
+namespace cumf {
+
+// One rating block trained by a contiguous range of workers in a launch.
+struct SgdTask {
+    const int32_t* u;         // block records (three arrays), global ids
+    const int32_t* v;
+    const float* r;
+    int64_t nnz;
+    void* P;                  // factor storage, row-major [rows][k]
+    void* Q;
+    float* ps;                // FP8 row scales (else unused)
+    float* qs;
+    const float* lr;          // learning-rate table indexed by epoch
+    float lambda_p, lambda_q;
+    int32_t worker_begin;     // workers [worker_begin, worker_begin + worker_count)
+    int32_t worker_count;
+    int32_t update_count;     // windows per worker per epoch
+    uint32_t problem;         // Philox key word (batched problems)
+    uint32_t tag;             // block id folded into the window counter
+    int32_t pad;
+};
+
+struct SgdLaunch {
+    const SgdTask* tasks;     // device array
+    int32_t num_tasks;
+    int32_t epoch_begin, epoch_end;
+    int32_t batch_len;        // <= 128
+    uint64_t seed;
+    int32_t num_xcd;          // > 1: workers numbered XCD-major over blockIdx % num_xcd
+    int32_t pad;
+};
+
+constexpr int kSgdWavesPerBlock = 4;
+constexpr int kSgdMaxBatch = 128;
+
+bool sgd_shape_supported(int k, int prec, int G);
+int sgd_default_group(int k, int prec);
+hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, hipStream_t s);
+
+hipError_t launch_init(void* F, float* scales, int64_t rows, int k, int prec, float s, uint64_t seed,
+                       uint32_t problem, uint32_t stream_tag, hipStream_t st);
+hipError_t launch_fp8_rescale(uint8_t* F, float* scales, int64_t rows, int k, hipStream_t st);
+hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k, int prec, float mult,
+                         const int32_t* map, float* out, hipStream_t st);
+int rmse_blocks(int64_t nnz);
+hipError_t launch_rmse(const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz, const void* P,
+                       const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k, int prec,
+                       float mult, float b, double* partials, double* out, hipStream_t st);
+hipError_t launch_split(const void* aos, int64_t nnz, int32_t* u, int32_t* v, float* r, hipStream_t st);
+hipError_t launch_check_ids(const int32_t* u, const int32_t* v, int64_t nnz, int32_t m, int32_t n,
+                            unsigned long long* bad, hipStream_t st);
+hipError_t launch_remap(int32_t* u, int32_t* v, int64_t nnz, const int32_t* pmap, const int32_t* qmap, int32_t m,
+                        int32_t n, hipStream_t st);
+hipError_t launch_scale(float* r, int64_t nnz, float s, hipStream_t st);
+int moments_blocks(int64_t nnz);
+hipError_t launch_moments(const float* r, int64_t nnz, double* partials, double* out2, hipStream_t st);
+int64_t part_hist_entries(int64_t nnz, int nb);
+hipError_t launch_partition(const int32_t* u, const int32_t* v, const float* r, int64_t nnz, const int32_t* useg,
+                            const int32_t* vseg, int vy, int nb, int32_t* hist, int64_t* tot, int64_t* base,
+                            int64_t* off, int32_t* uo, int32_t* vo, float* ro, hipStream_t st, int64_t* host_tot);
+hipError_t launch_seg_table(int32_t* seg, int64_t rows, const int64_t* bnd, int ns, hipStream_t st);
+hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, hipStream_t st);
+
+}  // namespace cumf

@@ -1,0 +1,804 @@
+// gfx950 kernels of the SGD hot path.
+//
+//  sgd_update_kernel   batch-Hogwild update (replaces
+//                      sgd_k128_kernel_hogwild_warp32_lrate,
+//                      singleGPU/sgd_k128_kernel_hogwild_warp32.h:3-104)
+//  init_factors_kernel random factor init (random_init, singleGPU/sgd.cu:497-514)
+//  fp8_rescale_kernel  between-epoch power-of-two row rescale (FP8 only)
+//  export_kernel       dequantize * mult + inverse permutation gather
+//                      (transform_half, sgd_kernel.cu:29-38; scale_model and
+//                      shuffle_model, sgd.cu:593-635)
+//  rmse_kernel         test RMSE (mf_predict + calc_rmse, test/mf.cpp:3794-3839)
+//  split/remap/scale/moments/partition kernels: on-device preprocessing
+//                      (sgd.cu:78-134, 411-487)
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace cumf {
+
+// ---------------------------------------------------------------------------
+// Batch-Hogwild update.
+//
+// A wave of 64 lanes holds S = 64/G workers ("slots"); the G lanes of a slot
+// share one rating update, lane l of the slot owning the E = K/G consecutive
+// elements [lE, lE+E) of the P and Q rows, so a row is one coalesced
+// K*bytes gather.  Per window, the slot stages its L records (u, v, r) in LDS,
+// then walks them in order with the next record's rows prefetched while the
+// current one is reduced and written back.  When the prefetched row is the
+// row just written, the written value is forwarded from registers, so one
+// slot is exactly a serial SGD over its window sequence.
+//
+// Arithmetic (shared with the CPU oracle): lane partial = p0*q0 then fma over
+// the lane's other elements; butterfly sum over the slot (lane l + lane l^d,
+// d = 1, 2, 4, ...); e = r - dot; p' = fma(lr, fma(e, q, -(lp*p)), p) and
+// q' = fma(lr, fma(e, p, -(lq*q)), q) with the old p; store rounds p', q' to
+// the storage format.
+// ---------------------------------------------------------------------------
+constexpr int kWavesPerBlock = 4;
+constexpr int kMaxBatch = 128;
+
+template <int G>
+__device__ inline float slot_sum(float x)
+{
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+template <int K, int G, int PREC, bool COH>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
+{
+    constexpr int S = 64 / G;
+    constexpr int E = K / G;
+    constexpr int EB = PrecBytes<PREC>::value;
+    constexpr int TB = E * EB;
+    constexpr int NW = (TB + 3) / 4;
+    constexpr bool FP8 = PREC == kE4M3;
+    static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
+    static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
+
+    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
+    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
+    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / G;
+    const int ls = lane % G;
+
+    int64_t blk = blockIdx.x;
+    if (a.num_xcd > 1) blk = (int64_t)(blockIdx.x % a.num_xcd) * (gridDim.x / a.num_xcd) + blockIdx.x / a.num_xcd;
+    const int worker = (int)((blk * kWavesPerBlock + wave) * S + slot);
+
+    int t = 0;
+    while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
+    if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
+    const SgdTask T = a.tasks[t];
+    if (T.nnz <= 0) return;
+    const uint32_t lw = (uint32_t)(worker - T.worker_begin);
+    const Key key = make_key(a.seed, T.problem);
+    const int L = a.batch_len;
+
+    int32_t* su = s_u[wave][slot];
+    int32_t* sv = s_v[wave][slot];
+    float* sr = s_r[wave][slot];
+
+    char* Pb = (char*)T.P + (int64_t)ls * TB;
+    char* Qb = (char*)T.Q + (int64_t)ls * TB;
+    constexpr int64_t kRow = (int64_t)K * EB;
+
+    for (int ep = a.epoch_begin; ep < a.epoch_end; ++ep) {
+        const float lr = T.lr[ep];
+        for (int b = 0; b < T.update_count; ++b) {
+            const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)b, T.nnz);
+            for (int j = ls; j < L; j += G) {
+                int64_t idx = start + j;
+                while (idx >= T.nnz) idx -= T.nnz;
+                su[j] = T.u[idx];
+                sv[j] = T.v[idx];
+                sr[j] = T.r[idx];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+            int uc = su[0], vc = sv[0];
+            float rc = sr[0];
+            Words<NW> pw = load_raw<TB, COH>(Pb + (int64_t)uc * kRow);
+            Words<NW> qw = load_raw<TB, COH>(Qb + (int64_t)vc * kRow);
+            float psc = 1.0f, qsc = 1.0f;
+            if constexpr (FP8) { psc = T.ps[uc]; qsc = T.qs[vc]; }
+
+            for (int i = 0; i < L; ++i) {
+                const bool more = i + 1 < L;
+                int un = 0, vn = 0;
+                float rn = 0.0f, psn = 1.0f, qsn = 1.0f;
+                Words<NW> pnw, qnw;
+                if (more) {
+                    un = su[i + 1];
+                    vn = sv[i + 1];
+                    rn = sr[i + 1];
+                    pnw = load_raw<TB, COH>(Pb + (int64_t)un * kRow);
+                    qnw = load_raw<TB, COH>(Qb + (int64_t)vn * kRow);
+                    if constexpr (FP8) { psn = T.ps[un]; qsn = T.qs[vn]; }
+                }
+
+                float p[E], q[E];
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    p[e] = unpack<PREC>(pw, e);
+                    q[e] = unpack<PREC>(qw, e);
+                    if constexpr (FP8) { p[e] *= psc; q[e] *= qsc; }
+                }
+                float part = p[0] * q[0];
+#pragma unroll
+                for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
+                const float dot = slot_sum<G>(part);
+                const float err = rc - dot;
+
+                Words<NW> po, qo;
+                uint32_t rbits[4] = {0u, 0u, 0u, 0u};
+                const float pinv = FP8 ? 1.0f / psc : 1.0f;
+                const float qinv = FP8 ? 1.0f / qsc : 1.0f;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const float pn = __builtin_fmaf(lr, __builtin_fmaf(err, q[e], -(T.lambda_p * p[e])), p[e]);
+                    const float qn = __builtin_fmaf(lr, __builtin_fmaf(err, p[e], -(T.lambda_q * q[e])), q[e]);
+                    uint32_t pb, qb;
+                    if constexpr (PREC == kF32) {
+                        pb = __float_as_uint(pn);
+                        qb = __float_as_uint(qn);
+                    } else if constexpr (PREC == kF16) {
+                        pb = f32_to_f16(pn);
+                        qb = f32_to_f16(qn);
+                    } else if constexpr (PREC == kBF16) {
+                        pb = f32_to_bf16(pn);
+                        qb = f32_to_bf16(qn);
+                    } else {
+                        const int j = ls * E + e;
+                        if ((e & 3) == 0) {
+                            U4 rr = philox(U4{(uint32_t)(b * L + i), (uint32_t)ep, lw,
+                                              kStreamRound | ((uint32_t)(j >> 2) << 8)},
+                                           key.k0, key.k1);
+                            rbits[0] = rr.x; rbits[1] = rr.y; rbits[2] = rr.z; rbits[3] = rr.w;
+                        }
+                        const uint32_t wp = rbits[(j >> 1) & 1], wq = rbits[2 | ((j >> 1) & 1)];
+                        pb = f32_to_e4m3_sr(pn * pinv, (j & 1) ? (wp >> 16) : wp);
+                        qb = f32_to_e4m3_sr(qn * qinv, (j & 1) ? (wq >> 16) : wq);
+                    }
+                    pack<PREC>(po, e, pb);
+                    pack<PREC>(qo, e, qb);
+                }
+                store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
+                store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
+
+                if (more) {
+                    if (un == uc) { pnw = po; psn = psc; }
+                    if (vn == vc) { qnw = qo; qsn = qsc; }
+                    uc = un; vc = vn; rc = rn;
+                    pw = pnw; qw = qnw;
+                    psc = psn; qsc = qsn;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+template <int K, int G, int PREC, bool COH>
+static hipError_t launch_sgd_t(const SgdLaunch& a, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int K, int G, int PREC>
+static hipError_t launch_sgd_c(const SgdLaunch& a, int blocks, bool coh, hipStream_t s)
+{
+    constexpr int TB = (K / G) * PrecBytes<PREC>::value;
+    if constexpr (TB == 2) {
+        if (coh) return hipErrorInvalidValue;
+        return launch_sgd_t<K, G, PREC, false>(a, blocks, s);
+    } else {
+        return coh ? launch_sgd_t<K, G, PREC, true>(a, blocks, s) : launch_sgd_t<K, G, PREC, false>(a, blocks, s);
+    }
+}
+
+template <int K, int PREC>
+static hipError_t launch_sgd_g(const SgdLaunch& a, int blocks, int G, bool coh, hipStream_t s)
+{
+    constexpr int EB = PrecBytes<PREC>::value;
+    // lane groups whose per-lane chunk is 2..16 bytes (4..16 for FP8)
+    switch (G) {
+#define CUMF_G(GG)                                                                                     \
+    case GG:                                                                                           \
+        if constexpr (K % GG == 0 && (K / GG) * EB <= 16 && (K / GG) * EB >= 2 &&                      \
+                      (PREC != kE4M3 || (K / GG) % 4 == 0))                                            \
+            return launch_sgd_c<K, GG, PREC>(a, blocks, coh, s);                                       \
+        break;
+        CUMF_G(64)
+        CUMF_G(32)
+        CUMF_G(16)
+        CUMF_G(8)
+        CUMF_G(4)
+#undef CUMF_G
+    default:
+        break;
+    }
+    return hipErrorInvalidValue;
+}
+
+bool sgd_shape_supported(int k, int prec, int G)
+{
+    if (G != 64 && G != 32 && G != 16 && G != 8 && G != 4) return false;
+    if (k != 32 && k != 64 && k != 128) return false;
+    if (k % G) return false;
+    int E = k / G;
+    int eb = prec == kF32 ? 4 : (prec == kE4M3 ? 1 : 2);
+    if (E * eb < 2 || E * eb > 16) return false;
+    if (prec == kE4M3 && E % 4) return false;
+    return true;
+}
+
+int sgd_default_group(int k, int prec)
+{
+    // 64 lanes per update at k = 128 for 16/32-bit formats (one 256/512-byte
+    // coalesced row), FP8 needs 4 elements per lane.
+    int G = prec == kE4M3 ? k / 4 : k / 2;
+    if (G > 64) G = 64;
+    return G;
+}
+
+hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, hipStream_t s)
+{
+    switch (k) {
+    case 32:
+        switch (prec) {
+        case kF32: return launch_sgd_g<32, kF32>(a, blocks, G, coh, s);
+        case kF16: return launch_sgd_g<32, kF16>(a, blocks, G, coh, s);
+        case kBF16: return launch_sgd_g<32, kBF16>(a, blocks, G, coh, s);
+        case kE4M3: return launch_sgd_g<32, kE4M3>(a, blocks, G, coh, s);
+        }
+        break;
+    case 64:
+        switch (prec) {
+        case kF32: return launch_sgd_g<64, kF32>(a, blocks, G, coh, s);
+        case kF16: return launch_sgd_g<64, kF16>(a, blocks, G, coh, s);
+        case kBF16: return launch_sgd_g<64, kBF16>(a, blocks, G, coh, s);
+        case kE4M3: return launch_sgd_g<64, kE4M3>(a, blocks, G, coh, s);
+        }
+        break;
+    case 128:
+        switch (prec) {
+        case kF32: return launch_sgd_g<128, kF32>(a, blocks, G, coh, s);
+        case kF16: return launch_sgd_g<128, kF16>(a, blocks, G, coh, s);
+        case kBF16: return launch_sgd_g<128, kBF16>(a, blocks, G, coh, s);
+        case kE4M3: return launch_sgd_g<128, kE4M3>(a, blocks, G, coh, s);
+        }
+        break;
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Factor init: element e of row r is U(0,1] * s from word (e & 3) of
+// Philox(ctr = {e >> 2, r_lo, r_hi, stream}).  One wave per row.  FP8 rows get
+// the power-of-two scale for kFp8InitTarget and RNE quantization.
+// ---------------------------------------------------------------------------
+__device__ inline float wave_max(float x)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) x = fmaxf(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void init_factors_kernel(void* F, float* scales, int64_t rows, int k,
+                                                           float s, Key key, uint32_t stream)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += nwaves) {
+        float vals[4];
+        float amax = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int e = lane + 64 * c;
+            vals[c] = 0.0f;
+            if (e < k) {
+                U4 w = philox(U4{(uint32_t)(e >> 2), (uint32_t)r, (uint32_t)((uint64_t)r >> 32), stream}, key.k0, key.k1);
+                const uint32_t word = (e & 3) == 0 ? w.x : (e & 3) == 1 ? w.y : (e & 3) == 2 ? w.z : w.w;
+                vals[c] = u01_of(word) * s;
+                amax = fmaxf(amax, vals[c]);
+            }
+        }
+        float inv = 1.0f;
+        if constexpr (PREC == kE4M3) {
+            amax = wave_max(amax);
+            const float sc = pow2_scale(amax, kFp8InitTarget);
+            inv = 1.0f / sc;
+            if (lane == 0) scales[r] = sc;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int e = lane + 64 * c;
+            if (e >= k) continue;
+            const int64_t i = r * k + e;
+            if constexpr (PREC == kF32) ((float*)F)[i] = vals[c];
+            else if constexpr (PREC == kF16) ((uint16_t*)F)[i] = (uint16_t)f32_to_f16(vals[c]);
+            else if constexpr (PREC == kBF16) ((uint16_t*)F)[i] = (uint16_t)f32_to_bf16(vals[c]);
+            else ((uint8_t*)F)[i] = (uint8_t)f32_to_e4m3_rne(vals[c] * inv);
+        }
+    }
+}
+
+hipError_t launch_init(void* F, float* scales, int64_t rows, int k, int prec, float s, uint64_t seed,
+                       uint32_t problem, uint32_t stream_tag, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    int64_t blocks64 = (rows + 3) / 4;
+    int blocks = (int)(blocks64 > 65536 ? 65536 : blocks64);
+    Key key = make_key(seed, problem);
+    switch (prec) {
+    case kF32: hipLaunchKernelGGL(init_factors_kernel<kF32>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, s, key, stream_tag); break;
+    case kF16: hipLaunchKernelGGL(init_factors_kernel<kF16>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, s, key, stream_tag); break;
+    case kBF16: hipLaunchKernelGGL(init_factors_kernel<kBF16>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, s, key, stream_tag); break;
+    case kE4M3: hipLaunchKernelGGL(init_factors_kernel<kE4M3>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, s, key, stream_tag); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP8 between-epoch rescale.  One wave per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fp8_rescale_kernel(uint8_t* F, float* scales, int64_t rows, int k)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += nwaves) {
+        const float old = scales[r];
+        float vals[4];
+        float amax = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int e = lane + 64 * c;
+            vals[c] = 0.0f;
+            if (e < k) {
+                vals[c] = e4m3_to_f32(F[r * k + e]) * old;
+                amax = fmaxf(amax, fabsf(vals[c]));
+            }
+        }
+        amax = wave_max(amax);
+        if (!(amax > 0.0f)) continue;
+        const float sc = pow2_scale(amax, kFp8EpochTarget);
+        const float inv = 1.0f / sc;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int e = lane + 64 * c;
+            if (e < k) F[r * k + e] = (uint8_t)f32_to_e4m3_rne(vals[c] * inv);
+        }
+        if (lane == 0) scales[r] = sc;
+    }
+}
+
+hipError_t launch_fp8_rescale(uint8_t* F, float* scales, int64_t rows, int k, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    int64_t blocks64 = (rows + 3) / 4;
+    int blocks = (int)(blocks64 > 65536 ? 65536 : blocks64);
+    hipLaunchKernelGGL(fp8_rescale_kernel, dim3(blocks), dim3(256), 0, st, F, scales, rows, k);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Widening helpers for the export and RMSE kernels.
+// ---------------------------------------------------------------------------
+template <int PREC>
+__device__ inline float widen_at(const void* F, const float* scales, int64_t row, int k, int e)
+{
+    const int64_t i = row * k + e;
+    if constexpr (PREC == kF32) return ((const float*)F)[i];
+    else if constexpr (PREC == kF16) return f16_to_f32(((const uint16_t*)F)[i]);
+    else if constexpr (PREC == kBF16) return bf16_to_f32(((const uint16_t*)F)[i]);
+    else return e4m3_to_f32(((const uint8_t*)F)[i]) * scales[row];
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void export_kernel(const void* F, const float* scales, int64_t rows, int k,
+                                                     float mult, const int32_t* map, float* out)
+{
+    const int64_t total = rows * k;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / k;
+        const int e = (int)(i - r * k);
+        const int64_t src = map ? (int64_t)map[r] : r;
+        out[i] = __fmul_rn(widen_at<PREC>(F, scales, src, k, e), mult);
+    }
+}
+
+hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k, int prec, float mult,
+                         const int32_t* map, float* out, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    int64_t total = rows * k;
+    int blocks = (int)((total + 255) / 256 > 65536 ? 65536 : (total + 255) / 256);
+    switch (prec) {
+    case kF32: hipLaunchKernelGGL(export_kernel<kF32>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, mult, map, out); break;
+    case kF16: hipLaunchKernelGGL(export_kernel<kF16>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, mult, map, out); break;
+    case kBF16: hipLaunchKernelGGL(export_kernel<kBF16>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, mult, map, out); break;
+    case kE4M3: hipLaunchKernelGGL(export_kernel<kE4M3>, dim3(blocks), dim3(256), 0, st, F, scales, rows, k, mult, map, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Test RMSE: one thread per test rating, sequential FP32 dot of the exported
+// values with separate multiply and add, as std::inner_product in
+// test/mf.cpp:3803; FP64 sums, reduced per block and then in block order.
+// ---------------------------------------------------------------------------
+constexpr int kRmseThreads = 256;
+
+template <int PREC>
+__global__ __launch_bounds__(kRmseThreads) void rmse_kernel(const int32_t* tu, const int32_t* tv, const float* tr,
+                                                            int64_t nnz, const void* P, const float* ps,
+                                                            const void* Q, const float* qs, int32_t m, int32_t n,
+                                                            int k, float mult, float b, double* partials)
+{
+    __shared__ double red[kRmseThreads];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = tu[i], v = tv[i];
+        float z;
+        if (u < 0 || u >= m || v < 0 || v >= n) {
+            z = b;
+        } else {
+            z = 0.0f;
+            for (int e = 0; e < k; ++e) {
+                const float pe = __fmul_rn(widen_at<PREC>(P, ps, u, k, e), mult);
+                const float qe = __fmul_rn(widen_at<PREC>(Q, qs, v, k, e), mult);
+                z = __fadd_rn(z, __fmul_rn(pe, qe));
+            }
+            if (__builtin_isnan(z)) z = b;
+        }
+        const float err = __fsub_rn(tr[i], z);
+        acc += (double)__fmul_rn(err, err);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kRmseThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// Sum n doubles in a fixed order with one block (tree over a strided split).
+__global__ __launch_bounds__(256) void sum_doubles_kernel(const double* in, int64_t n, double* out)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += in[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+int rmse_blocks(int64_t nnz)
+{
+    int64_t b = (nnz + kRmseThreads - 1) / kRmseThreads;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+hipError_t launch_rmse(const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz, const void* P,
+                       const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k, int prec,
+                       float mult, float b, double* partials, double* out, hipStream_t st)
+{
+    const int blocks = rmse_blocks(nnz);
+    switch (prec) {
+    case kF32: hipLaunchKernelGGL(rmse_kernel<kF32>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
+    case kF16: hipLaunchKernelGGL(rmse_kernel<kF16>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
+    case kBF16: hipLaunchKernelGGL(rmse_kernel<kBF16>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
+    case kE4M3: hipLaunchKernelGGL(rmse_kernel<kE4M3>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sum_doubles_kernel, dim3(1), dim3(256), 0, st, partials, (int64_t)blocks, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Preprocessing kernels.
+// ---------------------------------------------------------------------------
+static int grid_for(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// AoS 12-byte records -> three arrays.
+__global__ void split_kernel(const int32_t* aos, int64_t nnz, int32_t* u, int32_t* v, float* r)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        u[i] = aos[3 * i];
+        v[i] = aos[3 * i + 1];
+        r[i] = __int_as_float(aos[3 * i + 2]);
+    }
+}
+
+hipError_t launch_split(const void* aos, int64_t nnz, int32_t* u, int32_t* v, float* r, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(split_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, (const int32_t*)aos, nnz, u, v, r);
+    return hipGetLastError();
+}
+
+// Range check of ids: counts records with u outside [0,m) or v outside [0,n).
+__global__ void check_ids_kernel(const int32_t* u, const int32_t* v, int64_t nnz, int32_t m, int32_t n,
+                                 unsigned long long* bad)
+{
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        c += (u[i] < 0 || u[i] >= m || v[i] < 0 || v[i] >= n) ? 1ull : 0ull;
+    if (c) atomicAdd(bad, c);
+}
+
+hipError_t launch_check_ids(const int32_t* u, const int32_t* v, int64_t nnz, int32_t m, int32_t n,
+                            unsigned long long* bad, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(check_ids_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, u, v, nnz, m, n, bad);
+    return hipGetLastError();
+}
+
+// u <- pmap[u], v <- qmap[v]; ids outside [0,m) x [0,n) are left alone.
+__global__ void remap_kernel(int32_t* u, int32_t* v, int64_t nnz, const int32_t* pmap, const int32_t* qmap,
+                             int32_t m, int32_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = u[i], c = v[i];
+        if (a >= 0 && a < m) u[i] = pmap[a];
+        if (c >= 0 && c < n) v[i] = qmap[c];
+    }
+}
+
+hipError_t launch_remap(int32_t* u, int32_t* v, int64_t nnz, const int32_t* pmap, const int32_t* qmap, int32_t m,
+                        int32_t n, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(remap_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, u, v, nnz, pmap, qmap, m, n);
+    return hipGetLastError();
+}
+
+__global__ void scale_kernel(float* r, int64_t nnz, float s)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = __fmul_rn(r[i], s);
+}
+
+hipError_t launch_scale(float* r, int64_t nnz, float s, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, r, nnz, s);
+    return hipGetLastError();
+}
+
+// Sum r and r*r in double: per-block partials (fixed order), then one block.
+__global__ __launch_bounds__(256) void moments_kernel(const float* r, int64_t nnz, double* partials)
+{
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x = (double)r[i];
+        a += x;
+        b += x * x;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1[0];
+        partials[gridDim.x + blockIdx.x] = s2[0];
+    }
+}
+
+int moments_blocks(int64_t nnz) { return grid_for(nnz, 256) > 2048 ? 2048 : grid_for(nnz, 256); }
+
+hipError_t launch_moments(const float* r, int64_t nnz, double* partials, double* out2, hipStream_t st)
+{
+    const int blocks = moments_blocks(nnz);
+    hipLaunchKernelGGL(moments_kernel, dim3(blocks), dim3(256), 0, st, r, nnz, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sum_doubles_kernel, dim3(1), dim3(256), 0, st, partials, (int64_t)blocks, out2);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sum_doubles_kernel, dim3(1), dim3(256), 0, st, partials + blocks, (int64_t)blocks, out2 + 1);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting-sort partition by block id useg[u]*vy + vseg[v].
+// A wave owns a contiguous chunk and walks it 64 records at a time; a
+// record's rank among equal ids inside its group of 64 comes from a readlane
+// scan, so the order inside every block is the input order.
+//   pass 1: per-chunk histogram hist[chunk*nb + b]
+//   pass 2: per-block column scan -> write offsets off[chunk*nb + b]
+//   pass 3: scatter
+// ---------------------------------------------------------------------------
+__device__ inline int block_of(int32_t u, int32_t v, const int32_t* useg, const int32_t* vseg, int vy)
+{
+    return useg[u] * vy + vseg[v];
+}
+
+__global__ __launch_bounds__(64) void part_count_kernel(const int32_t* u, const int32_t* v, int64_t nnz, int64_t chunk,
+                                                        const int32_t* useg, const int32_t* vseg, int vy, int nb,
+                                                        int32_t* hist)
+{
+    extern __shared__ __align__(16) char dyn_smem[];
+    int32_t* cnt = (int32_t*)dyn_smem;
+    for (int b = threadIdx.x; b < nb; b += 64) cnt[b] = 0;
+    __syncthreads();
+    const int64_t c = blockIdx.x;
+    const int64_t beg = c * chunk, end = beg + chunk < nnz ? beg + chunk : nnz;
+    for (int64_t i = beg + threadIdx.x; i < end; i += 64) atomicAdd(&cnt[block_of(u[i], v[i], useg, vseg, vy)], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 64) hist[c * nb + b] = cnt[b];
+}
+
+// One thread per block id: exclusive scan of its column over chunks, plus the
+// block base (sum of all earlier blocks' totals, given in base[]).
+__global__ void part_scan_kernel(const int32_t* hist, int64_t nchunks, int nb, const int64_t* base, int64_t* off)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int64_t run = base[b];
+    for (int64_t c = 0; c < nchunks; ++c) {
+        off[c * nb + b] = run;
+        run += hist[c * nb + b];
+    }
+}
+
+// Column totals: tot[b] = sum over chunks of hist[c*nb+b].
+__global__ void part_total_kernel(const int32_t* hist, int64_t nchunks, int nb, int64_t* tot)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int64_t s = 0;
+    for (int64_t c = 0; c < nchunks; ++c) s += hist[c * nb + b];
+    tot[b] = s;
+}
+
+__global__ __launch_bounds__(64) void part_scatter_kernel(const int32_t* u, const int32_t* v, const float* r,
+                                                          int64_t nnz, int64_t chunk, const int32_t* useg,
+                                                          const int32_t* vseg, int vy, int nb, const int64_t* off,
+                                                          int32_t* uo, int32_t* vo, float* ro)
+{
+    extern __shared__ __align__(16) char dyn_smem[];
+    int64_t* pos = (int64_t*)dyn_smem;
+    const int64_t c = blockIdx.x;
+    for (int b = threadIdx.x; b < nb; b += 64) pos[b] = off[c * nb + b];
+    __syncthreads();
+    const int64_t beg = c * chunk, end = beg + chunk < nnz ? beg + chunk : nnz;
+    const int lane = threadIdx.x;
+    for (int64_t g = beg; g < end; g += 64) {
+        const int64_t i = g + lane;
+        const bool valid = i < end;
+        const int bid = valid ? block_of(u[i], v[i], useg, vseg, vy) : -1;
+        int rank = 0;
+        const int cnt = (int)(end - g < 64 ? end - g : 64);
+        for (int j = 0; j < cnt; ++j) {
+            const int bj = __builtin_amdgcn_readlane(bid, j);
+            rank += (j < lane && bj == bid) ? 1 : 0;
+        }
+        int64_t dst = 0;
+        if (valid) dst = pos[bid] + rank;
+        __syncthreads();
+        if (valid) {
+            uo[dst] = u[i];
+            vo[dst] = v[i];
+            ro[dst] = r[i];
+            atomicAdd((unsigned long long*)&pos[bid], 1ull);
+        }
+        __syncthreads();
+    }
+}
+
+int64_t part_chunk(int64_t nnz)
+{
+    // about 2048 chunks, at least 4096 records each
+    int64_t c = (nnz + 2047) / 2048;
+    if (c < 4096) c = 4096;
+    return c;
+}
+
+hipError_t launch_partition(const int32_t* u, const int32_t* v, const float* r, int64_t nnz, const int32_t* useg,
+                            const int32_t* vseg, int vy, int nb, int32_t* hist, int64_t* tot, int64_t* base,
+                            int64_t* off, int32_t* uo, int32_t* vo, float* ro, hipStream_t st,
+                            int64_t* host_tot)
+{
+    const int64_t chunk = part_chunk(nnz);
+    const int64_t nchunks = (nnz + chunk - 1) / chunk;
+    hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)nchunks), dim3(64), nb * sizeof(int32_t), st, u, v, nnz,
+                       chunk, useg, vseg, vy, nb, hist);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(part_total_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, hist, nchunks, nb, tot);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(host_tot, tot, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    // block bases on the host (nb is small)
+    int64_t* hb = host_tot + nb;
+    hb[0] = 0;
+    for (int b = 1; b < nb; ++b) hb[b] = hb[b - 1] + host_tot[b - 1];
+    if ((e = hipMemcpyAsync(base, hb, sizeof(int64_t) * nb, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(part_scan_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, hist, nchunks, nb, base, off);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)nchunks), dim3(64), nb * sizeof(int64_t), st, u, v, r, nnz,
+                       chunk, useg, vseg, vy, nb, off, uo, vo, ro);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+int64_t part_hist_entries(int64_t nnz, int nb)
+{
+    const int64_t chunk = part_chunk(nnz);
+    return ((nnz + chunk - 1) / chunk) * (int64_t)nb;
+}
+
+// Segment lookup tables: seg[i] for i < rows given ascending boundaries
+// bnd[0..ns] (bnd[0] = 0, bnd[ns] = rows).
+__global__ void seg_table_kernel(int32_t* seg, int64_t rows, const int64_t* bnd, int ns)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bnd[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        seg[i] = lo;
+    }
+}
+
+hipError_t launch_seg_table(int32_t* seg, int64_t rows, const int64_t* bnd, int ns, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_table_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, st, seg, rows, bnd, ns);
+    return hipGetLastError();
+}
+
+// Row/column degree histograms (for nnz-balanced stratum boundaries).
+__global__ void degree_kernel(const int32_t* ids, int64_t nnz, int32_t* deg)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&deg[ids[i]], 1);
+}
+
+hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(degree_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, ids, nnz, deg);
+    return hipGetLastError();
+}
+
+}  // namespace cumf

@@ -1,0 +1,223 @@
+/*
+ * cumf_sgd_amd: MI355X (gfx950) matrix-factorization SGD, C ABI.
+ *
+ * This header is the drop-in boundary.  It replaces the reference's host/GPU
+ * split, `void sgd_update_k128(Parameter, mf_model*, mf_problem*, float)`
+ * (singleGPU/sgd_kernel.h:17, called once from singleGPU/sgd.cu:682), and the
+ * library-level pipeline `sgd_train` (singleGPU/sgd.h:141) around it.
+ *
+ * Conventions (all entry points):
+ *  - every function returns CUMF_OK (0) or a positive CUMF_ERR_* code and
+ *    never calls exit(); cumf_last_error() returns the message of the last
+ *    failure on the calling thread;
+ *  - host memory belongs to the caller; device memory belongs to the context;
+ *  - pointers passed as "data" may be host or device pointers (the copy uses
+ *    unified addressing);
+ *  - one context per host thread; calls on one context are not reentrant.
+ */
+#ifndef CUMF_SGD_AMD_H
+#define CUMF_SGD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUMF_ABI_VERSION 1
+
+enum cumf_status {
+    CUMF_OK = 0,
+    CUMF_ERR_ARG = 1,     /* invalid argument */
+    CUMF_ERR_HIP = 2,     /* HIP runtime error */
+    CUMF_ERR_STATE = 3,   /* call out of order (e.g. train before ratings) */
+    CUMF_ERR_NOMEM = 4,   /* allocation failed */
+    CUMF_ERR_IO = 5       /* file I/O failed */
+};
+
+/* Factor storage format.  Arithmetic is always FP32. */
+enum cumf_precision {
+    CUMF_FP32 = 0,
+    CUMF_FP16 = 1,        /* the reference's native format (singleGPU/sgd.h:124-133) */
+    CUMF_BF16 = 2,
+    CUMF_FP8_E4M3 = 3     /* OCP E4M3 + one power-of-two FP32 scale per row */
+};
+
+/* Which rating blocks run in one launch. */
+enum cumf_schedule {
+    CUMF_SCHED_HOGWILD = 0,  /* all workers on the whole matrix (reference 1x1 mode) */
+    CUMF_SCHED_SERIAL = 1,   /* one worker, deterministic window order */
+    CUMF_SCHED_GRID = 2,     /* blocks one after another, all workers each (u x v / x x y) */
+    CUMF_SCHED_STRATUM = 3   /* s x s blocks, a conflict-free diagonal per launch */
+};
+
+/* One rating: mf_node, singleGPU/sgd.h:79-83 (12 bytes). */
+typedef struct cumf_node {
+    int32_t u;
+    int32_t v;
+    float rate;
+} cumf_node;
+
+/* Training options.  Field meanings follow Parameter (singleGPU/sgd.h:26-46). */
+typedef struct cumf_train_opts {
+    int32_t k;                 /* factor length: 32, 64 or 128 */
+    int32_t num_workers;       /* Hogwild workers (-s); 0 = autotuned */
+    int32_t batch_len;         /* records per window, update_vector_size (sgd_kernel.cu:198); 0 = 128 */
+    int32_t schedule;          /* enum cumf_schedule */
+    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = default for (k, precision) */
+    int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
+    float lambda_p;            /* already divided by the rating scale (sgd.cu:673) */
+    float lambda_q;
+    float alpha;               /* lr_t = alpha/(1+beta*t^1.5) + lrate (sgd_kernel.cu:144-149) */
+    float beta;
+    float lrate;
+    uint64_t seed;             /* Philox key for windows and rounding */
+} cumf_train_opts;
+
+typedef struct cumf_ctx cumf_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int cumf_abi_version(void);
+const char* cumf_last_error(void);
+int cumf_create(int device, cumf_ctx** out);
+int cumf_destroy(cumf_ctx* ctx);
+int cumf_sync(cumf_ctx* ctx);
+
+/* ---- ratings ------------------------------------------------------------ */
+/* Training ratings as 12-byte records (the reference's R array,
+ * singleGPU/sgd.cu:239-265) or as three arrays.  m and n are the factor row
+ * counts; every id must lie in [0, m) x [0, n). */
+int cumf_set_ratings(cumf_ctx* ctx, const cumf_node* R, int64_t nnz, int32_t m, int32_t n);
+int cumf_set_ratings_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, const float* r,
+                         int64_t nnz, int32_t m, int32_t n);
+/* Test ratings for cumf_test_rmse (ids outside [0,m) x [0,n) predict b,
+ * test/mf.cpp:3796-3797). */
+int cumf_set_test(cumf_ctx* ctx, const cumf_node* R, int64_t nnz);
+int cumf_set_test_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, const float* r, int64_t nnz);
+
+/* Device copies of the ratings (3 arrays of nnz), e.g. for a checker. */
+int cumf_get_ratings(cumf_ctx* ctx, int32_t* u, int32_t* v, float* r);
+
+/* ---- on-device preprocessing (singleGPU/sgd.cu:78-134) ------------------ */
+/* sum r and sum r^2 in double (collect_data, sgd.cu:78-96). */
+int cumf_rating_moments(cumf_ctx* ctx, double* sum, double* sum_sq);
+/* u <- p_map[u], v <- q_map[v] on training and test ratings
+ * (shuffle_problem, sgd.cu:126-134).  Test ids out of range are kept. */
+int cumf_remap(cumf_ctx* ctx, const int32_t* p_map, const int32_t* q_map);
+/* r <- r * scale on training ratings (scale_problem, sgd.cu:98-107). */
+int cumf_scale_ratings(cumf_ctx* ctx, float scale);
+/* Partition the training ratings into ux x vy blocks of the reference's
+ * shape, block (u/u_seg)*vy + v/v_seg (grid_problem, sgd.cu:411-487), with a
+ * stable counting sort.  offsets (ux*vy+1 entries) may be NULL. */
+int cumf_partition(cumf_ctx* ctx, int32_t ux, int32_t vy, int64_t* offsets);
+/* Partition into s x s blocks whose row and column ranges hold about equal
+ * rating counts (for CUMF_SCHED_STRATUM), stable like cumf_partition. */
+int cumf_partition_strata(cumf_ctx* ctx, int32_t s, int64_t* offsets);
+/* Declare that the uploaded ratings are already grouped into ux x vy blocks
+ * with the given prefix offsets (as the host cycle-swap leaves them). */
+int cumf_set_blocks(cumf_ctx* ctx, int32_t ux, int32_t vy, const int64_t* offsets);
+
+/* ---- factors ------------------------------------------------------------ */
+/* Allocate P (m x k) and Q (n x k) and fill every element with
+ * U(0,1] * sqrt(1/k) (random_init, sgd.cu:497-546) in the given format. */
+int cumf_init_factors(cumf_ctx* ctx, int32_t k, int32_t precision, uint64_t seed);
+/* Raw storage in/out: which = 0 for P, 1 for Q.  data holds rows*k elements
+ * of the storage format; scales (FP8 only, else NULL) holds rows floats. */
+int cumf_set_factors(cumf_ctx* ctx, int32_t which, const void* data, const float* scales);
+int cumf_get_factors(cumf_ctx* ctx, int32_t which, void* data, float* scales);
+/* FP32 export: out[i] = widen(F[map ? map[i] : i]) * mult for i < rows
+ * (transform_half + scale_model + shuffle_model, sgd_kernel.cu:29-68,
+ * sgd.cu:593-635).  out holds rows*k floats. */
+int cumf_export_factors(cumf_ctx* ctx, int32_t which, float mult, const int32_t* map, float* out);
+
+/* ---- training ----------------------------------------------------------- */
+/* Run epochs [epoch_begin, epoch_end) under opts->schedule.  For
+ * CUMF_SCHED_GRID, block_order lists the blocks of each epoch in the order
+ * they run (order_len entries, reused every epoch when order_len equals the
+ * block count, or epoch-major when it is (epoch_end-epoch_begin) times that);
+ * NULL runs blocks in index order. */
+int cumf_train(cumf_ctx* ctx, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end,
+               const int32_t* block_order, int64_t order_len);
+/* Test RMSE of the current factors: predictions use widen(F) * mult (the
+ * exported model), then test/mf.cpp:3794-3839 exactly: sequential FP32 dot,
+ * b for out-of-range ids or nan, FP64 sum of FP32 squared errors. */
+int cumf_test_rmse(cumf_ctx* ctx, float mult, float b, double* rmse);
+
+/* Time spent in training kernels since the last reset (HIP events on the
+ * context's stream), the number of training launches, and the number of
+ * rating updates they performed. */
+int cumf_kernel_time(cumf_ctx* ctx, double* ms, int64_t* launches, int64_t* updates);
+int cumf_reset_kernel_time(cumf_ctx* ctx);
+
+/* Resolved launch shape of the last cumf_train call. */
+typedef struct cumf_launch_info {
+    int32_t num_workers;
+    int32_t lanes_per_sample;
+    int32_t waves_per_block;
+    int32_t blocks;
+    int32_t batch_len;
+    int64_t updates_per_epoch;
+} cumf_launch_info;
+int cumf_last_launch(cumf_ctx* ctx, cumf_launch_info* info);
+
+/* ---- reference-seam entry points ---------------------------------------- */
+/* Mirror of Parameter (singleGPU/sgd.h:26-46). */
+typedef struct cumf_param {
+    int32_t k, num_workers, u_grid, v_grid, x_grid, y_grid, ux, vy, num_iters, gpu;
+    float lambda_p, lambda_q, lrate, alpha, beta;
+} cumf_param;
+
+/* Drop-in for sgd_update_k128 (singleGPU/sgd_kernel.h:17,
+ * singleGPU/sgd_kernel.cu:125-720): the caller passes the grid-ordered,
+ * scaled, id-localised ratings (R2D[ux*vy] block pointers with gridSize
+ * counts), the FP16 factor bits halfp (ux*u_seg*k) and halfq (vy*v_seg*k),
+ * updated in place, and receives FP32 copies in floatp/floatq.  Schedule:
+ * 1x1 -> Hogwild on the whole matrix; otherwise blocks in the reference's
+ * order, driven by the C library rand() exactly as the reference does
+ * (sgd_kernel.cu:253-267, 399-451).  seed is the Philox key. */
+int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u_seg, int64_t v_seg,
+                         const cumf_node* const* R2D, const int64_t* grid_size,
+                         uint16_t* halfp, uint16_t* halfq, float* floatp, float* floatq,
+                         uint64_t seed);
+
+/* Drop-in for the sgd_train pipeline (singleGPU/sgd.cu:638-690) plus
+ * save-ready output: statistics, id permutation (srand(seed) +
+ * std::random_shuffle, sgd.cu:39-60), grid, scaling, init, training, rescale
+ * and inverse permutation.  R is read only.  opts (may be NULL) overrides
+ * the schedule, lanes_per_sample, coherent and batch_len; its k, lambdas,
+ * alpha, beta, lrate and seed are taken from para and seed.  With a 1x1 grid
+ * everything runs on the device; with more blocks the id shuffle and the
+ * cycle-swap partition run on the host first so that block contents and
+ * order are the reference's.  P_out (m*k) and Q_out (n*k) receive the final model in original id order,
+ * *b_out the model's b (ave/std_dev, sgd.cu:679).  precision selects the
+ * factor format (CUMF_FP16 is the reference's).  test (may be NULL, test_nnz
+ * records, original ids) is scored after every epoch into rmse_out[epoch]
+ * (may be NULL). */
+int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* opts, const cumf_node* R,
+                   int64_t nnz, int32_t m, int32_t n, int32_t precision, uint64_t seed,
+                   const cumf_node* test, int64_t test_nnz, float* P_out, float* Q_out,
+                   float* b_out, double* rmse_out);
+
+/* ---- host helpers (the reference's host pipeline, bit-exact) ------------- */
+/* gen_random_map (singleGPU/sgd.cu:39-53) with srand(seed) in place of
+ * srand(time(NULL)). */
+int cumf_gen_random_map(uint32_t seed, int32_t size, int32_t* map);
+/* collect_data (singleGPU/sgd.cu:78-96): serial double sums. */
+int cumf_collect_data(const cumf_node* R, int64_t nnz, float* ave, float* std_dev);
+/* grid_problem (singleGPU/sgd.cu:411-487): in-place cycle-swap partition. */
+int cumf_grid_problem(cumf_node* R, int64_t nnz, int32_t m, int32_t n, int32_t ux, int32_t vy,
+                      int64_t* grid_size, int64_t* offsets, int64_t* u_seg, int64_t* v_seg);
+/* read_problem (singleGPU/sgd.cu:190-335): one file, or shards path0..path79
+ * read in parallel.  *R_out is malloc'd (free with cumf_free_host). */
+int cumf_read_problem(const char* path, cumf_node** R_out, int64_t* nnz, int32_t* m, int32_t* n);
+void cumf_free_host(void* p);
+/* save_model (singleGPU/main.cpp:231-273): 20-byte header + P rows + Q rows. */
+int cumf_save_model(const char* path, int32_t m, int32_t n, int32_t k, float b,
+                    const float* P, const float* Q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUMF_SGD_AMD_H */
