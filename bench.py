@@ -1,0 +1,233 @@
+"""Benchmark: rating-updates/s at k=128 and time-to-target test RMSE.
+
+Workload (one GPU): the medium Netflix-shaped problem, 480,189 users x
+17,770 items, 99,072,112 training + 1,408,395 test ratings (integers 1..5),
+k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated on the device.  One
+step = one epoch of batch-Hogwild SGD over all training ratings (the
+reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244).
+Inputs are resident in HBM before the timed region starts.
+
+Printed: one JSON line with value (rating updates/s of the timed epochs),
+roofline of the update kernel (algorithmic bytes per launch over its
+HIP-event duration), time-to-target RMSE for each factor format against the
+FP32 GPU run, and the reference's CPU trainer (test/mf-train, built from the
+reference sources into oracle/_ref) timed on a bounded sample on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+K = 128
+# algorithmic bytes per update at k=128: 12-byte rating + P and Q rows read and
+# written (+ the two FP32 row scales read, FP8)
+BYTES_PER_UPDATE = {0: 12 + 4 * K * 4, 1: 12 + 4 * K * 2, 2: 12 + 4 * K * 2, 3: 12 + 4 * K * 1 + 2 * 4}
+PREC_NAME = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8_e4m3"}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def prepare(ctx, R, seed: int):
+    """Device preprocessing of the reference pipeline (singleGPU/sgd.cu:644-679):
+    statistics, id permutation, rating scaling.  Returns (scale, b, lambda)."""
+    import cumf_sgd_amd as cs
+
+    ctx.set_ratings(R.u, R.v, R.r, R.m, R.n)
+    ctx.set_test(R.tu, R.tv, R.tr)
+    s1, s2 = ctx.rating_moments()
+    ex, ex2 = s1 / R.nnz, s2 / R.nnz
+    ave = np.float32(ex)
+    std = np.float32(np.sqrt(ex2 - ex * ex))
+    scale = max(np.float32(1e-4), std)
+    ctx.remap(cs.gen_random_map(seed, R.m), cs.gen_random_map(seed, R.n))
+    ctx.scale_ratings(np.float32(1.0 / scale))
+    return scale, np.float32(ave / std), np.float32(np.float32(0.05) / scale)
+
+
+def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, rmse_every=True):
+    """Fresh factors, `epochs` epochs; returns (per-epoch train seconds, rmse list)."""
+    import cumf_sgd_amd as cs
+
+    ctx.init_factors(K, prec, seed)
+    opts = cs.make_opts(k=K, schedule=schedule, num_workers=workers, lambda_p=float(lam), seed=seed,
+                        lanes_per_sample=lanes)
+    fs = np.float32(np.sqrt(scale))
+    times, rmses = [], []
+    for ep in range(epochs):
+        t0 = time.perf_counter()
+        ctx.train(opts, ep, ep + 1)
+        times.append(time.perf_counter() - t0)
+        if rmse_every:
+            rmses.append(ctx.test_rmse(fs, b))
+    return times, rmses
+
+
+def cpu_baseline(R, threads: int, sample: int, iters: int):
+    """The reference's CPU trainer on a bounded sample of the training set."""
+    import oracle
+
+    if not os.access(oracle.MF_TRAIN, os.X_OK):
+        return None
+    from cumf_sgd_amd import datagen
+
+    u, v, r = (t[:sample].cpu().numpy() for t in (R.u, R.v, R.r))
+    with tempfile.TemporaryDirectory() as d:
+        tr = os.path.join(d, "train.bin")
+        datagen.write_triplets(tr, u, v, r)
+        cmd = [oracle.MF_TRAIN, "--quiet", "-l2", "0.05", "-k", str(K), "-t", str(iters), "-s", str(threads),
+               "-n", str(2 * threads + 1), "-r", "0.08", tr, os.path.join(d, "model")]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    for line in out.stdout.splitlines():
+        if line.startswith("perf:"):
+            return {"value": float(line.split()[1]), "unit": "rating-updates/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"test/mf-train (LIBMF FPSGD, SSE+OpenMP) on the first {sample} training ratings "
+                              f"of the same workload, {iters} iterations, k={K}, -s {threads}; its optimizer is "
+                              f"AdaGrad-style FPSGD, not the GPU kernel's plain SGD"}
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "fp8"])
+    ap.add_argument("--workers", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--schedule", default="hogwild", choices=["hogwild", "stratum"])
+    ap.add_argument("--rmse-epochs", type=int, default=20)
+    ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design (see DESIGN.md), "
+                                   "run with --gpus 1"}))
+        return 2
+
+    import torch
+
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}))
+        return 2
+    prec = cs.PRECISIONS[args.precision]
+    sched = cs.SCHEDULES[args.schedule]
+    t0 = time.perf_counter()
+    R = datagen.medium(seed=args.seed, device="cuda")
+    torch.cuda.synchronize()
+    log(f"generated {R.nnz} train + {R.tu.numel()} test ratings in {time.perf_counter() - t0:.1f}s")
+
+    ctx = cs.Context(0)
+    scale, b, lam = prepare(ctx, R, args.seed)
+    if sched == cs.STRATUM:
+        ctx.partition_strata(8)
+    torch.cuda.synchronize()
+    log(f"preprocessed: scale={scale:.5f} b={b:.5f}")
+
+    # ---- throughput: warmup + K timed epochs (one step = one epoch)
+    ctx.init_factors(K, prec, args.seed)
+    opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
+                        lanes_per_sample=args.lanes)
+    for ep in range(args.warmup):
+        ctx.train(opts, ep, ep + 1)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ctx.reset_kernel_time()
+    t0 = time.perf_counter()
+    for ep in range(args.warmup, args.warmup + args.steps):
+        ctx.train(opts, ep, ep + 1)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = ctx.kernel_time()
+    info = ctx.last_launch()
+    value = R.nnz * args.steps / wall
+    avg_launch_s = kt.ms / 1e3 / kt.launches
+    upd_per_launch = kt.updates / kt.launches
+    achieved = BYTES_PER_UPDATE[prec] * upd_per_launch / avg_launch_s / 1e9
+    log(f"{args.steps} epochs in {wall:.3f}s -> {value / 1e9:.3f}e9 updates/s; kernel avg {avg_launch_s * 1e3:.3f} ms "
+        f"per launch ({kt.launches} launches), {achieved:.0f} GB/s algorithmic; workers={info.num_workers} "
+        f"lanes={info.lanes_per_sample}")
+
+    # ---- time to target RMSE (evaluation excluded from the time)
+    ttt = None
+    if not args.no_rmse:
+        ttt = {}
+        curves = {}
+        for p in (0, 1, 2, 3):
+            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, args.workers, args.lanes,
+                                      sched)
+            curves[p] = (times, rmses)
+            log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
+        target = curves[0][1][-1] + 0.005
+        for p, (times, rmses) in curves.items():
+            hit = next((i for i, x in enumerate(rmses) if x <= target), None)
+            ttt[PREC_NAME[p]] = {
+                "final_rmse": round(rmses[-1], 5),
+                "epochs_to_target": None if hit is None else hit + 1,
+                "seconds_to_target": None if hit is None else round(float(sum(times[:hit + 1])), 4),
+                "train_seconds_total": round(float(sum(times)), 4),
+            }
+        ttt["target_rmse"] = round(target, 5)
+        ttt["target_rule"] = f"FP32 GPU test RMSE after {args.rmse_epochs} epochs + 0.005"
+
+    # ---- CPU baseline (rank 0, N=1 only)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cpu = None
+    try:
+        cpu = cpu_baseline(R, threads, args.cpu_sample, args.cpu_iters)
+    except Exception as e:  # the baseline never decides the GPU result
+        log(f"cpu baseline failed: {e}")
+
+    line = {
+        "metric": "rating-updates/s at k=128 + time-to-target test RMSE, BF16/FP8 vs FP32 on GPU",
+        "value": value,
+        "unit": "rating-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3"}[prec] + " storage / f32 math",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": "medium Netflix-shaped MF: 480189 x 17770, 99072112 train + 1408395 test, k=128, "
+                               "lambda 0.05, alpha 0.08, beta 0.3, batch-Hogwild",
+                   "precision": args.precision, "schedule": args.schedule, "workers": info.num_workers,
+                   "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
+                   "updates_per_epoch": info.updates_per_epoch},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upd_per_launch},
+        "time_to_target": ttt,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -44,16 +44,15 @@ def build_library() -> str:
     os.makedirs(BUILD, exist_ok=True)
     csrc = os.path.join(PKG, "csrc")
     host = os.path.join(PKG, "host")
-    jobs = [
-        [HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, "sgd_kernels.hip"), "-o", os.path.join(BUILD, "sgd_kernels.o")],
-        [HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, "capi.hip"), "-o", os.path.join(BUILD, "capi.o")],
-        ["g++", *HOST_FLAGS, "-c", os.path.join(host, "host.cpp"), "-o", os.path.join(BUILD, "host.o")],
-    ]
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    hip_units = ["sgd_kernels", "sgd_update_f32", "sgd_update_f16", "sgd_update_bf16", "sgd_update_e4m3", "capi"]
+    jobs = [[HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, f"{u}.hip"), "-o", os.path.join(BUILD, f"{u}.o")]
+            for u in hip_units]
+    jobs.append(["g++", *HOST_FLAGS, "-c", os.path.join(host, "host.cpp"), "-o", os.path.join(BUILD, "host.o")])
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-          os.path.join(BUILD, "sgd_kernels.o"), os.path.join(BUILD, "capi.o"), os.path.join(BUILD, "host.o")])
+          *[os.path.join(BUILD, f"{u}.o") for u in hip_units], os.path.join(BUILD, "host.o")])
     os.replace(tmp, LIB)
     _run(["g++", *HOST_FLAGS, "-o", CLI, os.path.join(host, "main.cpp"), f"-L{PKG}", "-lcumf_sgd_amd",
           "-Wl,-rpath,$ORIGIN"])

@@ -241,7 +241,9 @@ int cumf_create(int device, cumf_ctx** out)
     HIP_TRY(hipSetDevice(device));
     cumf_ctx* c = new cumf_ctx;
     c->device = device;
-    hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // a blocking stream: work the caller queued on the null stream (e.g. a
+    // framework producing the ratings on the device) completes before ours
+    hipError_t se = hipStreamCreate(&c->stream);
     if (se != hipSuccess) {
         delete c;
         return fail(CUMF_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(se));
@@ -671,7 +673,7 @@ int cumf_last_launch(cumf_ctx* c, cumf_launch_info* info)
 namespace {
 
 struct Shape {
-    int G, S, L, W, blocks;
+    int G, S, L, W, blocks, D;
 };
 
 int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
@@ -684,14 +686,27 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     sh->S = 64 / sh->G;
     sh->L = o->batch_len ? o->batch_len : 128;
     if (sh->L < 1 || sh->L > kSgdMaxBatch) return fail(CUMF_ERR_ARG, "batch_len=%d out of range 1..%d", sh->L, kSgdMaxBatch);
+    if (sh->L % 2) return fail(CUMF_ERR_ARG, "batch_len=%d must be even", sh->L);
+    int D = o->pipeline_depth ? o->pipeline_depth : 4;
+    if (D != 2 && D != 4 && D != 8) return fail(CUMF_ERR_ARG, "pipeline_depth=%d not 2, 4 or 8", D);
+    while (sh->L % D) D >>= 1;   // the depth must divide the window length
+    sh->D = D;
     int W;
     if (o->schedule == CUMF_SCHED_SERIAL) {
         W = 1;
     } else if (o->num_workers > 0) {
         W = o->num_workers;
     } else {
-        // default: 8 waves per CU
-        W = c->cu_count * 8 * sh->S;
+        // default: 64 waves per CU (two rounds of full residency), then the
+        // fewest workers that keep the same windows per worker, so an epoch
+        // over-samples the ratings as little as possible
+        const int64_t w0 = (int64_t)c->cu_count * 64 * sh->S;
+        const int64_t nnz = c->nnz;
+        const int64_t uc = std::max<int64_t>(1, (nnz + w0 * sh->L - 1) / (w0 * sh->L));
+        int64_t w = (nnz + uc * sh->L - 1) / (uc * sh->L);
+        const int64_t gran = (int64_t)kSgdWavesPerBlock * sh->S;
+        w = std::max<int64_t>(gran, (w + gran - 1) / gran * gran);
+        W = (int)std::min<int64_t>(w, 1 << 24);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;
@@ -733,7 +748,7 @@ int run_launch(cumf_ctx* c, const std::vector<SgdTask>& tasks, const Shape& sh, 
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     if (!e0 || !e1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
     HIP_TRY(hipEventRecord(e0, c->stream));
-    HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, c->stream));
+    HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D, c->stream));
     HIP_TRY(hipEventRecord(e1, c->stream));
     c->launches += 1;
     for (const SgdTask& t : tasks) c->updates += (int64_t)t.worker_count * t.update_count * sh.L;
@@ -795,6 +810,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     c->last.waves_per_block = kSgdWavesPerBlock;
     c->last.blocks = sh.blocks;
     c->last.batch_len = sh.L;
+    c->last.pipeline_depth = sh.D;
     c->last.updates_per_epoch = 0;
 
     if (o->schedule == CUMF_SCHED_GRID && block_order) {
@@ -845,6 +861,9 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
                                                       : (int)std::max<int64_t>(1, tot ? (int64_t)per_xcd * sz / tot : 1);
                         w = std::min(w, left - (int)(mine.size() - 1 - j));
                         if (w < 1 || sz == 0) continue;
+                        // no more workers than windows: a small block gets one
+                        // window per worker instead of many redundant ones
+                        w = (int)std::min<int64_t>(w, (sz + sh.L - 1) / sh.L);
                         SgdTask tk = base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk);
                         tk.worker_begin = wb;
                         wb += w;

@@ -83,6 +83,11 @@ __device__ inline float f16_to_f32(uint32_t h)
 }
 __device__ inline uint32_t f32_to_f16(float f)
 {
+    // The value is first rounded to FP32 and then to FP16, as __float2half of
+    // an FP32 expression does in the reference.  The empty asm keeps the
+    // compiler from fusing the producing fma into one fma-to-f16 rounding
+    // (v_fma_mixlo_f16), which rounds once and can differ in the last bit.
+    asm("" : "+v"(f));
     _Float16 x = (_Float16)f;   // v_cvt_f16_f32, round to nearest even
     return __builtin_bit_cast(uint16_t, x);
 }
@@ -156,36 +161,49 @@ __device__ inline float pow2_scale(float amax, float target)
 // across XCDs (bypasses the per-XCD L2 copies).
 // ---------------------------------------------------------------------------
 template <int NW> struct Words { uint32_t w[NW]; };
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Global-address-space pointers: the kernels take their buffers from a task
+// table in memory, so the compiler cannot infer the address space and would
+// emit flat accesses, which also count against the LDS wait counter.
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) const int32_t gcint;
+typedef __attribute__((address_space(1))) const float gcfloat;
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* to_global(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
 
 template <int TB, bool COH>
-__device__ inline Words<(TB + 3) / 4> load_raw(const void* p)
+__device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
 {
     Words<(TB + 3) / 4> r;
     if constexpr (TB == 2) {
         static_assert(!COH, "2-byte coherent loads are not supported");
-        r.w[0] = *(const uint16_t*)p;
+        r.w[0] = *(const __attribute__((address_space(1))) uint16_t*)p;
     } else if constexpr (TB == 4) {
         if constexpr (COH)
-            r.w[0] = __hip_atomic_load((uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            r.w[0] = __hip_atomic_load((__attribute__((address_space(1))) uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
-            r.w[0] = *(const uint32_t*)p;
+            r.w[0] = *(const __attribute__((address_space(1))) uint32_t*)p;
     } else if constexpr (TB == 8) {
         uint64_t x;
         if constexpr (COH)
-            x = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x = __hip_atomic_load((__attribute__((address_space(1))) uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
-            x = *(const uint64_t*)p;
+            x = *(const __attribute__((address_space(1))) uint64_t*)p;
         r.w[0] = (uint32_t)x;
         r.w[1] = (uint32_t)(x >> 32);
     } else {
         static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
         if constexpr (COH) {
-            uint64_t a = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint64_t b = __hip_atomic_load((uint64_t*)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t a = __hip_atomic_load((__attribute__((address_space(1))) uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t b = __hip_atomic_load((__attribute__((address_space(1))) uint64_t*)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             r.w[0] = (uint32_t)a; r.w[1] = (uint32_t)(a >> 32);
             r.w[2] = (uint32_t)b; r.w[3] = (uint32_t)(b >> 32);
         } else {
-            uint4 x = *(const uint4*)p;
+            u32x4 x = *(const __attribute__((address_space(1))) u32x4*)p;
             r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
         }
     }
@@ -193,30 +211,30 @@ __device__ inline Words<(TB + 3) / 4> load_raw(const void* p)
 }
 
 template <int TB, bool COH>
-__device__ inline void store_raw(void* p, const Words<(TB + 3) / 4>& r)
+__device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
 {
     if constexpr (TB == 2) {
         static_assert(!COH, "2-byte coherent stores are not supported");
-        *(uint16_t*)p = (uint16_t)r.w[0];
+        *(__attribute__((address_space(1))) uint16_t*)p = (uint16_t)r.w[0];
     } else if constexpr (TB == 4) {
         if constexpr (COH)
-            __hip_atomic_store((uint32_t*)p, r.w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((__attribute__((address_space(1))) uint32_t*)p, r.w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
-            *(uint32_t*)p = r.w[0];
+            *(__attribute__((address_space(1))) uint32_t*)p = r.w[0];
     } else if constexpr (TB == 8) {
         uint64_t x = ((uint64_t)r.w[1] << 32) | r.w[0];
         if constexpr (COH)
-            __hip_atomic_store((uint64_t*)p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((__attribute__((address_space(1))) uint64_t*)p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
-            *(uint64_t*)p = x;
+            *(__attribute__((address_space(1))) uint64_t*)p = x;
     } else {
         if constexpr (COH) {
-            __hip_atomic_store((uint64_t*)p, ((uint64_t)r.w[1] << 32) | r.w[0], __ATOMIC_RELAXED,
+            __hip_atomic_store((__attribute__((address_space(1))) uint64_t*)p, ((uint64_t)r.w[1] << 32) | r.w[0], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store((uint64_t*)p + 1, ((uint64_t)r.w[3] << 32) | r.w[2], __ATOMIC_RELAXED,
+            __hip_atomic_store((__attribute__((address_space(1))) uint64_t*)p + 1, ((uint64_t)r.w[3] << 32) | r.w[2], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            *(uint4*)p = make_uint4(r.w[0], r.w[1], r.w[2], r.w[3]);
+            *(__attribute__((address_space(1))) u32x4*)p = u32x4{r.w[0], r.w[1], r.w[2], r.w[3]};
         }
     }
 }

@@ -43,7 +43,8 @@ constexpr int kSgdMaxBatch = 128;
 
 bool sgd_shape_supported(int k, int prec, int G);
 int sgd_default_group(int k, int prec);
-hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, hipStream_t s);
+// D: software-pipeline depth (2, 4 or 8; must divide the batch length)
+hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, int D, hipStream_t s);
 
 hipError_t launch_init(void* F, float* scales, int64_t rows, int k, int prec, float s, uint64_t seed,
                        uint32_t problem, uint32_t stream_tag, hipStream_t st);

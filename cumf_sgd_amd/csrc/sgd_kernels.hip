@@ -1,8 +1,6 @@
 // gfx950 kernels of the SGD hot path.
 //
-//  sgd_update_kernel   batch-Hogwild update (replaces
-//                      sgd_k128_kernel_hogwild_warp32_lrate,
-//                      singleGPU/sgd_k128_kernel_hogwild_warp32.h:3-104)
+//  launch_sgd          dispatch of the batch-Hogwild update (sgd_update.h)
 //  init_factors_kernel random factor init (random_init, singleGPU/sgd.cu:497-514)
 //  fp8_rescale_kernel  between-epoch power-of-two row rescale (FP8 only)
 //  export_kernel       dequantize * mult + inverse permutation gather
@@ -12,228 +10,15 @@
 //  split/remap/scale/moments/partition kernels: on-device preprocessing
 //                      (sgd.cu:78-134, 411-487)
 #include "kernels.h"
+#include "sgd_update.h"
 
 #include <hip/hip_runtime.h>
 
 namespace cumf {
 
-// ---------------------------------------------------------------------------
-// Batch-Hogwild update.
-//
-// A wave of 64 lanes holds S = 64/G workers ("slots"); the G lanes of a slot
-// share one rating update, lane l of the slot owning the E = K/G consecutive
-// elements [lE, lE+E) of the P and Q rows, so a row is one coalesced
-// K*bytes gather.  Per window, the slot stages its L records (u, v, r) in LDS,
-// then walks them in order with the next record's rows prefetched while the
-// current one is reduced and written back.  When the prefetched row is the
-// row just written, the written value is forwarded from registers, so one
-// slot is exactly a serial SGD over its window sequence.
-//
-// Arithmetic (shared with the CPU oracle): lane partial = p0*q0 then fma over
-// the lane's other elements; butterfly sum over the slot (lane l + lane l^d,
-// d = 1, 2, 4, ...); e = r - dot; p' = fma(lr, fma(e, q, -(lp*p)), p) and
-// q' = fma(lr, fma(e, p, -(lq*q)), q) with the old p; store rounds p', q' to
-// the storage format.
-// ---------------------------------------------------------------------------
-constexpr int kWavesPerBlock = 4;
-constexpr int kMaxBatch = 128;
-
-template <int G>
-__device__ inline float slot_sum(float x)
-{
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) x += __shfl_xor(x, d, 64);
-    return x;
-}
-
-template <int K, int G, int PREC, bool COH>
-__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
-{
-    constexpr int S = 64 / G;
-    constexpr int E = K / G;
-    constexpr int EB = PrecBytes<PREC>::value;
-    constexpr int TB = E * EB;
-    constexpr int NW = (TB + 3) / 4;
-    constexpr bool FP8 = PREC == kE4M3;
-    static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
-    static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
-
-    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
-    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
-    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
-
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int slot = lane / G;
-    const int ls = lane % G;
-
-    int64_t blk = blockIdx.x;
-    if (a.num_xcd > 1) blk = (int64_t)(blockIdx.x % a.num_xcd) * (gridDim.x / a.num_xcd) + blockIdx.x / a.num_xcd;
-    const int worker = (int)((blk * kWavesPerBlock + wave) * S + slot);
-
-    int t = 0;
-    while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
-    if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
-    const SgdTask T = a.tasks[t];
-    if (T.nnz <= 0) return;
-    const uint32_t lw = (uint32_t)(worker - T.worker_begin);
-    const Key key = make_key(a.seed, T.problem);
-    const int L = a.batch_len;
-
-    int32_t* su = s_u[wave][slot];
-    int32_t* sv = s_v[wave][slot];
-    float* sr = s_r[wave][slot];
-
-    char* Pb = (char*)T.P + (int64_t)ls * TB;
-    char* Qb = (char*)T.Q + (int64_t)ls * TB;
-    constexpr int64_t kRow = (int64_t)K * EB;
-
-    for (int ep = a.epoch_begin; ep < a.epoch_end; ++ep) {
-        const float lr = T.lr[ep];
-        for (int b = 0; b < T.update_count; ++b) {
-            const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)b, T.nnz);
-            for (int j = ls; j < L; j += G) {
-                int64_t idx = start + j;
-                while (idx >= T.nnz) idx -= T.nnz;
-                su[j] = T.u[idx];
-                sv[j] = T.v[idx];
-                sr[j] = T.r[idx];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-            int uc = su[0], vc = sv[0];
-            float rc = sr[0];
-            Words<NW> pw = load_raw<TB, COH>(Pb + (int64_t)uc * kRow);
-            Words<NW> qw = load_raw<TB, COH>(Qb + (int64_t)vc * kRow);
-            float psc = 1.0f, qsc = 1.0f;
-            if constexpr (FP8) { psc = T.ps[uc]; qsc = T.qs[vc]; }
-
-            for (int i = 0; i < L; ++i) {
-                const bool more = i + 1 < L;
-                int un = 0, vn = 0;
-                float rn = 0.0f, psn = 1.0f, qsn = 1.0f;
-                Words<NW> pnw, qnw;
-                if (more) {
-                    un = su[i + 1];
-                    vn = sv[i + 1];
-                    rn = sr[i + 1];
-                    pnw = load_raw<TB, COH>(Pb + (int64_t)un * kRow);
-                    qnw = load_raw<TB, COH>(Qb + (int64_t)vn * kRow);
-                    if constexpr (FP8) { psn = T.ps[un]; qsn = T.qs[vn]; }
-                }
-
-                float p[E], q[E];
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    p[e] = unpack<PREC>(pw, e);
-                    q[e] = unpack<PREC>(qw, e);
-                    if constexpr (FP8) { p[e] *= psc; q[e] *= qsc; }
-                }
-                float part = p[0] * q[0];
-#pragma unroll
-                for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
-                const float dot = slot_sum<G>(part);
-                const float err = rc - dot;
-
-                Words<NW> po, qo;
-                uint32_t rbits[4] = {0u, 0u, 0u, 0u};
-                const float pinv = FP8 ? 1.0f / psc : 1.0f;
-                const float qinv = FP8 ? 1.0f / qsc : 1.0f;
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const float pn = __builtin_fmaf(lr, __builtin_fmaf(err, q[e], -(T.lambda_p * p[e])), p[e]);
-                    const float qn = __builtin_fmaf(lr, __builtin_fmaf(err, p[e], -(T.lambda_q * q[e])), q[e]);
-                    uint32_t pb, qb;
-                    if constexpr (PREC == kF32) {
-                        pb = __float_as_uint(pn);
-                        qb = __float_as_uint(qn);
-                    } else if constexpr (PREC == kF16) {
-                        pb = f32_to_f16(pn);
-                        qb = f32_to_f16(qn);
-                    } else if constexpr (PREC == kBF16) {
-                        pb = f32_to_bf16(pn);
-                        qb = f32_to_bf16(qn);
-                    } else {
-                        const int j = ls * E + e;
-                        if ((e & 3) == 0) {
-                            U4 rr = philox(U4{(uint32_t)(b * L + i), (uint32_t)ep, lw,
-                                              kStreamRound | ((uint32_t)(j >> 2) << 8)},
-                                           key.k0, key.k1);
-                            rbits[0] = rr.x; rbits[1] = rr.y; rbits[2] = rr.z; rbits[3] = rr.w;
-                        }
-                        const uint32_t wp = rbits[(j >> 1) & 1], wq = rbits[2 | ((j >> 1) & 1)];
-                        pb = f32_to_e4m3_sr(pn * pinv, (j & 1) ? (wp >> 16) : wp);
-                        qb = f32_to_e4m3_sr(qn * qinv, (j & 1) ? (wq >> 16) : wq);
-                    }
-                    pack<PREC>(po, e, pb);
-                    pack<PREC>(qo, e, qb);
-                }
-                store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
-                store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
-
-                if (more) {
-                    if (un == uc) { pnw = po; psn = psc; }
-                    if (vn == vc) { qnw = qo; qsn = qsc; }
-                    uc = un; vc = vn; rc = rn;
-                    pw = pnw; qw = qnw;
-                    psc = psn; qsc = qsn;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-    }
-}
-
-template <int K, int G, int PREC, bool COH>
-static hipError_t launch_sgd_t(const SgdLaunch& a, int blocks, hipStream_t s)
-{
-    hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
-    return hipGetLastError();
-}
-
-template <int K, int G, int PREC>
-static hipError_t launch_sgd_c(const SgdLaunch& a, int blocks, bool coh, hipStream_t s)
-{
-    constexpr int TB = (K / G) * PrecBytes<PREC>::value;
-    if constexpr (TB == 2) {
-        if (coh) return hipErrorInvalidValue;
-        return launch_sgd_t<K, G, PREC, false>(a, blocks, s);
-    } else {
-        return coh ? launch_sgd_t<K, G, PREC, true>(a, blocks, s) : launch_sgd_t<K, G, PREC, false>(a, blocks, s);
-    }
-}
-
-template <int K, int PREC>
-static hipError_t launch_sgd_g(const SgdLaunch& a, int blocks, int G, bool coh, hipStream_t s)
-{
-    constexpr int EB = PrecBytes<PREC>::value;
-    // lane groups whose per-lane chunk is 2..16 bytes (4..16 for FP8)
-    switch (G) {
-#define CUMF_G(GG)                                                                                     \
-    case GG:                                                                                           \
-        if constexpr (K % GG == 0 && (K / GG) * EB <= 16 && (K / GG) * EB >= 2 &&                      \
-                      (PREC != kE4M3 || (K / GG) % 4 == 0))                                            \
-            return launch_sgd_c<K, GG, PREC>(a, blocks, coh, s);                                       \
-        break;
-        CUMF_G(64)
-        CUMF_G(32)
-        CUMF_G(16)
-        CUMF_G(8)
-        CUMF_G(4)
-#undef CUMF_G
-    default:
-        break;
-    }
-    return hipErrorInvalidValue;
-}
-
 bool sgd_shape_supported(int k, int prec, int G)
 {
-    if (G != 64 && G != 32 && G != 16 && G != 8 && G != 4) return false;
+    if (G != 64 && G != 32 && G != 16 && G != 8) return false;
     if (k != 32 && k != 64 && k != 128) return false;
     if (k % G) return false;
     int E = k / G;
@@ -252,33 +37,13 @@ int sgd_default_group(int k, int prec)
     return G;
 }
 
-hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, hipStream_t s)
+hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, int D, hipStream_t s)
 {
-    switch (k) {
-    case 32:
-        switch (prec) {
-        case kF32: return launch_sgd_g<32, kF32>(a, blocks, G, coh, s);
-        case kF16: return launch_sgd_g<32, kF16>(a, blocks, G, coh, s);
-        case kBF16: return launch_sgd_g<32, kBF16>(a, blocks, G, coh, s);
-        case kE4M3: return launch_sgd_g<32, kE4M3>(a, blocks, G, coh, s);
-        }
-        break;
-    case 64:
-        switch (prec) {
-        case kF32: return launch_sgd_g<64, kF32>(a, blocks, G, coh, s);
-        case kF16: return launch_sgd_g<64, kF16>(a, blocks, G, coh, s);
-        case kBF16: return launch_sgd_g<64, kBF16>(a, blocks, G, coh, s);
-        case kE4M3: return launch_sgd_g<64, kE4M3>(a, blocks, G, coh, s);
-        }
-        break;
-    case 128:
-        switch (prec) {
-        case kF32: return launch_sgd_g<128, kF32>(a, blocks, G, coh, s);
-        case kF16: return launch_sgd_g<128, kF16>(a, blocks, G, coh, s);
-        case kBF16: return launch_sgd_g<128, kBF16>(a, blocks, G, coh, s);
-        case kE4M3: return launch_sgd_g<128, kE4M3>(a, blocks, G, coh, s);
-        }
-        break;
+    switch (prec) {
+    case kF32: return launch_sgd_prec<kF32>(a, blocks, k, G, coh, D, s);
+    case kF16: return launch_sgd_prec<kF16>(a, blocks, k, G, coh, D, s);
+    case kBF16: return launch_sgd_prec<kBF16>(a, blocks, k, G, coh, D, s);
+    case kE4M3: return launch_sgd_prec<kE4M3>(a, blocks, k, G, coh, D, s);
     }
     return hipErrorInvalidValue;
 }

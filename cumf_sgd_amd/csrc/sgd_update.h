@@ -1,0 +1,325 @@
+// The batch-Hogwild update kernel template, instantiated per factor format in
+// sgd_update_<format>.hip (one translation unit each, compiled in parallel).
+#pragma once
+
+#include "kernels.h"
+
+namespace cumf {
+
+// ---------------------------------------------------------------------------
+// Batch-Hogwild update.
+//
+// A wave of 64 lanes holds S = 64/G workers ("slots"); the G lanes of a slot
+// share one rating update, lane l of the slot owning the E = K/G consecutive
+// elements [lE, lE+E) of the P and Q rows, so a row is one coalesced
+// K*bytes gather.  Per window, a slot stages its L records (u, v, r) in LDS
+// (the next window's records are loaded into registers while the current one
+// runs), then walks them in order as a D-deep software pipeline: the rows of
+// sample j + D are requested right after sample j is written back, so D row
+// pairs are in flight per slot.  A prefetched row may predate the write-back
+// of one of the D - 1 samples before it; those written rows are kept in a
+// register ring and forwarded when the ids match, so one slot is exactly a
+// serial SGD over its window sequence.
+//
+// Arithmetic (shared with the CPU oracle): lane partial = p0*q0 then fma over
+// the lane's other elements; butterfly sum over the slot (lane l + lane l^d,
+// d = 1, 2, 4, ...); e = r - dot; p' = fma(lr, fma(e, q, -(lp*p)), p) and
+// q' = fma(lr, fma(e, p, -(lq*q)), q) with the old p; store rounds p', q' to
+// the storage format.
+// ---------------------------------------------------------------------------
+constexpr int kWavesPerBlock = kSgdWavesPerBlock;
+constexpr int kMaxBatch = kSgdMaxBatch;
+
+template <int CTRL>
+__device__ inline float dpp(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+
+// Butterfly sum over aligned groups of G lanes.  Every lane of a group ends
+// with the same value: the balanced pairwise tree over the group in lane
+// order.  Steps 1, 2 use quad permutes, 4 and 8 the half-row / row mirrors
+// (lane i reads lane 7-i / 15-i, whose partial sums equal those of lane
+// i^4 / i^8 at that point), 16 a swizzle, and the two rows-pairs of a
+// 64-lane group are combined from lanes 0, 16, 32, 48.
+template <int G>
+__device__ inline float slot_sum(float x)
+{
+    if constexpr (G >= 2) x += dpp<0xB1>(x);    // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) x += dpp<0x4E>(x);    // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) x += dpp<0x141>(x);   // row_half_mirror
+    if constexpr (G >= 16) x += dpp<0x140>(x);  // row_mirror
+    if constexpr (G == 32) x += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401F));
+    if constexpr (G == 64) {
+        const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0)) +
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16));
+        const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32)) +
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+        x = a + b;
+    }
+    return x;
+}
+
+template <int NW>
+struct Pending {
+    Words<NW> p, q;
+    int u, v;
+    float r, ps, qs;
+};
+
+template <int K, int G, int PREC, bool COH, int D>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
+{
+    constexpr int S = 64 / G;
+    constexpr int E = K / G;
+    constexpr int EB = PrecBytes<PREC>::value;
+    constexpr int TB = E * EB;
+    constexpr int NW = (TB + 3) / 4;
+    constexpr bool FP8 = PREC == kE4M3;
+    constexpr int RPL = (kMaxBatch + G - 1) / G;   // staged records per lane
+    static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
+    static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
+
+    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
+    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
+    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / G;
+    const int ls = lane % G;
+
+    int64_t blk = blockIdx.x;
+    if (a.num_xcd > 1) blk = (int64_t)(blockIdx.x % a.num_xcd) * (gridDim.x / a.num_xcd) + blockIdx.x / a.num_xcd;
+    const int worker = (int)((blk * kWavesPerBlock + wave) * S + slot);
+
+    int t = 0;
+    while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
+    if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
+    const SgdTask T = a.tasks[t];
+    if (T.nnz <= 0 || T.update_count <= 0) return;
+    const uint32_t lw = (uint32_t)(worker - T.worker_begin);
+    const Key key = make_key(a.seed, T.problem);
+    const int L = a.batch_len;   // host guarantees 1 <= L <= kMaxBatch and L % D == 0
+
+    int32_t* su = s_u[wave][slot];
+    int32_t* sv = s_v[wave][slot];
+    float* sr = s_r[wave][slot];
+
+    gchar* Pb = to_global((char*)T.P) + (int64_t)ls * TB;
+    gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * TB;
+    gcint* Ru = to_global(T.u);
+    gcint* Rv = to_global(T.v);
+    gcfloat* Rr = to_global(T.r);
+    gcfloat* Ps = to_global((const float*)T.ps);
+    gcfloat* Qs = to_global((const float*)T.qs);
+    gcfloat* LR = to_global(T.lr);
+    constexpr int64_t kRow = (int64_t)K * EB;
+    const int64_t nnz = T.nnz;
+
+    // records of one window, j = ls + G*c, held in registers until staged
+    int32_t ru[RPL], rv[RPL];
+    float rr[RPL];
+    auto fetch_window = [&](int ep, int b) {
+        const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)b, nnz);
+#pragma unroll
+        for (int c = 0; c < RPL; ++c) {
+            const int j = ls + G * c;
+            if (j < L) {
+                int64_t idx = start + j;
+                while (idx >= nnz) idx -= nnz;
+                ru[c] = Ru[idx];
+                rv[c] = Rv[idx];
+                rr[c] = Rr[idx];
+            }
+        }
+    };
+    auto stage_window = [&]() {
+#pragma unroll
+        for (int c = 0; c < RPL; ++c) {
+            const int j = ls + G * c;
+            if (j < L) {
+                su[j] = ru[c];
+                sv[j] = rv[c];
+                sr[j] = rr[c];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto request = [&](Pending<NW>& pd, int j) {
+        pd.u = su[j];
+        pd.v = sv[j];
+        pd.r = sr[j];
+        pd.p = load_raw<TB, COH>(Pb + (int64_t)pd.u * kRow);
+        pd.q = load_raw<TB, COH>(Qb + (int64_t)pd.v * kRow);
+        if constexpr (FP8) {
+            pd.ps = Ps[pd.u];
+            pd.qs = Qs[pd.v];
+        }
+    };
+
+    const int nwin = (a.epoch_end - a.epoch_begin) * T.update_count;
+    if (nwin <= 0) return;
+    fetch_window(a.epoch_begin, 0);
+
+    for (int w = 0; w < nwin; ++w) {
+        const int ep = a.epoch_begin + w / T.update_count;
+        const int b = w % T.update_count;
+        const float lr = LR[ep];
+        stage_window();
+        if (w + 1 < nwin) fetch_window(a.epoch_begin + (w + 1) / T.update_count, (w + 1) % T.update_count);
+
+        Pending<NW> pend[D];
+        // ring of the last D written rows (slot d holds the sample j with j % D == d)
+        Words<NW> wp[D], wq[D];
+        int wu[D], wv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            wu[d] = -1;
+            wv[d] = -1;
+            request(pend[d], d);
+            // keep the requests in consumption order, so the wait for the
+            // oldest one is a partial wait (the loop header merges this state)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        for (int base = 0; base < L; base += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int j = base + d;
+                Words<NW> pw = pend[d].p, qw = pend[d].q;
+                const int uc = pend[d].u, vc = pend[d].v;
+                const float rc = pend[d].r;
+                float psc = 1.0f, qsc = 1.0f;
+                if constexpr (FP8) { psc = pend[d].ps; qsc = pend[d].qs; }
+                // forward rows written after this sample's rows were requested
+                // (samples j-D+1 .. j-1), oldest first so the newest wins
+#pragma unroll
+                for (int back = D - 1; back >= 1; --back) {
+                    const int s = (d - back + D) % D;
+                    if (wu[s] == uc) pw = wp[s];
+                    if (wv[s] == vc) qw = wq[s];
+                }
+
+                float p[E], q[E];
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    p[e] = unpack<PREC>(pw, e);
+                    q[e] = unpack<PREC>(qw, e);
+                    if constexpr (FP8) { p[e] *= psc; q[e] *= qsc; }
+                }
+                float part = p[0] * q[0];
+#pragma unroll
+                for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
+                const float dot = slot_sum<G>(part);
+                const float err = rc - dot;
+
+                Words<NW> po, qo;
+                uint32_t rbits[4] = {0u, 0u, 0u, 0u};
+                const float pinv = FP8 ? 1.0f / psc : 1.0f;
+                const float qinv = FP8 ? 1.0f / qsc : 1.0f;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const float pn = __builtin_fmaf(lr, __builtin_fmaf(err, q[e], -(T.lambda_p * p[e])), p[e]);
+                    const float qn = __builtin_fmaf(lr, __builtin_fmaf(err, p[e], -(T.lambda_q * q[e])), q[e]);
+                    uint32_t pb, qb;
+                    if constexpr (PREC == kF32) {
+                        pb = __float_as_uint(pn);
+                        qb = __float_as_uint(qn);
+                    } else if constexpr (PREC == kF16) {
+                        pb = f32_to_f16(pn);
+                        qb = f32_to_f16(qn);
+                    } else if constexpr (PREC == kBF16) {
+                        pb = f32_to_bf16(pn);
+                        qb = f32_to_bf16(qn);
+                    } else {
+                        const int jj = ls * E + e;
+                        if ((e & 3) == 0) {
+                            U4 rb = philox(U4{(uint32_t)(b * L + j), (uint32_t)ep, lw,
+                                              kStreamRound | ((uint32_t)(jj >> 2) << 8)},
+                                           key.k0, key.k1);
+                            rbits[0] = rb.x; rbits[1] = rb.y; rbits[2] = rb.z; rbits[3] = rb.w;
+                        }
+                        const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
+                        pb = f32_to_e4m3_sr(pn * pinv, (jj & 1) ? (bp >> 16) : bp);
+                        qb = f32_to_e4m3_sr(qn * qinv, (jj & 1) ? (bq >> 16) : bq);
+                    }
+                    pack<PREC>(po, e, pb);
+                    pack<PREC>(qo, e, qb);
+                }
+                store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
+                store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
+                wp[d] = po;
+                wq[d] = qo;
+                wu[d] = uc;
+                wv[d] = vc;
+                if (j + D < L) request(pend[d], j + D);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+template <int K, int G, int PREC, bool COH, int D>
+inline hipError_t launch_sgd_t(const SgdLaunch& a, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH, D>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int K, int G, int PREC, bool COH>
+inline hipError_t launch_sgd_d(const SgdLaunch& a, int blocks, int D, hipStream_t s)
+{
+    switch (D) {
+    case 2: return launch_sgd_t<K, G, PREC, COH, 2>(a, blocks, s);
+    case 4: return launch_sgd_t<K, G, PREC, COH, 4>(a, blocks, s);
+    case 8: return launch_sgd_t<K, G, PREC, COH, 8>(a, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int K, int G, int PREC>
+inline hipError_t launch_sgd_c(const SgdLaunch& a, int blocks, bool coh, int D, hipStream_t s)
+{
+    constexpr int TB = (K / G) * PrecBytes<PREC>::value;
+    if constexpr (TB == 2) {
+        if (coh) return hipErrorInvalidValue;
+        return launch_sgd_d<K, G, PREC, false>(a, blocks, D, s);
+    } else {
+        return coh ? launch_sgd_d<K, G, PREC, true>(a, blocks, D, s) : launch_sgd_d<K, G, PREC, false>(a, blocks, D, s);
+    }
+}
+
+template <int K, int PREC>
+inline hipError_t launch_sgd_g(const SgdLaunch& a, int blocks, int G, bool coh, int D, hipStream_t s)
+{
+    constexpr int EB = PrecBytes<PREC>::value;
+    // lane groups whose per-lane chunk is 2..16 bytes (4..16 for FP8)
+    switch (G) {
+#define CUMF_G(GG)                                                                                     \
+    case GG:                                                                                           \
+        if constexpr (K % GG == 0 && (K / GG) * EB <= 16 && (K / GG) * EB >= 2 &&                      \
+                      (PREC != kE4M3 || (K / GG) % 4 == 0))                                            \
+            return launch_sgd_c<K, GG, PREC>(a, blocks, coh, D, s);                                    \
+        break;
+        CUMF_G(64)
+        CUMF_G(32)
+        CUMF_G(16)
+        CUMF_G(8)
+#undef CUMF_G
+    default:
+        break;
+    }
+    return hipErrorInvalidValue;
+}
+
+
+// Launch for one factor format; defined in sgd_update_<format>.hip.
+template <int PREC>
+hipError_t launch_sgd_prec(const SgdLaunch& a, int blocks, int k, int G, bool coh, int D, hipStream_t s);
+
+}  // namespace cumf

@@ -1,0 +1,17 @@
+// Batch-Hogwild update instantiations for f32 factor storage.
+#include "sgd_update.h"
+
+namespace cumf {
+
+template <>
+hipError_t launch_sgd_prec<kF32>(const SgdLaunch& a, int blocks, int k, int G, bool coh, int D, hipStream_t s)
+{
+    switch (k) {
+    case 32: return launch_sgd_g<32, kF32>(a, blocks, G, coh, D, s);
+    case 64: return launch_sgd_g<64, kF32>(a, blocks, G, coh, D, s);
+    case 128: return launch_sgd_g<128, kF32>(a, blocks, G, coh, D, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace cumf

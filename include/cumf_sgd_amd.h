@@ -63,10 +63,12 @@ typedef struct cumf_node {
 typedef struct cumf_train_opts {
     int32_t k;                 /* factor length: 32, 64 or 128 */
     int32_t num_workers;       /* Hogwild workers (-s); 0 = autotuned */
-    int32_t batch_len;         /* records per window, update_vector_size (sgd_kernel.cu:198); 0 = 128 */
+    int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128 */
     int32_t schedule;          /* enum cumf_schedule */
     int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = default for (k, precision) */
     int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
+    int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 4 */
+    int32_t reserved;          /* must be 0 */
     float lambda_p;            /* already divided by the rating scale (sgd.cu:673) */
     float lambda_q;
     float alpha;               /* lr_t = alpha/(1+beta*t^1.5) + lrate (sgd_kernel.cu:144-149) */
@@ -157,6 +159,7 @@ typedef struct cumf_launch_info {
     int32_t waves_per_block;
     int32_t blocks;
     int32_t batch_len;
+    int32_t pipeline_depth;
     int64_t updates_per_epoch;
 } cumf_launch_info;
 int cumf_last_launch(cumf_ctx* ctx, cumf_launch_info* info);

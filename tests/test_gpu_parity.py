@@ -1,0 +1,248 @@
+"""GPU parity: the HIP kernels against the CPU oracle, through the C ABI.
+
+Bars: bit-exact for integer/index work and for every FP32-arithmetic kernel
+whose operation order the oracle restates (init, serialized SGD, export);
+RMSE sums within 1e-12 relative (FP64 sum order differs); Hogwild test RMSE
+within 0.005 of the serialized FP32 oracle at the same epoch count.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_PREC = {0: 0, 1: 1, 2: 2, 3: 3}
+
+
+def _problem(seed=3, m=700, n=300, nnz=40_000, n_test=4_000):
+    from cumf_sgd_amd import datagen
+
+    R = datagen.uniform(seed, m, n, nnz, n_test)
+    return R.numpy() + (m, n)
+
+
+def _ctx(u, v, r, m, n):
+    import cumf_sgd_amd as cs
+
+    c = cs.Context(0)
+    c.set_ratings(u, v, r, m, n)
+    return c
+
+
+@pytest.mark.parametrize("k", [32, 64, 128])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_init_bit_exact(gpu, k, prec):
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(m=333, n=129, nnz=1000, n_test=0)
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(k, prec, seed=12345)
+        for which, rows in ((0, m), (1, n)):
+            got, gs = c.get_factors(which)
+            want, ws = oracle.init_factors(rows, k, prec, 12345, which)
+            assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (which, k, prec)
+            if prec == cs.FP8_E4M3:
+                assert np.array_equal(gs, ws)
+    # distribution: U(0,1] * sqrt(1/k) -> mean ~ 0.5*sqrt(1/k)
+    if prec == 0:
+        assert abs(got.mean() - 0.5 * np.sqrt(1.0 / k)) < 0.02 * np.sqrt(1.0 / k) + 1e-3
+
+
+SERIAL_CASES = [
+    (128, 64, 0, 8), (128, 64, 1, 8), (128, 64, 2, 8), (128, 64, 1, 2), (128, 64, 0, 4), (128, 32, 0, 8),
+    (128, 16, 1, 8), (128, 32, 3, 8), (128, 16, 3, 4), (64, 32, 0, 8), (64, 16, 1, 2), (64, 16, 3, 8),
+    (32, 16, 0, 8), (32, 8, 2, 4), (32, 8, 3, 8),
+]
+
+
+@pytest.mark.parametrize("k,G,prec,D", SERIAL_CASES)
+def test_serial_epoch_matches_oracle(gpu, k, G, prec, D):
+    """Serialized single-worker epochs: GPU factors == CPU FP32 oracle factors.
+
+    The oracle restates the kernel's lane-group dot order, so the result is
+    expected bit-exact; the north-star bar (1e-5 relative) is asserted too."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=30_000)
+    lp, lq, seed = 0.05, 0.04, 777
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(k, prec, seed=seed)
+        P0, ps0 = c.get_factors(0)
+        Q0, qs0 = c.get_factors(1)
+        opts = cs.make_opts(k=k, schedule=cs.SERIAL, lambda_p=lp, lambda_q=lq, seed=seed, lanes_per_sample=G,
+                            pipeline_depth=D)
+        c.train(opts, 0, 2)
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+        info = c.last_launch()
+        assert info.num_workers == 1 and info.lanes_per_sample == G and info.pipeline_depth == D
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 8)
+    Pw, Qw, psw, qsw = P0, Q0, ps0, qs0
+    for ep in range(2):
+        Pw, Qw, psw, qsw = oracle.sgd_serial(u, v, r, Pw, Qw, lr, ep, ep + 1, k=k, G=G, prec=prec, lambda_p=lp,
+                                             lambda_q=lq, seed=seed, ps=psw, qs=qsw)
+        if prec == 3:
+            Pw, psw = oracle.fp8_rescale(Pw, psw)
+            Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+    gotP = oracle.widen(P1, ps1, prec)
+    wantP = oracle.widen(Pw, psw, prec)
+    gotQ = oracle.widen(Q1, qs1, prec)
+    wantQ = oracle.widen(Qw, qsw, prec)
+    assert not np.array_equal(oracle.widen(P0, ps0, prec), wantP)   # training moved the factors
+    tol = 1e-5
+    for g, w in ((gotP, wantP), (gotQ, wantQ)):
+        rel = np.abs(g - w) / np.maximum(np.abs(w), 1e-3)
+        assert rel.max() <= tol, rel.max()
+    assert np.array_equal(P1.view(np.uint8), Pw.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3])
+def test_export_and_rmse_match_oracle(gpu, prec):
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=20_000, n_test=5_000)
+    # out-of-range test ids predict b
+    tu = tu.copy()
+    tv = tv.copy()
+    tu[:7] = m + 3
+    tv[7:11] = -1
+    k = 64
+    with _ctx(u, v, r, m, n) as c:
+        c.set_test(tu, tv, tr)
+        c.init_factors(k, prec, seed=9)
+        c.train(cs.make_opts(k=k, schedule=cs.HOGWILD, num_workers=64, seed=9), 0, 2)
+        raw_p, sp = c.get_factors(0)
+        raw_q, sq = c.get_factors(1)
+        mult, b = np.float32(1.0763), np.float32(3.25)
+        pmap = np.random.default_rng(1).permutation(m).astype(np.int32)
+        gotP = c.export_factors(0, mult, pmap)
+        gotQ = c.export_factors(1, mult)
+        got_rmse = c.test_rmse(mult, b)
+    wantP = oracle.widen(raw_p, sp, prec)
+    wantQ = oracle.widen(raw_q, sq, prec)
+    wantP_s = (wantP * mult).astype(np.float32)   # FP32 multiply, as __fmul_rn
+    wantQ_s = (wantQ * mult).astype(np.float32)
+    assert np.array_equal(gotP, wantP_s[pmap])
+    assert np.array_equal(gotQ, wantQ_s)
+    want_rmse = oracle.rmse(tu, tv, tr, wantP_s, wantQ_s, b)
+    assert abs(got_rmse - want_rmse) <= 1e-12 * want_rmse
+
+
+def test_remap_scale_moments(gpu):
+    import math
+
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=50_000)
+    with _ctx(u, v, r, m, n) as c:
+        s1, s2 = c.rating_moments()
+        pm = oracle.gen_random_map(5, m)
+        qm = oracle.gen_random_map(5, n)
+        c.remap(pm, qm)
+        c.scale_ratings(np.float32(1 / 1.1234))
+        gu, gv, gr = c.get_ratings()
+    # integer ratings: every partial sum is exact, so any order gives fsum
+    assert s1 == math.fsum(r.astype(np.float64))
+    assert s2 == math.fsum(r.astype(np.float64) ** 2)
+    nodes = np.zeros(len(u), oracle.NODE_DTYPE)
+    nodes["u"], nodes["v"], nodes["rate"] = u, v, r
+    want = oracle.shuffle_problem(nodes, pm, qm)
+    want = oracle.scale_problem(want, np.float32(1 / 1.1234), 1, 1, m, n)
+    assert np.array_equal(gu, want["u"]) and np.array_equal(gv, want["v"])
+    assert np.array_equal(gr.view(np.uint32), want["rate"].view(np.uint32))
+
+
+@pytest.mark.parametrize("ux,vy", [(1, 1), (3, 2), (8, 8), (5, 1)])
+def test_device_partition_is_stable_counting_sort(gpu, ux, vy):
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=60_000)
+    with _ctx(u, v, r, m, n) as c:
+        off = c.partition(ux, vy)
+        gu, gv, gr = c.get_ratings()
+    us, vs = oracle.grid_segments(m, n, ux, vy)
+    bid = (u // us) * vy + (v // vs)
+    order = np.argsort(bid, kind="stable")
+    assert np.array_equal(gu, u[order]) and np.array_equal(gv, v[order]) and np.array_equal(gr, r[order])
+    counts = np.bincount(bid, minlength=ux * vy)
+    assert np.array_equal(np.diff(off), counts)
+    # same multiset as the reference's cycle-swap partition, same block sizes
+    nodes = np.zeros(len(u), oracle.NODE_DTYPE)
+    nodes["u"], nodes["v"], nodes["rate"] = u, v, r
+    ref, gs, roff = oracle.grid_problem(nodes, m, n, ux, vy)
+    assert np.array_equal(roff, off)
+    for b in range(ux * vy):
+        a = np.sort(ref[roff[b]:roff[b + 1]], order=["u", "v", "rate"])
+        g = np.zeros(off[b + 1] - off[b], oracle.NODE_DTYPE)
+        g["u"], g["v"], g["rate"] = gu[off[b]:off[b + 1]], gv[off[b]:off[b + 1]], gr[off[b]:off[b + 1]]
+        assert np.array_equal(a, np.sort(g, order=["u", "v", "rate"]))
+
+
+def _c1_rmse_curve(prec, schedule, epochs, workers=0, strata=0, coherent=0, lanes=0):
+    """Train the small config (10k x 2k, 900k/100k) through the device
+    pipeline; returns test RMSE per epoch in rating units."""
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+    import oracle
+
+    R = datagen.small(seed=1)
+    u, v, r, tu, tv, tr = R.numpy()
+    ave = np.float32(r.astype(np.float64).mean())
+    std = np.float32(np.sqrt((r.astype(np.float64) ** 2).mean() - np.float64(r.astype(np.float64).mean()) ** 2))
+    scale = max(np.float32(1e-4), std)
+    out = []
+    with _ctx(u, v, r, R.m, R.n) as c:
+        c.set_test(tu, tv, tr)
+        c.scale_ratings(np.float32(1.0 / scale))
+        if strata:
+            c.partition_strata(strata)
+        c.init_factors(128, prec, seed=1)
+        opts = cs.make_opts(k=128, schedule=schedule, num_workers=workers, lambda_p=0.05 / scale, seed=1,
+                            coherent=coherent, lanes_per_sample=lanes)
+        for ep in range(epochs):
+            c.train(opts, ep, ep + 1)
+            out.append(c.test_rmse(np.float32(np.sqrt(scale)), ave / std))
+    return np.array(out), (u, v, r, tu, tv, tr, scale, ave / std)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle_rmse():
+    """Serialized FP32 oracle on the small config, 20 epochs."""
+    from cumf_sgd_amd import datagen
+    import oracle
+
+    R = datagen.small(seed=1)
+    u, v, r, tu, tv, tr = R.numpy()
+    std = np.float32(np.sqrt((r.astype(np.float64) ** 2).mean() - np.float64(r.astype(np.float64).mean()) ** 2))
+    ave = np.float32(r.astype(np.float64).mean())
+    scale = max(np.float32(1e-4), std)
+    rs = (r * np.float32(1.0 / scale)).astype(np.float32)
+    P, _ = oracle.init_factors(R.m, 128, 0, 1, 0)
+    Q, _ = oracle.init_factors(R.n, 128, 0, 1, 1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 24)
+    P, Q, _, _ = oracle.sgd_serial(u, v, rs, P, Q, lr, 0, 20, k=128, G=64, prec=0,
+                                   lambda_p=np.float32(0.05) / scale, lambda_q=np.float32(0.05) / scale, seed=1)
+    fs = np.float32(np.sqrt(scale))
+    return oracle.rmse(tu, tv, tr, (P * fs).astype(np.float32), (Q * fs).astype(np.float32), ave / std)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_hogwild_rmse_within_0005_of_serial_oracle(gpu, c1_oracle_rmse, prec):
+    import cumf_sgd_amd as cs
+
+    curve, _ = _c1_rmse_curve(prec, cs.HOGWILD, 20)
+    print(f"prec={prec} hogwild rmse={curve[-1]:.5f} oracle={c1_oracle_rmse:.5f}")
+    assert np.all(np.isfinite(curve))
+    assert curve[-1] <= c1_oracle_rmse + 0.005
+
+
+def test_stratum_rmse_within_0005_of_serial_oracle(gpu, c1_oracle_rmse):
+    import cumf_sgd_amd as cs
+
+    curve, _ = _c1_rmse_curve(1, cs.STRATUM, 20, strata=8)
+    print(f"stratum rmse={curve[-1]:.5f} oracle={c1_oracle_rmse:.5f}")
+    assert curve[-1] <= c1_oracle_rmse + 0.005

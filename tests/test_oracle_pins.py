@@ -1,0 +1,328 @@
+"""Pins of the CPU oracle against facts fixed outside it: published known-answer
+vectors, independent library implementations of the number formats, the
+reference's own compiled evaluator, closed forms and brute force."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+# ---------------------------------------------------------------- Philox
+# Known-answer vectors of Philox4x32-10 published with Random123 (kat_vectors).
+PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert list(oracle.philox(ctr, key)) == want
+
+
+def test_window_starts_uniform_and_in_range():
+    nnz = 1_000_003
+    s = np.array([oracle.window_start(9, 0, 0, w, 0, b, nnz) for w in range(20) for b in range(500)])
+    assert s.min() >= 0 and s.max() < nnz
+    # uniform over [0, nnz): mean and quartiles within a few standard errors
+    assert abs(s.mean() / nnz - 0.5) < 0.01
+    assert abs(np.mean(s < nnz // 4) - 0.25) < 0.015
+    # every record reachable: starts are not on a coarse lattice (FP32 product in the reference)
+    big = 3_101_144_313
+    t = np.array([oracle.window_start(1, 0, 0, 0, 0, b, big) for b in range(2000)])
+    assert np.any(t % 256 != 0) and t.max() < big
+
+
+# ---------------------------------------------------------------- formats
+def _probe_floats(n=200_000, seed=0):
+    rng = np.random.default_rng(seed)
+    bits = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    f = bits.view(np.float32)
+    return f[np.isfinite(f)]
+
+
+def test_f16_matches_numpy_rne():
+    f = _probe_floats()
+    f = np.concatenate([f, np.float32([65504, 65519.99, 65520, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8, -0.0, 0.0])])
+    want = f.astype(np.float16).view(np.uint16)
+    got = np.array([oracle.f32_to_f16(float(x)) for x in f], np.uint16)
+    assert np.array_equal(got, want)
+    h = np.arange(0, 65536, dtype=np.uint32).astype(np.uint16)
+    back = np.array([oracle.f16_to_f32(int(x)) for x in h], np.float32)
+    ref = h.view(np.float16).astype(np.float32)
+    same = (back.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(back) & np.isnan(ref))
+    assert same.all()
+
+
+def test_bf16_matches_torch_rne():
+    import torch
+
+    f = _probe_floats(100_000, 1)
+    want = torch.from_numpy(f).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    got = np.array([oracle.f32_to_bf16(float(x)) for x in f], np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_e4m3_matches_torch_in_range():
+    import torch
+
+    rng = np.random.default_rng(2)
+    f = np.concatenate([rng.uniform(-447.9, 447.9, 50_000), rng.normal(0, 0.01, 50_000),
+                        rng.normal(0, 1e-3, 20_000)]).astype(np.float32)
+    want = torch.from_numpy(f).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    got = np.array([oracle.f32_to_e4m3_rne(float(x)) for x in f], np.uint8)
+    assert np.array_equal(got, want)
+    allb = torch.arange(256, dtype=torch.uint8).view(torch.float8_e4m3fn).float().numpy()
+    back = np.array([oracle.e4m3_to_f32(i) for i in range(256)], np.float32)
+    assert np.array_equal(np.isnan(back), np.isnan(allb))
+    ok = ~np.isnan(allb)
+    assert np.array_equal(back[ok], allb[ok])
+    # saturation instead of nan above the largest finite value
+    assert oracle.f32_to_e4m3_rne(1e6) == 0x7E and oracle.f32_to_e4m3_rne(-500.0) == 0xFE
+
+
+def test_e4m3_stochastic_rounding_brackets_and_is_unbiased():
+    grid = np.array([oracle.e4m3_to_f32(i) for i in range(0x7F)], np.float32)
+    rng = np.random.default_rng(3)
+    for x in [0.3, 1.1, 3.3e-3, 1.7e-3, 250.0, 0.01171875]:
+        lo = grid[grid <= x].max()
+        hi = grid[grid >= x].min()
+        vals = np.array([oracle.e4m3_to_f32(oracle.f32_to_e4m3_sr(x, int(r)))
+                         for r in rng.integers(0, 65536, 40_000)])
+        assert set(np.unique(vals)) <= {lo, hi}
+        se = (hi - lo) / 2 / np.sqrt(len(vals))
+        assert abs(vals.mean() - np.float32(x)) < 5 * se + 1e-12
+    # representable values are fixed points for every random draw
+    for b in [0x01, 0x08, 0x3C, 0x7E]:
+        x = oracle.e4m3_to_f32(b)
+        assert all(oracle.f32_to_e4m3_sr(x, r) == b for r in (0, 1, 0x7FFF, 0xFFFF))
+
+
+@pytest.mark.parametrize("amax,target", [(0.0884, 14.0), (3.0, 56.0), (56.0, 56.0), (56.01, 56.0), (1e-7, 14.0)])
+def test_pow2_scale_is_smallest_power_of_two(amax, target):
+    s = oracle.pow2_scale(amax, target)
+    e = math.log2(s)
+    assert e == int(e)
+    assert np.float32(amax) <= np.float32(target) * s
+    assert np.float32(amax) > np.float32(target) * (s / 2)
+
+
+# ---------------------------------------------------------------- schedule
+def test_lr_table_known_values():
+    # alpha=0.08, beta=0.3: 0.08, 0.0615385, 0.0432777, 0.0312641, 0.0235294
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 5)
+    assert np.allclose(lr, [0.08, 0.0615385, 0.0432777, 0.0312641, 0.0235294], rtol=0, atol=5e-8)
+    assert lr[0] == np.float32(np.float32(0.08))
+    lr2 = oracle.lr_table(0.08, 0.3, 0.01, 3)
+    assert np.allclose(lr2 - lr[:3], 0.01, atol=1e-7)
+
+
+# ---------------------------------------------------------------- host preprocessing
+def test_random_map_matches_libstdcxx_random_shuffle():
+    """The restatement (rand() % (i+1) swaps) equals std::random_shuffle as
+    compiled into the product library."""
+    import cumf_sgd_amd as cs
+
+    for seed, size in [(0, 1), (1, 2), (7, 1000), (12345, 17770), (2, 480_189)]:
+        a = oracle.gen_random_map(seed, size)
+        assert np.array_equal(a, cs.gen_random_map(seed, size))
+        assert np.array_equal(np.sort(a), np.arange(size))
+        inv = oracle.gen_inv_map(a)
+        assert np.array_equal(inv[a], np.arange(size))
+
+
+def _nodes(u, v, r):
+    x = np.zeros(len(u), oracle.NODE_DTYPE)
+    x["u"], x["v"], x["rate"] = u, v, r
+    return x
+
+
+def test_grid_problem_hand_example():
+    # 2x2 grid over m=4, n=4 (u_seg = v_seg = 2): blocks 0:(u<2,v<2) 1:(u<2,v>=2) 2 3
+    R = _nodes([3, 0, 1, 2, 0, 3], [3, 0, 2, 1, 1, 0], [1, 2, 3, 4, 5, 6])
+    out, gs, off = oracle.grid_problem(R, 4, 4, 2, 2)
+    assert list(gs) == [2, 1, 2, 1] and list(off) == [0, 2, 3, 5, 6]
+    # cycle-swap trace by hand:
+    # start [(3,3),(0,0),(1,2),(2,1),(0,1),(3,0)], pivots [0,2,3,5]
+    # g0 p0:(3,3)->b3 swap with 5 -> [(3,0),...,(3,3)] piv3=6; p0:(3,0)->b2 swap with 3:
+    #   [(2,1),(0,0),(1,2),(3,0),(0,1),(3,3)] piv2=4; p0:(2,1)->b2 swap with 4:
+    #   [(0,1),(0,0),(1,2),(3,0),(2,1),(3,3)] piv2=5; p0 ok, p1 ok
+    assert [(int(a), int(b)) for a, b in zip(out["u"], out["v"])] == [(0, 1), (0, 0), (1, 2), (3, 0), (2, 1), (3, 3)]
+
+
+def test_grid_problem_invariants_and_product_equality():
+    import cumf_sgd_amd as cs
+
+    rng = np.random.default_rng(4)
+    m, n = 1000, 300
+    R = _nodes(rng.integers(0, m, 20000), rng.integers(0, n, 20000), rng.random(20000).astype(np.float32))
+    for ux, vy in [(1, 1), (3, 2), (7, 5), (16, 16)]:
+        out, gs, off = oracle.grid_problem(R, m, n, ux, vy)
+        us, vs = oracle.grid_segments(m, n, ux, vy)
+        bid = (out["u"] // us) * vy + out["v"] // vs
+        assert np.all(np.diff(bid) >= 0)
+        assert np.array_equal(np.bincount(bid, minlength=ux * vy), gs)
+        assert np.array_equal(np.sort(out, order=["u", "v", "rate"]), np.sort(R, order=["u", "v", "rate"]))
+        mine = R.copy()
+        gs2, off2, us2, vs2 = cs.grid_problem(mine, m, n, ux, vy)
+        assert np.array_equal(mine, out) and np.array_equal(off2, off) and (us2, vs2) == (us, vs)
+
+
+def test_collect_data_matches_exact_sums():
+    rng = np.random.default_rng(5)
+    r = rng.integers(1, 6, 100_000).astype(np.float32)
+    R = _nodes(np.zeros(len(r)), np.zeros(len(r)), r)
+    ave, std = oracle.collect_data(R)
+    ex = math.fsum(r.astype(np.float64)) / len(r)
+    ex2 = math.fsum(r.astype(np.float64) ** 2) / len(r)
+    assert ave == np.float32(ex) and std == np.float32(math.sqrt(ex2 - ex * ex))
+    import cumf_sgd_amd as cs
+
+    assert cs.collect_data(R) == (ave, std)
+
+
+def test_scale_problem_localises_ids():
+    R = _nodes([5, 9, 2], [1, 7, 3], [2.0, 4.0, 5.0])
+    out = oracle.scale_problem(R, np.float32(0.5), 2, 2, 5, 4)
+    assert list(out["u"]) == [0, 4, 2] and list(out["v"]) == [1, 3, 3]
+    assert list(out["rate"]) == [1.0, 2.0, 2.5]
+
+
+def test_inverse_shuffle_is_the_gather_by_forward_map():
+    rng = np.random.default_rng(6)
+    for size in [1, 2, 17, 1000]:
+        pm = rng.permutation(size).astype(np.int32)
+        inv = oracle.gen_inv_map(pm)
+        vec = rng.random((size, 3)).astype(np.float32)
+        assert np.array_equal(oracle.inv_shuffle(vec, inv), vec[pm])
+
+
+# ---------------------------------------------------------------- SGD arithmetic
+def test_tree_dot_is_exact_on_integers_and_order_sensitive_otherwise():
+    rng = np.random.default_rng(7)
+    p = rng.integers(-8, 8, 128).astype(np.float32)
+    q = rng.integers(-8, 8, 128).astype(np.float32)
+    for G in (64, 32, 16, 8, 4):
+        assert oracle.tree_dot(p, q, G) == float(np.dot(p.astype(np.float64), q.astype(np.float64)))
+    p = rng.normal(size=128).astype(np.float32)
+    q = rng.normal(size=128).astype(np.float32)
+    exact = math.fsum(float(a) * float(b) for a, b in zip(p, q))
+    for G in (64, 16):
+        assert abs(oracle.tree_dot(p, q, G) - exact) < 1e-5
+
+
+def test_single_update_known_answer_and_gradient():
+    """k=2 hand-computed update, then a finite-difference gradient check of the
+    loss 1/2 e^2 + 1/2 lp|p|^2 + 1/2 lq|q|^2."""
+    # one rating r=3 at (0,0); p=(1,0.5), q=(1,2); dot = 2, e = 1, lr = 0.1, lp=lq=0.1
+    P = np.array([[1.0, 0.5]], np.float32)
+    Q = np.array([[1.0, 2.0]], np.float32)
+    lr = np.full(4, 0.1, np.float32)
+    P1, Q1, _, _ = oracle.sgd_serial([0], [0], [3.0], P, Q, lr, 0, 1, k=2, G=1, prec=0, lambda_p=0.1, lambda_q=0.1,
+                                     seed=0, batch_len=1)
+    # p' = p + 0.1*(1*q - 0.1 p) = (1 + 0.1*(1-0.1), 0.5 + 0.1*(2-0.05)) = (1.09, 0.695)
+    # q' = q + 0.1*(1*p - 0.1 q) = (1 + 0.1*(1-0.1), 2 + 0.1*(0.5-0.2)) = (1.09, 2.03)
+    assert np.allclose(P1, [[1.09, 0.695]], atol=1e-6) and np.allclose(Q1, [[1.09, 2.03]], atol=1e-6)
+    # gradient check at a random point with k=8
+    rng = np.random.default_rng(8)
+    p = rng.normal(size=(1, 8)).astype(np.float64)
+    q = rng.normal(size=(1, 8)).astype(np.float64)
+    r, lp, lq, eta = 1.7, 0.05, 0.02, 1e-3
+
+    def loss(pp, qq):
+        e = r - pp @ qq
+        return 0.5 * e * e + 0.5 * lp * pp @ pp + 0.5 * lq * qq @ qq
+
+    g_p = np.array([(loss(p[0] + h, q[0]) - loss(p[0] - h, q[0])) / 2e-6 for h in np.eye(8) * 1e-6])
+    g_q = np.array([(loss(p[0], q[0] + h) - loss(p[0], q[0] - h)) / 2e-6 for h in np.eye(8) * 1e-6])
+    P2, Q2 = oracle.sgd_serial_f64([0], [0], [r], p, q, np.full(2, eta, np.float32), 0, 1, k=8, lambda_p=lp,
+                                   lambda_q=lq, seed=0, batch_len=1)
+    lr32 = float(np.float32(eta))
+    assert np.allclose((P2[0] - p[0]) / -lr32, g_p, rtol=1e-5, atol=1e-7)
+    assert np.allclose((Q2[0] - q[0]) / -lr32, g_q, rtol=1e-5, atol=1e-7)
+    P3, Q3, _, _ = oracle.sgd_serial([0], [0], [r], p.astype(np.float32), q.astype(np.float32),
+                                     np.full(2, eta, np.float32), 0, 1, k=8, G=8, prec=0, lambda_p=lp, lambda_q=lq,
+                                     seed=0, batch_len=1)
+    assert np.allclose(P3, P2, atol=1e-6) and np.allclose(Q3, Q2, atol=1e-6)
+
+
+def test_serial_sgd_fits_a_rank_one_problem():
+    rng = np.random.default_rng(9)
+    m, n, k = 60, 40, 8
+    a = rng.uniform(0.5, 1.5, m)
+    b = rng.uniform(0.5, 1.5, n)
+    uu, vv = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    u, v = uu.ravel().astype(np.int32), vv.ravel().astype(np.int32)
+    r = (a[u] * b[v]).astype(np.float32)
+    P, _ = oracle.init_factors(m, k, 0, 3, 0)
+    Q, _ = oracle.init_factors(n, k, 0, 3, 1)
+    lr = np.full(60, 0.05, np.float32)
+    P1, Q1, _, _ = oracle.sgd_serial(u, v, r, P, Q, lr, 0, 60, k=k, G=4, prec=0, lambda_p=0.0, lambda_q=0.0, seed=3,
+                                     batch_len=16)
+    fit = np.sqrt(np.mean((np.sum(P1[u] * Q1[v], 1) - r) ** 2))
+    assert fit < 0.01
+
+
+def test_serial_windows_follow_the_window_draws():
+    """Each epoch walks ceil(nnz/L) windows of L records at the drawn starts:
+    with lr = 0 nothing changes; with one record per window the update count
+    per record equals the number of windows starting on it."""
+    rng = np.random.default_rng(10)
+    m, n, k, nnz = 20, 10, 4, 50
+    u = rng.integers(0, m, nnz).astype(np.int32)
+    v = rng.integers(0, n, nnz).astype(np.int32)
+    r = np.ones(nnz, np.float32)
+    P, _ = oracle.init_factors(m, k, 0, 1, 0)
+    Q, _ = oracle.init_factors(n, k, 0, 1, 1)
+    P1, Q1, _, _ = oracle.sgd_serial(u, v, r, P, Q, np.zeros(3, np.float32), 0, 2, k=k, G=2, prec=0, lambda_p=0.1,
+                                     lambda_q=0.1, seed=5, batch_len=7)
+    assert np.array_equal(P1, P) and np.array_equal(Q1, Q)
+    starts = [oracle.window_start(5, 0, 0, 0, 0, b, nnz) for b in range(math.ceil(nnz / 7))]
+    assert len(set(starts)) > 1
+
+
+# ---------------------------------------------------------------- evaluation vs the reference binary
+@pytest.mark.skipif(not oracle.have_mf_predict(), reason="reference mf-predict not built")
+def test_rmse_and_model_file_match_reference_mf_predict(tmp_path):
+    from cumf_sgd_amd import datagen
+
+    rng = np.random.default_rng(11)
+    m, n, k = 50, 30, 16
+    P = rng.normal(0, 0.5, (m, k)).astype(np.float32)
+    Q = rng.normal(0, 0.5, (n, k)).astype(np.float32)
+    tu = rng.integers(-2, m + 3, 3000).astype(np.int32)
+    tv = rng.integers(-2, n + 3, 3000).astype(np.int32)
+    tr = rng.integers(1, 6, 3000).astype(np.float32)
+    b = np.float32(3.3)
+    model = str(tmp_path / "model")
+    test = str(tmp_path / "test.bin")
+    assert oracle.save_model(model, P, Q, b) == 0
+    datagen.write_triplets(test, tu, tv, tr)
+    ref = oracle.mf_predict_rmse(test, model)
+    # mf-predict drops negative ids? it reads them as-is and predicts b (u < 0)
+    mine = oracle.rmse(tu, tv, tr, P, Q, b)
+    assert ref == f"{mine:.4f}"
+    # byte layout: 20-byte header, then P rows, then Q rows
+    raw = open(model, "rb").read()
+    assert len(raw) == 20 + 4 * (m + n) * k
+    assert np.array_equal(np.frombuffer(raw[:16], "<i4"), [0, m, n, k])
+    assert np.frombuffer(raw[16:20], "<f4")[0] == b
+
+
+def test_golden_fixtures():
+    """Committed vectors produced by the reference's compiled tools."""
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    path = os.path.join(here, "mf_predict_rmse.json")
+    if not os.path.exists(path):
+        pytest.skip("golden fixtures not generated")
+    import json
+
+    g = json.load(open(path))
+    for case in g["cases"]:
+        d = np.load(os.path.join(here, case["file"]))
+        mine = oracle.rmse(d["tu"], d["tv"], d["tr"], d["P"], d["Q"], np.float32(d["b"]))
+        assert f"{mine:.4f}" == case["rmse"]
