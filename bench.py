@@ -2,7 +2,8 @@
 
 Workload (one GPU): the medium Netflix-shaped problem, 480,189 users x
 17,770 items, 99,072,112 training + 1,408,395 test ratings (integers 1..5),
-k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated on the device.  One
+k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated (seeded) on the host
+and copied to HBM.  One
 step = one epoch of batch-Hogwild SGD over all training ratings (the
 reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244).
 Inputs are resident in HBM before the timed region starts.
@@ -57,13 +58,13 @@ def prepare(ctx, R, seed: int):
     return scale, np.float32(ave / std), np.float32(np.float32(0.05) / scale)
 
 
-def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, rmse_every=True):
+def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, depth=0, rmse_every=True):
     """Fresh factors, `epochs` epochs; returns (per-epoch train seconds, rmse list)."""
     import cumf_sgd_amd as cs
 
     ctx.init_factors(K, prec, seed)
     opts = cs.make_opts(k=K, schedule=schedule, num_workers=workers, lambda_p=float(lam), seed=seed,
-                        lanes_per_sample=lanes)
+                        lanes_per_sample=lanes, pipeline_depth=depth)
     fs = np.float32(np.sqrt(scale))
     times, rmses = [], []
     for ep in range(epochs):
@@ -100,6 +101,18 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
+def measured_traffic(prec: int, info):
+    """HBM bytes per launch of the update kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), when it was
+    measured on the same kernel shape; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"sgd_update_{PREC_NAME[prec]}_medium.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}>"
+    return d["traffic_bytes_per_launch"] if shape in d["kernel"] else None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +128,8 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--depth", type=int, default=0, help="pipeline depth (2, 4, 8; 0 = library default)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,9 +148,19 @@ def main() -> int:
     prec = cs.PRECISIONS[args.precision]
     sched = cs.SCHEDULES[args.schedule]
     t0 = time.perf_counter()
-    R = datagen.medium(seed=args.seed, device="cuda")
-    torch.cuda.synchronize()
+    # generated on the host (deterministic across machines), then made resident in HBM
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    R = datagen.medium(seed=args.seed, device="cpu")
     log(f"generated {R.nnz} train + {R.tu.numel()} test ratings in {time.perf_counter() - t0:.1f}s")
+    R = datagen.Ratings(R.m, R.n, *(t.to("cuda") for t in (R.u, R.v, R.r, R.tu, R.tv, R.tr)))
+    torch.cuda.synchronize()
+    if args.copy_probe_mb:
+        # a plain copy of known size, for scaling the HBM byte counters of a profiled run
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx = cs.Context(0)
     scale, b, lam = prepare(ctx, R, args.seed)
@@ -147,7 +172,7 @@ def main() -> int:
     # ---- throughput: warmup + K timed epochs (one step = one epoch)
     ctx.init_factors(K, prec, args.seed)
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
-                        lanes_per_sample=args.lanes)
+                        lanes_per_sample=args.lanes, pipeline_depth=args.depth)
     for ep in range(args.warmup):
         ctx.train(opts, ep, ep + 1)
     ctx.sync()
@@ -176,7 +201,7 @@ def main() -> int:
         curves = {}
         for p in (0, 1, 2, 3):
             times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, args.workers, args.lanes,
-                                      sched)
+                                      sched, args.depth)
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005
@@ -194,10 +219,11 @@ def main() -> int:
     # ---- CPU baseline (rank 0, N=1 only)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None
-    try:
-        cpu = cpu_baseline(R, threads, args.cpu_sample, args.cpu_iters)
-    except Exception as e:  # the baseline never decides the GPU result
-        log(f"cpu baseline failed: {e}")
+    if args.cpu_sample > 0:
+        try:
+            cpu = cpu_baseline(R, threads, args.cpu_sample, args.cpu_iters)
+        except Exception as e:  # the baseline never decides the GPU result
+            log(f"cpu baseline failed: {e}")
 
     line = {
         "metric": "rating-updates/s at k=128 + time-to-target test RMSE, BF16/FP8 vs FP32 on GPU",
@@ -211,14 +237,14 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3"}[prec] + " storage / f32 math",
-        "data": "synthetic (seeded, generated on device)",
+        "data": "synthetic (seeded, generated on the host, resident in HBM)",
         "config": {"workload": "medium Netflix-shaped MF: 480189 x 17770, 99072112 train + 1408395 test, k=128, "
                                "lambda 0.05, alpha 0.08, beta 0.3, batch-Hogwild",
                    "precision": args.precision, "schedule": args.schedule, "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info),
                      "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upd_per_launch},
         "time_to_target": ttt,
