@@ -13,6 +13,15 @@ from dataclasses import dataclass
 
 import numpy as np
 
+try:
+    # PyTorch-ROCm ships its own HIP/HSA runtime (an unversioned
+    # libamdhip64.so); the library links the system one.  Both can live in one
+    # process only when the framework's runtime is loaded first, so import it
+    # before the library when it is installed.
+    import torch  # noqa: F401
+except ImportError:  # the library itself does not need torch
+    pass
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcumf_sgd_amd.so")
 CLI_PATH = os.path.join(_HERE, "cumf_sgd")

@@ -92,12 +92,10 @@ __device__ inline uint32_t f32_to_f16(float f)
     return __builtin_bit_cast(uint16_t, x);
 }
 __device__ inline float bf16_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+// Round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ inline uint32_t f32_to_bf16(float f)
 {
-    uint32_t x = __float_as_uint(f);
-    if ((x & 0x7fffffffu) > 0x7f800000u) return ((x >> 16) | 0x40u) & 0xffffu;
-    x += 0x7fffu + ((x >> 16) & 1u);
-    return x >> 16;
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ inline float e4m3_to_f32(uint32_t b)
 {
@@ -240,13 +238,40 @@ __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
 }
 
 // Element i of a packed chunk, widened to FP32 (row scale applied by caller).
+// OCP E4M3 byte `sel` of a word, widened exactly (v_cvt_f32_fp8 on gfx950).
+template <int SEL>
+__device__ inline float e4m3_byte_to_f32(uint32_t w)
+{
+    return __builtin_amdgcn_cvt_f32_fp8((int)w, SEL);
+}
+
 template <int PREC, int NW>
 __device__ inline float unpack(const Words<NW>& r, int i)
 {
     if constexpr (PREC == kF32) return __uint_as_float(r.w[i]);
     else if constexpr (PREC == kF16) return f16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
     else if constexpr (PREC == kBF16) return bf16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
-    else return e4m3_to_f32((r.w[i >> 2] >> ((i & 3) * 8)) & 0xffu);
+    else {
+        switch (i & 3) {
+        case 0: return e4m3_byte_to_f32<0>(r.w[i >> 2]);
+        case 1: return e4m3_byte_to_f32<1>(r.w[i >> 2]);
+        case 2: return e4m3_byte_to_f32<2>(r.w[i >> 2]);
+        default: return e4m3_byte_to_f32<3>(r.w[i >> 2]);
+        }
+    }
+}
+
+// Stochastic-rounding bits: one Philox block per window (key material
+// K0..K3), then word i of element group c of sample j in the window is
+// lowbias32(K_i ^ (j * 0x9E3779B9 + c * 0x85EBCA77)), a bijective 32-bit mixer.
+__host__ __device__ inline uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
 }
 
 template <int PREC, int NW>

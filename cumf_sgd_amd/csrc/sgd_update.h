@@ -168,6 +168,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
         const int ep = a.epoch_begin + w / T.update_count;
         const int b = w % T.update_count;
         const float lr = LR[ep];
+        U4 srk = U4{0u, 0u, 0u, 0u};
+        if constexpr (FP8) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
         stage_window();
         if (w + 1 < nwin) fetch_window(a.epoch_begin + (w + 1) / T.update_count, (w + 1) % T.update_count);
 
@@ -237,10 +239,11 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
                     } else {
                         const int jj = ls * E + e;
                         if ((e & 3) == 0) {
-                            U4 rb = philox(U4{(uint32_t)(b * L + j), (uint32_t)ep, lw,
-                                              kStreamRound | ((uint32_t)(jj >> 2) << 8)},
-                                           key.k0, key.k1);
-                            rbits[0] = rb.x; rbits[1] = rb.y; rbits[2] = rb.z; rbits[3] = rb.w;
+                            const uint32_t t = (uint32_t)j * 0x9E3779B9u + (uint32_t)(jj >> 2) * 0x85EBCA77u;
+                            rbits[0] = lowbias32(srk.x ^ t);
+                            rbits[1] = lowbias32(srk.y ^ t);
+                            rbits[2] = lowbias32(srk.z ^ t);
+                            rbits[3] = lowbias32(srk.w ^ t);
                         }
                         const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
                         pb = f32_to_e4m3_sr(pn * pinv, (jj & 1) ? (bp >> 16) : bp);

@@ -499,26 +499,33 @@ typedef struct {
     float lambda_p, lambda_q;
 } orc_sgd_cfg;
 
-/* Random bits for stochastic rounding of the FP8 write-back of update
- * `sample` (index within the worker's epoch).  Elements are taken in groups
- * of four: group c = j >> 2 draws Philox(ctr = {sample, epoch, worker,
- * STREAM_ROUND | c << 8}); word 0 holds P[4c], P[4c+1] (low, high 16 bits),
- * word 1 P[4c+2], P[4c+3], word 2 and 3 the same four elements of Q. */
-static inline uint32_t sr_bits(uint64_t seed, uint32_t problem, uint32_t worker, uint32_t epoch,
-                               uint32_t sample, int which, int j)
+/* Random bits for stochastic rounding of the FP8 write-back of sample j
+ * (0..L-1) of window `batch`: one Philox block per window,
+ * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}), then for element
+ * group c = e >> 2 the words W_i = lowbias32(K_i ^ (j*0x9E3779B9 +
+ * c*0x85EBCA77)), i = 0..3; W_0 holds P[4c], P[4c+1] (low, high 16 bits),
+ * W_1 P[4c+2], P[4c+3], W_2 and W_3 the same elements of Q. */
+static inline uint32_t lowbias32(uint32_t x)
 {
-    uint32_t ctr[4] = {sample, epoch, worker, STREAM_ROUND | ((uint32_t)(j >> 2) << 8)};
-    uint32_t key[2], out[4];
-    key_of(seed, problem, key);
-    orc_philox4x32_10(ctr, key, out);
-    uint32_t w = out[(which << 1) | ((j >> 1) & 1)];
-    return (j & 1) ? (w >> 16) : (w & 0xffffu);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e)
+{
+    uint32_t t = j * 0x9E3779B9u + (uint32_t)(e >> 2) * 0x85EBCA77u;
+    int i = (which << 1) | ((e >> 1) & 1);
+    uint32_t w = lowbias32(K[i] ^ t);
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
 /* One SGD update of (u, v, r) on FP32 math with storage rounding. */
 static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, float* qs,
-                           int32_t u, int32_t v, float r, float lr,
-                           uint32_t worker, uint32_t epoch, uint32_t sample,
+                           int32_t u, int32_t v, float r, float lr, const uint32_t K[4], uint32_t sj,
                            float* p, float* q, float* lanes)
 {
     int k = c->k;
@@ -536,8 +543,8 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
         float qn = fmaf(lr, fmaf(e, p[j], -(c->lambda_q * q[j])), q[j]);
         uint32_t rp = 0, rq = 0;
         if (c->prec == PREC_E4M3) {
-            rp = sr_bits(c->seed, c->problem, worker, epoch, sample, 0, j);
-            rq = sr_bits(c->seed, c->problem, worker, epoch, sample, 1, j);
+            rp = sr_bits(K, sj, 0, j);
+            rq = sr_bits(K, sj, 1, j);
         }
         store_elem(P, c->prec, bp + j, pn, ip, sp, rp);
         store_elem(Q, c->prec, bq + j, qn, iq, sq, rq);
@@ -563,10 +570,16 @@ EXPORT int orc_sgd_serial(const orc_sgd_cfg* c, const int32_t* ru, const int32_t
         float lr = lr_table[ep];
         for (int64_t b = 0; b < nb; ++b) {
             int64_t start = orc_window_start(c->seed, c->problem, 0, 0, (uint32_t)ep, (uint32_t)b, nnz);
+            uint32_t K[4] = {0, 0, 0, 0};
+            if (c->prec == PREC_E4M3) {
+                uint32_t ctr[4] = {(uint32_t)b, (uint32_t)ep, 0u, STREAM_ROUND};
+                uint32_t key[2];
+                key_of(c->seed, c->problem, key);
+                orc_philox4x32_10(ctr, key, K);
+            }
             for (int i = 0; i < L; ++i) {
                 int64_t off = (start + i) % nnz;
-                sgd_update_one(c, P, Q, ps, qs, ru[off], rv[off], rr[off], lr, 0, (uint32_t)ep,
-                               (uint32_t)(b * L + i), p, q, lanes);
+                sgd_update_one(c, P, Q, ps, qs, ru[off], rv[off], rr[off], lr, K, (uint32_t)i, p, q, lanes);
             }
         }
     }

@@ -1,0 +1,131 @@
+"""GPU tests of the reference-seam entry points: the CLI, cumf_sgd_train
+(sgd_train) and cumf_sgd_update_k128 (sgd_update_k128), with the model file
+scored by the reference's compiled mf-predict."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(seed=5, m=3000, n=800, nnz=200_000, n_test=20_000):
+    from cumf_sgd_amd import datagen
+
+    R = datagen.uniform(seed, m, n, nnz, n_test)
+    return R
+
+
+def _nodes(u, v, r):
+    import cumf_sgd_amd as cs
+
+    x = np.zeros(len(u), cs.NODE_DTYPE)
+    x["u"], x["v"], x["rate"] = u, v, r
+    return x
+
+
+@pytest.mark.parametrize("grid", [(1, 1, 1, 1), (2, 2, 1, 1), (2, 2, 2, 2)])
+def test_sgd_train_pipeline_and_model_file(gpu, tmp_path, grid):
+    import cumf_sgd_amd as cs
+    import oracle
+
+    R = _data()
+    u, v, r, tu, tv, tr = R.numpy()
+    m, n = int(u.max()) + 1, int(v.max()) + 1
+    p = cs.make_param(k=64, num_workers=256, u_grid=grid[0], v_grid=grid[1], x_grid=grid[2], y_grid=grid[3],
+                      num_iters=8)
+    P, Q, b, rmse = cs.sgd_train(p, _nodes(u, v, r), m, n, precision=cs.FP16, seed=3, test=_nodes(tu, tv, tr))
+    assert np.all(np.isfinite(P)) and np.all(np.isfinite(Q))
+    assert rmse[-1] < 0.95 * rmse[0]
+    # b = ave/std of the training ratings (sgd.cu:679)
+    ave, std = oracle.collect_data(_nodes(u, v, r))
+    assert b == np.float32(np.float32(ave) / np.float32(std))
+    # the exported model scores the same through the oracle and the reference's evaluator
+    want = oracle.rmse(tu, tv, tr, P, Q, np.float32(b))
+    assert abs(want - rmse[-1]) <= 1e-9
+    model = str(tmp_path / "model")
+    cs.save_model(model, P, Q, b)
+    if oracle.have_mf_predict():
+        from cumf_sgd_amd import datagen
+
+        test = str(tmp_path / "test.bin")
+        datagen.write_triplets(test, tu, tv, tr)
+        assert oracle.mf_predict_rmse(test, model) == f"{rmse[-1]:.4f}"
+
+
+def test_sgd_update_k128_seam(gpu):
+    """The reference's host pipeline (restated by the oracle: maps, grid,
+    scale with id localisation, FP16 init) feeding the drop-in for
+    sgd_update_k128; outputs are consistent and the model trains."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    R = _data(nnz=100_000, n_test=10_000)
+    u, v, r, tu, tv, tr = R.numpy()
+    m, n = int(u.max()) + 1, int(v.max()) + 1
+    nodes = _nodes(u, v, r)
+    ave, std = oracle.collect_data(nodes)
+    scale = max(np.float32(1e-4), np.float32(std))
+    pm, qm = oracle.gen_random_map(7, m), oracle.gen_random_map(7, n)
+    shuf = oracle.shuffle_problem(nodes, pm, qm)
+    ux = vy = 2
+    grid, gs, off = oracle.grid_problem(shuf, m, n, ux, vy)
+    us, vs = oracle.grid_segments(m, n, ux, vy)
+    local = oracle.scale_problem(grid, np.float32(1.0 / scale), ux, vy, us, vs)
+    blocks = [local[off[b]:off[b + 1]] for b in range(ux * vy)]
+    k = 128
+    hp, _ = oracle.init_factors(ux * us, k, oracle.F16, 7, 0)
+    hq, _ = oracle.init_factors(vy * vs, k, oracle.F16, 7, 1)
+    p = cs.make_param(k=k, num_workers=128, u_grid=2, v_grid=2, num_iters=5, lambda_p=0.05 / scale)
+    hp2, hq2 = hp.copy(), hq.copy()
+    fp, fq = cs.sgd_update_k128(p, m, n, us, vs, blocks, hp2, hq2, seed=7)
+    assert not np.array_equal(hp2, hp)
+    assert np.array_equal(fp, oracle.widen(hp2, None, oracle.F16))
+    assert np.array_equal(fq, oracle.widen(hq2, None, oracle.F16))
+    fs = np.float32(np.sqrt(scale))
+    before = oracle.rmse(pm[tu], qm[tv], tr, oracle.widen(hp, None, 1)[:m] * fs, oracle.widen(hq, None, 1)[:n] * fs, 0)
+    after = oracle.rmse(pm[tu], qm[tv], tr, fp[:m] * fs, fq[:n] * fs, 0)
+    assert after < before * 0.5
+
+
+def test_cli_end_to_end(gpu, tmp_path):
+    import cumf_sgd_amd as cs
+    import oracle
+    from cumf_sgd_amd import datagen
+
+    R = _data(nnz=150_000, n_test=15_000)
+    u, v, r, tu, tv, tr = R.numpy()
+    tr_path, te_path = str(tmp_path / "train.bin"), str(tmp_path / "test.bin")
+    datagen.write_triplets(tr_path, u, v, r)
+    datagen.write_triplets(te_path, tu, tv, tr)
+    model = str(tmp_path / "model")
+    out = subprocess.run([cs.CLI_PATH, "-g", "0", "-l", "0.05", "-a", "0.08", "-b", "0.3", "-s", "200", "-k", "128",
+                          "-t", "6", "--seed", "4", "-p", te_path, tr_path, model], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = [l for l in out.stdout.splitlines() if "test RMSE" in l]
+    assert len(lines) == 6
+    last = lines[-1].split()[-1]
+    size = os.path.getsize(model)
+    assert size == 20 + 4 * (int(u.max()) + 1 + int(v.max()) + 1) * 128
+    if oracle.have_mf_predict():
+        assert oracle.mf_predict_rmse(te_path, model) == last
+
+
+def test_cli_serial_mode_is_deterministic(gpu, tmp_path):
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    R = _data(nnz=30_000, n_test=0)
+    u, v, r, *_ = R.numpy()
+    tr_path = str(tmp_path / "train.bin")
+    datagen.write_triplets(tr_path, u, v, r)
+    outs = []
+    for i in range(2):
+        model = str(tmp_path / f"m{i}")
+        res = subprocess.run([cs.CLI_PATH, "-k", "64", "-t", "2", "--serial", "--seed", "9", tr_path, model],
+                             capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        outs.append(open(model, "rb").read())
+    assert outs[0] == outs[1]
