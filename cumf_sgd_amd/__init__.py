@@ -51,7 +51,7 @@ class TrainOpts(ctypes.Structure):
         ("lanes_per_sample", ctypes.c_int32),
         ("coherent", ctypes.c_int32),
         ("pipeline_depth", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
         ("lambda_p", ctypes.c_float),
         ("lambda_q", ctypes.c_float),
         ("alpha", ctypes.c_float),
@@ -304,7 +304,7 @@ class Context:
 def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambda_p: float = 0.05,
               lambda_q: float | None = None, alpha: float = 0.08, beta: float = 0.3, lrate: float = 0.0,
               seed: int = 1, batch_len: int = 0, lanes_per_sample: int = 0, coherent: int = 0,
-              pipeline_depth: int = 0) -> TrainOpts:
+              pipeline_depth: int = 0, use_graph: int = 0) -> TrainOpts:
     o = TrainOpts()
     o.k, o.schedule, o.num_workers = k, schedule, num_workers
     o.lambda_p = lambda_p
@@ -312,6 +312,7 @@ def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambd
     o.alpha, o.beta, o.lrate, o.seed = alpha, beta, lrate, seed
     o.batch_len, o.lanes_per_sample, o.coherent = batch_len, lanes_per_sample, coherent
     o.pipeline_depth = pipeline_depth
+    o.use_graph = use_graph
     return o
 
 

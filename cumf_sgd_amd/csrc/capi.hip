@@ -109,6 +109,9 @@ struct cumf_ctx {
 
     float* lr = nullptr; size_t lr_cap = 0;
     SgdTask* tasks = nullptr; size_t tasks_cap = 0;
+    int32_t* epoch_dev = nullptr; size_t epoch_dev_cap = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
     double* partials = nullptr; size_t partials_cap = 0;
 
     std::vector<hipEvent_t> ev;   // start/stop pairs of training launches
@@ -264,7 +267,9 @@ int cumf_destroy(cumf_ctx* c)
     release(&c->tu, &c->tu_cap); release(&c->tv, &c->tv_cap); release(&c->tr, &c->tr_cap);
     release(&c->P, &c->P_cap); release(&c->Q, &c->Q_cap);
     release(&c->ps, &c->ps_cap); release(&c->qs, &c->qs_cap);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     release(&c->lr, &c->lr_cap); release(&c->tasks, &c->tasks_cap); release(&c->partials, &c->partials_cap);
+    release(&c->epoch_dev, &c->epoch_dev_cap);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -731,32 +736,6 @@ int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
     return CUMF_OK;
 }
 
-int run_launch(cumf_ctx* c, const std::vector<SgdTask>& tasks, const Shape& sh, int ep, const cumf_train_opts* o,
-               int num_xcd)
-{
-    int rc = ensure(&c->tasks, &c->tasks_cap, tasks.size());
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
-    SgdLaunch a{};
-    a.tasks = c->tasks;
-    a.num_tasks = (int32_t)tasks.size();
-    a.epoch_begin = ep;
-    a.epoch_end = ep + 1;
-    a.batch_len = sh.L;
-    a.seed = o->seed;
-    a.num_xcd = num_xcd;
-    hipEvent_t e0 = next_event(c), e1 = next_event(c);
-    if (!e0 || !e1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
-    HIP_TRY(hipEventRecord(e0, c->stream));
-    HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D, c->stream));
-    HIP_TRY(hipEventRecord(e1, c->stream));
-    c->launches += 1;
-    for (const SgdTask& t : tasks) c->updates += (int64_t)t.worker_count * t.update_count * sh.L;
-    // the tasks buffer is reused by the next launch: wait for this one
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return CUMF_OK;
-}
-
 SgdTask base_task(cumf_ctx* c, const cumf_train_opts* o, int64_t beg, int64_t end, int W, int L, uint32_t tag)
 {
     SgdTask t{};
@@ -780,6 +759,118 @@ SgdTask base_task(cumf_ctx* c, const cumf_train_opts* o, int64_t beg, int64_t en
     return t;
 }
 
+// One launch of the update kernel: a slice of the uploaded task table.
+struct LaunchRec {
+    int64_t task_off;
+    int32_t task_cnt;
+    int32_t num_xcd;
+};
+
+// The launches of one epoch, appended to `tasks`.
+int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_xcd, int ep, int e0,
+               const int32_t* block_order, int64_t order_len, std::vector<SgdTask>& tasks,
+               std::vector<LaunchRec>& launches, int64_t* upd_epoch)
+{
+    const int nb = c->ux * c->vy;
+    *upd_epoch = 0;
+    auto push = [&](std::vector<SgdTask>& t, int nx) {
+        if (t.empty()) return;
+        launches.push_back(LaunchRec{(int64_t)tasks.size(), (int32_t)t.size(), nx});
+        for (const SgdTask& x : t) *upd_epoch += (int64_t)x.worker_count * x.update_count * sh.L;
+        tasks.insert(tasks.end(), t.begin(), t.end());
+    };
+    if (o->schedule == CUMF_SCHED_HOGWILD || o->schedule == CUMF_SCHED_SERIAL) {
+        std::vector<SgdTask> t{base_task(c, o, 0, c->nnz, sh.W, sh.L, 0)};
+        push(t, 1);
+    } else if (o->schedule == CUMF_SCHED_GRID) {
+        for (int i = 0; i < nb; ++i) {
+            int blk = i;
+            if (block_order) blk = block_order[order_len == nb ? i : (int64_t)(ep - e0) * nb + i];
+            const int64_t beg = c->offsets[blk], end = c->offsets[blk + 1];
+            if (end == beg) continue;
+            std::vector<SgdTask> t{base_task(c, o, beg, end, sh.W, sh.L, (uint32_t)blk)};
+            push(t, 1);
+        }
+    } else {
+        // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
+        // which share no P and no Q rows; block i runs on XCD i mod num_xcd.
+        const int s = c->ux;
+        const int per_xcd = sh.W / num_xcd;
+        for (int d = 0; d < s; ++d) {
+            std::vector<SgdTask> t;
+            for (int x = 0; x < num_xcd; ++x) {
+                std::vector<int> mine;
+                int64_t tot = 0;
+                for (int i = x; i < s; i += num_xcd) {
+                    const int blk = i * s + (i + d) % s;
+                    mine.push_back(blk);
+                    tot += c->offsets[blk + 1] - c->offsets[blk];
+                }
+                int wb = x * per_xcd, left = per_xcd;
+                for (size_t j = 0; j < mine.size(); ++j) {
+                    const int blk = mine[j];
+                    const int64_t sz = c->offsets[blk + 1] - c->offsets[blk];
+                    int w = j + 1 == mine.size() ? left
+                                                  : (int)std::max<int64_t>(1, tot ? (int64_t)per_xcd * sz / tot : 1);
+                    w = std::min(w, left - (int)(mine.size() - 1 - j));
+                    if (w < 1 || sz == 0) continue;
+                    // no more workers than windows: a small block gets one
+                    // window per worker instead of many redundant ones
+                    w = (int)std::min<int64_t>(w, (sz + sh.L - 1) / sh.L);
+                    SgdTask tk = base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk);
+                    tk.worker_begin = wb;
+                    wb += w;
+                    left -= w;
+                    t.push_back(tk);
+                }
+            }
+            push(t, num_xcd);
+        }
+    }
+    return CUMF_OK;
+}
+
+// Queue one epoch's launches (and the FP8 rescale) on the context stream.
+// epoch_dev non-null: the kernels read the epoch from device memory and the
+// counter is advanced at the end (the form captured into a graph).
+int queue_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const std::vector<LaunchRec>& ls, int ep,
+                int32_t* epoch_dev, bool timed)
+{
+    for (const LaunchRec& l : ls) {
+        SgdLaunch a{};
+        a.tasks = c->tasks + l.task_off;
+        a.num_tasks = l.task_cnt;
+        a.epoch_begin = ep;
+        a.epoch_end = ep + 1;
+        a.batch_len = sh.L;
+        a.seed = o->seed;
+        a.num_xcd = l.num_xcd;
+        a.epoch_dev = epoch_dev;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (timed) {
+            ev0 = next_event(c);
+            ev1 = next_event(c);
+            if (!ev0 || !ev1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
+            HIP_TRY(hipEventRecord(ev0, c->stream));
+        }
+        HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D, c->stream));
+        if (timed) HIP_TRY(hipEventRecord(ev1, c->stream));
+    }
+    if (c->prec == CUMF_FP8_E4M3) {
+        HIP_TRY(launch_fp8_rescale((uint8_t*)c->P, c->ps, c->m, c->k, c->stream));
+        HIP_TRY(launch_fp8_rescale((uint8_t*)c->Q, c->qs, c->n, c->k, c->stream));
+    }
+    if (epoch_dev) HIP_TRY(launch_advance_int(epoch_dev, c->stream));
+    return CUMF_OK;
+}
+
+uint64_t fnv1a(const void* p, size_t n, uint64_t h)
+{
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
 }  // namespace
 
 extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int32_t e1, const int32_t* block_order,
@@ -793,6 +884,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     if (o->k != c->k) return fail(CUMF_ERR_ARG, "opts k=%d but factors have k=%d", o->k, c->k);
     if (e0 < 0 || e1 < e0 || e1 > 1 << 20) return fail(CUMF_ERR_ARG, "bad epoch range [%d,%d)", e0, e1);
     if (o->schedule < CUMF_SCHED_HOGWILD || o->schedule > CUMF_SCHED_STRATUM) return fail(CUMF_ERR_ARG, "bad schedule");
+    if (o->use_graph != 0 && o->use_graph != 1) return fail(CUMF_ERR_ARG, "use_graph must be 0 or 1");
     if (c->nnz == 0 || e1 == e0) return CUMF_OK;
 
     const int nb = c->ux * c->vy;
@@ -813,6 +905,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     c->last.pipeline_depth = sh.D;
     c->last.updates_per_epoch = 0;
 
+    const bool per_epoch_order = o->schedule == CUMF_SCHED_GRID && block_order && order_len != nb;
     if (o->schedule == CUMF_SCHED_GRID && block_order) {
         if (order_len != nb && order_len != (int64_t)nb * (e1 - e0))
             return fail(CUMF_ERR_ARG, "block_order has %lld entries (need %d or %lld)", (long long)order_len, nb,
@@ -821,66 +914,72 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
             if (block_order[i] < 0 || block_order[i] >= nb) return fail(CUMF_ERR_ARG, "block_order entry out of range");
     }
 
-    for (int ep = e0; ep < e1; ++ep) {
-        int64_t upd_epoch = 0;
-        if (o->schedule == CUMF_SCHED_HOGWILD || o->schedule == CUMF_SCHED_SERIAL) {
-            std::vector<SgdTask> t{base_task(c, o, 0, c->nnz, sh.W, sh.L, 0)};
-            upd_epoch += (int64_t)sh.W * t[0].update_count * sh.L;
-            if ((rc = run_launch(c, t, sh, ep, o, 1))) return rc;
-        } else if (o->schedule == CUMF_SCHED_GRID) {
-            for (int i = 0; i < nb; ++i) {
-                int blk = i;
-                if (block_order) blk = block_order[order_len == nb ? i : (int64_t)(ep - e0) * nb + i];
-                const int64_t beg = c->offsets[blk], end = c->offsets[blk + 1];
-                if (end == beg) continue;
-                std::vector<SgdTask> t{base_task(c, o, beg, end, sh.W, sh.L, (uint32_t)blk)};
-                upd_epoch += (int64_t)sh.W * t[0].update_count * sh.L;
-                if ((rc = run_launch(c, t, sh, ep, o, 1))) return rc;
+    // plan every epoch's launches and upload the task table once
+    std::vector<SgdTask> tasks;
+    std::vector<std::vector<LaunchRec>> plan;
+    const int plans = per_epoch_order ? e1 - e0 : 1;
+    int64_t upd = 0;
+    for (int i = 0; i < plans; ++i) {
+        plan.emplace_back();
+        if ((rc = plan_epoch(c, o, sh, num_xcd, e0 + i, e0, block_order, order_len, tasks, plan.back(), &upd))) return rc;
+    }
+    c->last.updates_per_epoch = upd;
+    if (tasks.empty()) return CUMF_OK;
+    if ((rc = ensure(&c->tasks, &c->tasks_cap, tasks.size()))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
+    int64_t launches_per_epoch = 0;
+    for (const LaunchRec& l : plan[0]) (void)l, ++launches_per_epoch;
+
+    if (o->use_graph && !per_epoch_order) {
+        // the epoch loop as a graph: captured once per plan, replayed per epoch
+        uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
+        const int32_t shape[8] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec};
+        key = fnv1a(shape, sizeof(shape), key);
+        key = fnv1a(&o->seed, sizeof(o->seed), key);
+        if ((rc = ensure(&c->epoch_dev, &c->epoch_dev_cap, 1))) return rc;
+        if (!c->graph_exec || c->graph_key != key) {
+            if (c->graph_exec) {
+                (void)hipGraphExecDestroy(c->graph_exec);
+                c->graph_exec = nullptr;
             }
-        } else {
-            // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
-            // which share no P and no Q rows; block i runs on XCD i mod 8.
-            const int s = c->ux;
-            const int per_xcd = sh.W / num_xcd;
-            for (int d = 0; d < s; ++d) {
-                std::vector<SgdTask> t;
-                for (int x = 0; x < num_xcd; ++x) {
-                    // blocks of this diagonal assigned to XCD x, workers split by size
-                    std::vector<int> mine;
-                    int64_t tot = 0;
-                    for (int i = x; i < s; i += num_xcd) {
-                        const int blk = i * s + (i + d) % s;
-                        mine.push_back(blk);
-                        tot += c->offsets[blk + 1] - c->offsets[blk];
-                    }
-                    int wb = x * per_xcd, left = per_xcd;
-                    for (size_t j = 0; j < mine.size(); ++j) {
-                        const int blk = mine[j];
-                        const int64_t sz = c->offsets[blk + 1] - c->offsets[blk];
-                        int w = j + 1 == mine.size() ? left
-                                                      : (int)std::max<int64_t>(1, tot ? (int64_t)per_xcd * sz / tot : 1);
-                        w = std::min(w, left - (int)(mine.size() - 1 - j));
-                        if (w < 1 || sz == 0) continue;
-                        // no more workers than windows: a small block gets one
-                        // window per worker instead of many redundant ones
-                        w = (int)std::min<int64_t>(w, (sz + sh.L - 1) / sh.L);
-                        SgdTask tk = base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk);
-                        tk.worker_begin = wb;
-                        wb += w;
-                        left -= w;
-                        upd_epoch += (int64_t)w * tk.update_count * sh.L;
-                        t.push_back(tk);
-                    }
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int qrc = queue_epoch(c, o, sh, plan[0], e0, c->epoch_dev, false);
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (qrc) {
+                if (g) (void)hipGraphDestroy(g);
+                return qrc;
+            }
+            if (ce != hipSuccess) return fail(CUMF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                c->graph_exec = nullptr;
+                return fail(CUMF_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+            }
+            c->graph_key = key;
+        }
+        HIP_TRY(launch_set_int(c->epoch_dev, e0, c->stream));
+        for (int ep = e0; ep < e1; ++ep) {
+            hipEvent_t ev0 = next_event(c), ev1 = next_event(c);
+            if (!ev0 || !ev1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
+            HIP_TRY(hipEventRecord(ev0, c->stream));
+            HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+            HIP_TRY(hipEventRecord(ev1, c->stream));
+            c->launches += launches_per_epoch;
+            c->updates += upd;
+        }
+    } else {
+        for (int ep = e0; ep < e1; ++ep) {
+            const std::vector<LaunchRec>& ls = plan[per_epoch_order ? ep - e0 : 0];
+            if ((rc = queue_epoch(c, o, sh, ls, ep, nullptr, true))) return rc;
+            for (const LaunchRec& l : ls)
+                for (int32_t t = 0; t < l.task_cnt; ++t) {
+                    const SgdTask& x = tasks[(size_t)(l.task_off + t)];
+                    c->updates += (int64_t)x.worker_count * x.update_count * sh.L;
                 }
-                if (t.empty()) continue;
-                if ((rc = run_launch(c, t, sh, ep, o, num_xcd))) return rc;
-            }
+            c->launches += (int64_t)ls.size();
         }
-        if (c->prec == CUMF_FP8_E4M3) {
-            HIP_TRY(launch_fp8_rescale((uint8_t*)c->P, c->ps, c->m, c->k, c->stream));
-            HIP_TRY(launch_fp8_rescale((uint8_t*)c->Q, c->qs, c->n, c->k, c->stream));
-        }
-        c->last.updates_per_epoch = upd_epoch;
     }
     return harvest_events(c);
 }

@@ -36,6 +36,7 @@ struct SgdLaunch {
     uint64_t seed;
     int32_t num_xcd;          // > 1: workers numbered XCD-major over blockIdx % num_xcd
     int32_t pad;
+    const int32_t* epoch_dev; // non-null: the launch's first epoch is *epoch_dev (graph replay)
 };
 
 constexpr int kSgdWavesPerBlock = 4;
@@ -45,6 +46,10 @@ bool sgd_shape_supported(int k, int prec, int G);
 int sgd_default_group(int k, int prec);
 // D: software-pipeline depth (2, 4 or 8; must divide the batch length)
 hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, int D, hipStream_t s);
+
+// *p = value, and *p += 1 (device-side epoch counter of a captured epoch loop)
+hipError_t launch_set_int(int32_t* p, int32_t value, hipStream_t st);
+hipError_t launch_advance_int(int32_t* p, hipStream_t st);
 
 hipError_t launch_init(void* F, float* scales, int64_t rows, int k, int prec, float s, uint64_t seed,
                        uint32_t problem, uint32_t stream_tag, hipStream_t st);

@@ -49,6 +49,24 @@ hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bo
 }
 
 // ---------------------------------------------------------------------------
+// Device-side epoch counter.
+// ---------------------------------------------------------------------------
+__global__ void set_int_kernel(int32_t* p, int32_t value) { *p = value; }
+__global__ void advance_int_kernel(int32_t* p) { *p += 1; }
+
+hipError_t launch_set_int(int32_t* p, int32_t value, hipStream_t st)
+{
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, p, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_int(int32_t* p, hipStream_t st)
+{
+    hipLaunchKernelGGL(advance_int_kernel, dim3(1), dim3(1), 0, st, p);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Factor init: element e of row r is U(0,1] * s from word (e & 3) of
 // Philox(ctr = {e >> 2, r_lo, r_hi, stream}).  One wave per row.  FP8 rows get
 // the power-of-two scale for kFp8InitTarget and RNE quantization.

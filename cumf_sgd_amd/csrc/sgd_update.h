@@ -160,18 +160,19 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
         }
     };
 
+    const int ep0 = a.epoch_dev ? *to_global(a.epoch_dev) : a.epoch_begin;
     const int nwin = (a.epoch_end - a.epoch_begin) * T.update_count;
     if (nwin <= 0) return;
-    fetch_window(a.epoch_begin, 0);
+    fetch_window(ep0, 0);
 
     for (int w = 0; w < nwin; ++w) {
-        const int ep = a.epoch_begin + w / T.update_count;
+        const int ep = ep0 + w / T.update_count;
         const int b = w % T.update_count;
         const float lr = LR[ep];
         U4 srk = U4{0u, 0u, 0u, 0u};
         if constexpr (FP8) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
         stage_window();
-        if (w + 1 < nwin) fetch_window(a.epoch_begin + (w + 1) / T.update_count, (w + 1) % T.update_count);
+        if (w + 1 < nwin) fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
 
         Pending<NW> pend[D];
         // ring of the last D written rows (slot d holds the sample j with j % D == d)
