@@ -128,6 +128,14 @@ def _load():
                                ctypes.POINTER(I32)], ctypes.c_int),
         "cumf_free_host": ([P], None),
         "cumf_save_model": ([ctypes.c_char_p, I32, I32, I32, F32, P, P], ctypes.c_int),
+        "cumf_batch_create": ([ctypes.c_int, I32, ctypes.POINTER(P)], ctypes.c_int),
+        "cumf_batch_destroy": ([P], ctypes.c_int),
+        "cumf_batch_add": ([P, P, P, P, I64, I32, I32, I32, F32, F32, P, P, P, I64, U64, ctypes.POINTER(I32)],
+                           ctypes.c_int),
+        "cumf_batch_train": ([P, ctypes.POINTER(TrainOpts), I32, I32], ctypes.c_int),
+        "cumf_batch_test_rmse": ([P, I32, F32, F32, D], ctypes.c_int),
+        "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
+        "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -299,6 +307,67 @@ class Context:
 
     def sync(self) -> None:
         _check(lib.cumf_sync(self._h), "cumf_sync")
+
+
+class Batch:
+    """Many independent problems trained together (cumf_batch_*)."""
+
+    def __init__(self, precision: int, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib.cumf_batch_create(device, precision, ctypes.byref(h)), "cumf_batch_create")
+        self._h = h
+        self.precision = precision
+        self.shapes = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.cumf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, u, v, r, m, n, k, lambda_p, lambda_q=None, tu=None, tv=None, tr=None, seed=1) -> int:
+        idx = ctypes.c_int32()
+        lq = lambda_p if lambda_q is None else lambda_q
+        tn = 0 if tu is None else len(tu)
+        _check(lib.cumf_batch_add(self._h, _ptr(u), _ptr(v), _ptr(r), len(u), m, n, k, float(lambda_p), float(lq),
+                                  _ptr(tu), _ptr(tv), _ptr(tr), tn, seed, ctypes.byref(idx)), "cumf_batch_add")
+        self.shapes.append((m, n, k))
+        return idx.value
+
+    def train(self, opts: "TrainOpts", epoch_begin: int, epoch_end: int) -> None:
+        _check(lib.cumf_batch_train(self._h, ctypes.byref(opts), epoch_begin, epoch_end), "cumf_batch_train")
+
+    def test_rmse(self, index: int, mult: float = 1.0, b: float = 0.0) -> float:
+        out = ctypes.c_double()
+        _check(lib.cumf_batch_test_rmse(self._h, index, float(mult), float(b), ctypes.byref(out)),
+               "cumf_batch_test_rmse")
+        return out.value
+
+    def get_factors(self, index: int, which: int):
+        m, n, k = self.shapes[index]
+        rows = m if which == 0 else n
+        data = np.empty(rows * k, STORAGE_DTYPE[self.precision])
+        scales = np.empty(rows, np.float32) if self.precision == FP8_E4M3 else None
+        _check(lib.cumf_batch_get_factors(self._h, index, which, _ptr(data), _ptr(scales)), "cumf_batch_get_factors")
+        return data.reshape(rows, k), scales
+
+    def kernel_time(self) -> KernelTime:
+        ms = ctypes.c_double()
+        la, up = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib.cumf_batch_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)),
+               "cumf_batch_kernel_time")
+        return KernelTime(ms.value, la.value, up.value)
 
 
 def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambda_p: float = 0.05,

@@ -1,0 +1,368 @@
+// C ABI for batched training: many independent matrix factorizations of
+// uneven size and mixed factor length (32, 64, 128) trained together, one
+// launch per epoch (sgd_batched_kernel).  No reference counterpart: the
+// reference trains one problem per process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/cumf_sgd_amd.h"
+#include "errors.h"
+#include "kernels.h"
+
+using namespace cumf;
+
+namespace {
+
+#define BHIP_TRY(expr)                                                                               \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(CUMF_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                       \
+    } while (0)
+
+struct Problem {
+    int64_t nnz = 0, tnnz = 0;
+    int32_t m = 0, n = 0, k = 0;
+    float lp = 0, lq = 0;
+    int32_t* u = nullptr;
+    int32_t* v = nullptr;
+    float* r = nullptr;
+    int32_t* tu = nullptr;
+    int32_t* tv = nullptr;
+    float* tr = nullptr;
+    void* P = nullptr;
+    void* Q = nullptr;
+    float* ps = nullptr;
+    float* qs = nullptr;
+};
+
+int eb_of(int prec) { return prec == CUMF_FP32 ? 4 : (prec == CUMF_FP8_E4M3 ? 1 : 2); }
+
+int dev_alloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return CUMF_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(CUMF_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    return CUMF_OK;
+}
+
+void dev_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+struct cumf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    int prec = CUMF_BF16;
+    std::vector<Problem> probs;
+    float* lr = nullptr;
+    int lr_n = 0;
+    SgdTask* tasks = nullptr;
+    size_t tasks_n = 0;
+    double* partials = nullptr;
+    size_t partials_n = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0;
+    int64_t launches = 0, updates = 0;
+};
+
+namespace {
+
+int check_batch(cumf_batch* b)
+{
+    if (!b) return fail(CUMF_ERR_ARG, "null batch");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return fail(CUMF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    return CUMF_OK;
+}
+
+int problem_of(cumf_batch* b, int32_t index, Problem** p)
+{
+    if (index < 0 || index >= (int32_t)b->probs.size()) return fail(CUMF_ERR_ARG, "problem %d out of range", index);
+    *p = &b->probs[(size_t)index];
+    return CUMF_OK;
+}
+
+void free_problem(Problem& p)
+{
+    dev_free(p.u); dev_free(p.v); dev_free(p.r);
+    dev_free(p.tu); dev_free(p.tv); dev_free(p.tr);
+    dev_free(p.P); dev_free(p.Q); dev_free(p.ps); dev_free(p.qs);
+    p = Problem{};
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumf_batch_create(int device, int32_t precision, cumf_batch** out)
+{
+    if (!out) return fail(CUMF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (precision < CUMF_FP32 || precision > CUMF_FP8_E4M3) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(CUMF_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    device = (device < 0 ? 0 : device) % count;
+    BHIP_TRY(hipSetDevice(device));
+    cumf_batch* b = new cumf_batch;
+    b->device = device;
+    b->prec = precision;
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+        hipEventCreate(&b->ev1) != hipSuccess) {
+        cumf_batch_destroy(b);
+        return fail(CUMF_ERR_HIP, "stream/event creation failed");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cu_count = prop.multiProcessorCount;
+    *out = b;
+    return CUMF_OK;
+}
+
+int cumf_batch_destroy(cumf_batch* b)
+{
+    if (!b) return CUMF_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Problem& p : b->probs) free_problem(p);
+    dev_free(b->lr);
+    dev_free(b->tasks);
+    dev_free(b->partials);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CUMF_OK;
+}
+
+int cumf_batch_add(cumf_batch* b, const int32_t* u, const int32_t* v, const float* r, int64_t nnz, int32_t m,
+                   int32_t n, int32_t k, float lambda_p, float lambda_q, const int32_t* tu, const int32_t* tv,
+                   const float* tr, int64_t test_nnz, uint64_t seed, int32_t* index)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (nnz < 0 || m <= 0 || n <= 0 || (nnz > 0 && (!u || !v || !r)) || test_nnz < 0 ||
+        (test_nnz > 0 && (!tu || !tv || !tr)))
+        return fail(CUMF_ERR_ARG, "bad problem arguments");
+    if (k != 32 && k != 64 && k != 128) return fail(CUMF_ERR_ARG, "k=%d unsupported (32, 64 or 128)", k);
+    if (b->probs.size() >= (1u << 24)) return fail(CUMF_ERR_ARG, "too many problems");
+    const uint32_t idx = (uint32_t)b->probs.size();
+    Problem p;
+    p.nnz = nnz; p.tnnz = test_nnz; p.m = m; p.n = n; p.k = k; p.lp = lambda_p; p.lq = lambda_q;
+    const size_t eb = (size_t)eb_of(b->prec);
+    bool ok = !dev_alloc((void**)&p.u, 4 * (size_t)nnz) && !dev_alloc((void**)&p.v, 4 * (size_t)nnz) &&
+              !dev_alloc((void**)&p.r, 4 * (size_t)nnz) && !dev_alloc((void**)&p.tu, 4 * (size_t)test_nnz) &&
+              !dev_alloc((void**)&p.tv, 4 * (size_t)test_nnz) && !dev_alloc((void**)&p.tr, 4 * (size_t)test_nnz) &&
+              !dev_alloc(&p.P, (size_t)m * k * eb) && !dev_alloc(&p.Q, (size_t)n * k * eb);
+    if (ok && b->prec == CUMF_FP8_E4M3)
+        ok = !dev_alloc((void**)&p.ps, 4 * (size_t)m) && !dev_alloc((void**)&p.qs, 4 * (size_t)n);
+    if (!ok) {
+        free_problem(p);
+        return CUMF_ERR_NOMEM;
+    }
+    auto cleanup = [&](int code) {
+        free_problem(p);
+        return code;
+    };
+    hipError_t e = hipSuccess;
+    if (nnz) {
+        if (e == hipSuccess) e = hipMemcpyAsync(p.u, u, 4 * (size_t)nnz, hipMemcpyDefault, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(p.v, v, 4 * (size_t)nnz, hipMemcpyDefault, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(p.r, r, 4 * (size_t)nnz, hipMemcpyDefault, b->stream);
+    }
+    if (test_nnz) {
+        if (e == hipSuccess) e = hipMemcpyAsync(p.tu, tu, 4 * (size_t)test_nnz, hipMemcpyDefault, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(p.tv, tv, 4 * (size_t)test_nnz, hipMemcpyDefault, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(p.tr, tr, 4 * (size_t)test_nnz, hipMemcpyDefault, b->stream);
+    }
+    if (e != hipSuccess) return cleanup(fail(CUMF_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    // every training id must be a row of its factor matrices
+    unsigned long long* bad = nullptr;
+    if (dev_alloc((void**)&bad, sizeof(unsigned long long))) return cleanup(CUMF_ERR_NOMEM);
+    unsigned long long hb = 0;
+    e = hipMemcpyAsync(bad, &hb, sizeof(hb), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = launch_check_ids(p.u, p.v, nnz, m, n, bad, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    dev_free(bad);
+    if (e != hipSuccess) return cleanup(fail(CUMF_ERR_HIP, "id check: %s", hipGetErrorString(e)));
+    if (hb) return cleanup(fail(CUMF_ERR_ARG, "%llu ratings have ids outside [0,%d) x [0,%d)", hb, m, n));
+    const float s = (float)std::sqrt(1.0 / k);
+    e = launch_init(p.P, p.ps, m, k, b->prec, s, seed, idx, kStreamInitP, b->stream);
+    if (e == hipSuccess) e = launch_init(p.Q, p.qs, n, k, b->prec, s, seed, idx, kStreamInitQ, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return cleanup(fail(CUMF_ERR_HIP, "init: %s", hipGetErrorString(e)));
+    b->probs.push_back(p);
+    if (index) *index = (int32_t)idx;
+    return CUMF_OK;
+}
+
+int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_t e1)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!o) return fail(CUMF_ERR_ARG, "null options");
+    if (e0 < 0 || e1 < e0 || e1 > 1 << 20) return fail(CUMF_ERR_ARG, "bad epoch range");
+    if (o->schedule != CUMF_SCHED_HOGWILD && o->schedule != CUMF_SCHED_SERIAL)
+        return fail(CUMF_ERR_ARG, "batched training supports the hogwild and serial schedules");
+    if (o->coherent) return fail(CUMF_ERR_ARG, "batched training has no coherent mode");
+    const int L = o->batch_len ? o->batch_len : 128;
+    if (L < 2 || L > kSgdMaxBatch || L % 2) return fail(CUMF_ERR_ARG, "batch_len must be even and <= %d", kSgdMaxBatch);
+    int D = o->pipeline_depth ? o->pipeline_depth : 4;
+    if (D != 2 && D != 4 && D != 8) return fail(CUMF_ERR_ARG, "pipeline_depth must be 2, 4 or 8");
+    while (L % D) D >>= 1;
+    if (b->probs.empty() || e1 == e0) return CUMF_OK;
+
+    // learning-rate table, singleGPU/sgd_kernel.cu:137-154
+    std::vector<float> lr((size_t)e1 + 4);
+    for (size_t i = 0; i < lr.size(); ++i) lr[i] = (float)(o->alpha / (1 + o->beta * std::pow((double)i, 1.5)) + o->lrate);
+    if ((int)lr.size() > b->lr_n) {
+        dev_free(b->lr);
+        b->lr = nullptr;
+        b->lr_n = 0;
+        if ((rc = dev_alloc((void**)&b->lr, sizeof(float) * lr.size()))) return rc;
+        b->lr_n = (int)lr.size();
+    }
+    BHIP_TRY(hipMemcpyAsync(b->lr, lr.data(), sizeof(float) * lr.size(), hipMemcpyHostToDevice, b->stream));
+
+    // waves per problem in proportion to its bytes of work; no more workers
+    // than windows
+    const bool serial = o->schedule == CUMF_SCHED_SERIAL;
+    const int64_t total_waves = o->num_workers > 0 ? o->num_workers : (int64_t)b->cu_count * 16;
+    double cost_sum = 0;
+    for (const Problem& p : b->probs) cost_sum += (double)p.nnz * (12.0 + 4.0 * p.k * eb_of(b->prec));
+    std::vector<SgdTask> tasks;
+    int64_t wave = 0, upd = 0;
+    for (size_t i = 0; i < b->probs.size(); ++i) {
+        const Problem& p = b->probs[i];
+        if (p.nnz == 0) continue;
+        const int G = sgd_batch_group(p.k, b->prec), S = 64 / G;
+        const double cost = (double)p.nnz * (12.0 + 4.0 * p.k * eb_of(b->prec));
+        int64_t waves = serial ? 1 : std::max<int64_t>(1, (int64_t)std::llround(total_waves * cost / cost_sum));
+        // at least 16 windows per worker and epoch: a small problem runs on
+        // few workers (near-serial, like its serial solve) instead of many
+        // concurrent ones on a handful of rows
+        int64_t workers = serial ? 1 : std::max<int64_t>(1, std::min<int64_t>(waves * S, p.nnz / (16LL * L)));
+        waves = (workers + S - 1) / S;
+        SgdTask t{};
+        t.u = p.u; t.v = p.v; t.r = p.r; t.nnz = p.nnz;
+        t.P = p.P; t.Q = p.Q; t.ps = p.ps; t.qs = p.qs; t.lr = b->lr;
+        t.lambda_p = p.lp; t.lambda_q = p.lq;
+        t.worker_begin = (int32_t)wave;
+        t.wave_count = (int32_t)waves;
+        t.worker_count = (int32_t)workers;
+        t.update_count = (int32_t)((p.nnz + workers * L - 1) / (workers * L));
+        t.problem = (uint32_t)i;
+        t.tag = 0;
+        t.k = p.k;
+        tasks.push_back(t);
+        wave += waves;
+        upd += workers * t.update_count * L;
+    }
+    if (tasks.empty()) return CUMF_OK;
+    if (wave > (int64_t)1 << 30) return fail(CUMF_ERR_ARG, "too many waves");
+    if (tasks.size() > b->tasks_n) {
+        dev_free(b->tasks);
+        b->tasks = nullptr;
+        b->tasks_n = 0;
+        if ((rc = dev_alloc((void**)&b->tasks, sizeof(SgdTask) * tasks.size()))) return rc;
+        b->tasks_n = tasks.size();
+    }
+    BHIP_TRY(hipMemcpyAsync(b->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, b->stream));
+    const int blocks = (int)((wave + kSgdWavesPerBlock - 1) / kSgdWavesPerBlock);
+    BHIP_TRY(hipEventRecord(b->ev0, b->stream));
+    for (int ep = e0; ep < e1; ++ep) {
+        SgdLaunch a{};
+        a.tasks = b->tasks;
+        a.num_tasks = (int32_t)tasks.size();
+        a.epoch_begin = ep;
+        a.epoch_end = ep + 1;
+        a.batch_len = L;
+        a.seed = o->seed;
+        a.num_xcd = 1;
+        BHIP_TRY(launch_sgd_batched(a, blocks, b->prec, D, b->stream));
+        if (b->prec == CUMF_FP8_E4M3)
+            for (const Problem& p : b->probs) {
+                BHIP_TRY(launch_fp8_rescale((uint8_t*)p.P, p.ps, p.m, p.k, b->stream));
+                BHIP_TRY(launch_fp8_rescale((uint8_t*)p.Q, p.qs, p.n, p.k, b->stream));
+            }
+        b->launches += 1;
+        b->updates += upd;
+    }
+    BHIP_TRY(hipEventRecord(b->ev1, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0;
+    BHIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->kernel_ms += ms;
+    return CUMF_OK;
+}
+
+int cumf_batch_test_rmse(cumf_batch* b, int32_t index, float mult, float bias, double* rmse)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    Problem* p;
+    if ((rc = problem_of(b, index, &p))) return rc;
+    if (!rmse) return fail(CUMF_ERR_ARG, "null output");
+    if (p->tnnz == 0) {
+        *rmse = 0.0;
+        return CUMF_OK;
+    }
+    const size_t need = (size_t)rmse_blocks(p->tnnz) + 1;
+    if (need > b->partials_n) {
+        dev_free(b->partials);
+        b->partials = nullptr;
+        b->partials_n = 0;
+        if ((rc = dev_alloc((void**)&b->partials, sizeof(double) * need))) return rc;
+        b->partials_n = need;
+    }
+    double* out = b->partials + need - 1;
+    BHIP_TRY(launch_rmse(p->tu, p->tv, p->tr, p->tnnz, p->P, p->ps, p->Q, p->qs, p->m, p->n, p->k, b->prec, mult,
+                         bias, b->partials, out, b->stream));
+    double h = 0;
+    BHIP_TRY(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    *rmse = std::sqrt(h / (double)p->tnnz);
+    return CUMF_OK;
+}
+
+int cumf_batch_get_factors(cumf_batch* b, int32_t index, int32_t which, void* data, float* scales)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    Problem* p;
+    if ((rc = problem_of(b, index, &p))) return rc;
+    if (which != 0 && which != 1) return fail(CUMF_ERR_ARG, "which must be 0 (P) or 1 (Q)");
+    const int64_t rows = which ? p->n : p->m;
+    if (data)
+        BHIP_TRY(hipMemcpyAsync(data, which ? p->Q : p->P, (size_t)rows * p->k * eb_of(b->prec), hipMemcpyDefault,
+                                b->stream));
+    if (scales && b->prec == CUMF_FP8_E4M3)
+        BHIP_TRY(hipMemcpyAsync(scales, which ? p->qs : p->ps, sizeof(float) * rows, hipMemcpyDefault, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    return CUMF_OK;
+}
+
+int cumf_batch_kernel_time(cumf_batch* b, double* ms, int64_t* launches, int64_t* updates)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (ms) *ms = b->kernel_ms;
+    if (launches) *launches = b->launches;
+    if (updates) *updates = b->updates;
+    return CUMF_OK;
+}
+
+}  // extern "C"

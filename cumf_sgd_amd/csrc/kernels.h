@@ -25,6 +25,8 @@ struct SgdTask {
     int32_t update_count;     // windows per worker per epoch
     uint32_t problem;         // Philox key word (batched problems)
     uint32_t tag;             // block id folded into the window counter
+    int32_t k;                // factor length (batched launches dispatch on it)
+    int32_t wave_count;       // batched launches: waves owned by the task
     int32_t pad;
 };
 
@@ -46,6 +48,9 @@ bool sgd_shape_supported(int k, int prec, int G);
 int sgd_default_group(int k, int prec);
 // D: software-pipeline depth (2, 4 or 8; must divide the batch length)
 hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, int D, hipStream_t s);
+// Batched launch: tasks of mixed k (32, 64, 128), lane group k/2 (k/4 for FP8).
+hipError_t launch_sgd_batched(const SgdLaunch& a, int blocks, int prec, int D, hipStream_t s);
+int sgd_batch_group(int k, int prec);
 
 // *p = value, and *p += 1 (device-side epoch counter of a captured epoch loop)
 hipError_t launch_set_int(int32_t* p, int32_t value, hipStream_t st);

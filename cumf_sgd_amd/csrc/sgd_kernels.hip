@@ -48,6 +48,27 @@ hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bo
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_sgd_batched(const SgdLaunch& a, int blocks, int prec, int D, hipStream_t s)
+{
+    switch (prec) {
+    case kF32: return launch_batched_for<kF32>(a, blocks, D, s);
+    case kF16: return launch_batched_for<kF16>(a, blocks, D, s);
+    case kBF16: return launch_batched_for<kBF16>(a, blocks, D, s);
+    case kE4M3: return launch_batched_for<kE4M3>(a, blocks, D, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+int sgd_batch_group(int k, int prec)
+{
+    switch (prec) {
+    case kF32: return batch_group<kF32>(k);
+    case kF16: return batch_group<kF16>(k);
+    case kBF16: return batch_group<kBF16>(k);
+    default: return batch_group<kE4M3>(k);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Device-side epoch counter.
 // ---------------------------------------------------------------------------

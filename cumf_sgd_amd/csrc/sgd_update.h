@@ -67,10 +67,11 @@ struct Pending {
     float r, ps, qs;
 };
 
+// One worker (a slot of G lanes) running its windows of task T.
 template <int K, int G, int PREC, bool COH, int D>
-__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
+__device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int ls, int32_t* su,
+                                           int32_t* sv, float* sr)
 {
-    constexpr int S = 64 / G;
     constexpr int E = K / G;
     constexpr int EB = PrecBytes<PREC>::value;
     constexpr int TB = E * EB;
@@ -80,31 +81,9 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
     static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
     static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
 
-    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
-    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
-    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
-
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int slot = lane / G;
-    const int ls = lane % G;
-
-    int64_t blk = blockIdx.x;
-    if (a.num_xcd > 1) blk = (int64_t)(blockIdx.x % a.num_xcd) * (gridDim.x / a.num_xcd) + blockIdx.x / a.num_xcd;
-    const int worker = (int)((blk * kWavesPerBlock + wave) * S + slot);
-
-    int t = 0;
-    while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
-    if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
-    const SgdTask T = a.tasks[t];
     if (T.nnz <= 0 || T.update_count <= 0) return;
-    const uint32_t lw = (uint32_t)(worker - T.worker_begin);
     const Key key = make_key(a.seed, T.problem);
     const int L = a.batch_len;   // host guarantees 1 <= L <= kMaxBatch and L % D == 0
-
-    int32_t* su = s_u[wave][slot];
-    int32_t* sv = s_v[wave][slot];
-    float* sr = s_r[wave][slot];
 
     gchar* Pb = to_global((char*)T.P) + (int64_t)ls * TB;
     gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * TB;
@@ -269,6 +248,96 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
 }
 
 template <int K, int G, int PREC, bool COH, int D>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
+{
+    constexpr int S = 64 / G;
+    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
+    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
+    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / G;
+    const int ls = lane % G;
+
+    int64_t blk = blockIdx.x;
+    if (a.num_xcd > 1) blk = (int64_t)(blockIdx.x % a.num_xcd) * (gridDim.x / a.num_xcd) + blockIdx.x / a.num_xcd;
+    const int worker = (int)((blk * kWavesPerBlock + wave) * S + slot);
+
+    int t = 0;
+    while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
+    if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
+    const SgdTask T = a.tasks[t];
+    sgd_worker<K, G, PREC, COH, D>(a, T, (uint32_t)(worker - T.worker_begin), ls, s_u[wave][slot], s_v[wave][slot],
+                                   s_r[wave][slot]);
+}
+
+// Lane group of a factor length in batched launches (one per k).
+template <int PREC>
+__host__ __device__ constexpr int batch_group(int k)
+{
+    return PREC == kE4M3 ? k / 4 : (k / 2 > 64 ? 64 : k / 2);
+}
+
+// Many independent problems in one launch.  Tasks own whole waves: task t
+// holds waves [worker_begin, worker_begin + wave_count) (worker_begin and
+// wave_count counted in waves here) and its first worker_count workers are
+// active; each wave splits into 64/G slots for the task's k.
+template <int PREC, bool COH, int D>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLaunch a)
+{
+    constexpr int SMAX = 64 / batch_group<PREC>(32);
+    __shared__ int32_t s_u[kWavesPerBlock][SMAX * kMaxBatch];
+    __shared__ int32_t s_v[kWavesPerBlock][SMAX * kMaxBatch];
+    __shared__ float s_r[kWavesPerBlock][SMAX * kMaxBatch];
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * kWavesPerBlock + wave;
+    int t = 0;
+    while (t < a.num_tasks && gw >= a.tasks[t].worker_begin + a.tasks[t].wave_count) ++t;
+    if (t >= a.num_tasks || gw < a.tasks[t].worker_begin) return;
+    const SgdTask T = a.tasks[t];
+    const int wl = gw - T.worker_begin;
+#define CUMF_BATCH_K(KK)                                                                                 \
+    case KK: {                                                                                           \
+        constexpr int G = batch_group<PREC>(KK);                                                         \
+        const int slot = lane / G, ls = lane % G;                                                        \
+        const uint32_t lw = (uint32_t)(wl * (64 / G) + slot);                                            \
+        if (lw >= (uint32_t)T.worker_count) return;                                                      \
+        sgd_worker<KK, G, PREC, COH, D>(a, T, lw, ls, &s_u[wave][slot * kMaxBatch], &s_v[wave][slot * kMaxBatch], \
+                                        &s_r[wave][slot * kMaxBatch]);                                   \
+        return;                                                                                          \
+    }
+    switch (T.k) {
+        CUMF_BATCH_K(32)
+        CUMF_BATCH_K(64)
+        CUMF_BATCH_K(128)
+    default:
+        return;
+    }
+#undef CUMF_BATCH_K
+}
+
+template <int PREC, bool COH, int D>
+inline hipError_t launch_batched_t(const SgdLaunch& a, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((sgd_batched_kernel<PREC, COH, D>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int PREC>
+inline hipError_t launch_batched_prec(const SgdLaunch& a, int blocks, int D, hipStream_t s)
+{
+    switch (D) {
+    case 2: return launch_batched_t<PREC, false, 2>(a, blocks, s);
+    case 4: return launch_batched_t<PREC, false, 4>(a, blocks, s);
+    case 8: return launch_batched_t<PREC, false, 8>(a, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int K, int G, int PREC, bool COH, int D>
 inline hipError_t launch_sgd_t(const SgdLaunch& a, int blocks, hipStream_t s)
 {
     hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH, D>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
@@ -325,5 +394,7 @@ inline hipError_t launch_sgd_g(const SgdLaunch& a, int blocks, int G, bool coh, 
 // Launch for one factor format; defined in sgd_update_<format>.hip.
 template <int PREC>
 hipError_t launch_sgd_prec(const SgdLaunch& a, int blocks, int k, int G, bool coh, int D, hipStream_t s);
+template <int PREC>
+hipError_t launch_batched_for(const SgdLaunch& a, int blocks, int D, hipStream_t s);
 
 }  // namespace cumf

@@ -14,4 +14,10 @@ hipError_t launch_sgd_prec<kE4M3>(const SgdLaunch& a, int blocks, int k, int G, 
     return hipErrorInvalidValue;
 }
 
+template <>
+hipError_t launch_batched_for<kE4M3>(const SgdLaunch& a, int blocks, int D, hipStream_t s)
+{
+    return launch_batched_prec<kE4M3>(a, blocks, D, s);
+}
+
 }  // namespace cumf

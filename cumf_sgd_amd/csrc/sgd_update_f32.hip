@@ -14,4 +14,10 @@ hipError_t launch_sgd_prec<kF32>(const SgdLaunch& a, int blocks, int k, int G, b
     return hipErrorInvalidValue;
 }
 
+template <>
+hipError_t launch_batched_for<kF32>(const SgdLaunch& a, int blocks, int D, hipStream_t s)
+{
+    return launch_batched_prec<kF32>(a, blocks, D, s);
+}
+
 }  // namespace cumf

@@ -155,3 +155,44 @@ def write_triplets(path: str, u, v, r) -> None:
     rec = np.empty(len(u), dtype=[("u", "<i4"), ("v", "<i4"), ("r", "<f4")])
     rec["u"], rec["v"], rec["r"] = u, v, r
     rec.tofile(path)
+
+
+@dataclass
+class Problem:
+    m: int
+    n: int
+    k: int
+    u: torch.Tensor
+    v: torch.Tensor
+    r: torch.Tensor
+    tu: torch.Tensor
+    tv: torch.Tensor
+    tr: torch.Tensor
+
+
+def batched(seed: int = 4, count: int = 256, device="cpu", users=(1_000, 200_000), items=(500, 50_000),
+            density=(0.001, 0.02), ks=(32, 64, 128), test_frac: float = 0.1) -> list:
+    """Batched: `count` independent problems, users / items / density each
+    log-uniform over the given ranges, k drawn from `ks`, ratings 1..5 from a
+    rank-16 score + N(0, 0.5^2) noise as in the small config, uniform ids,
+    random 90/10 splits."""
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+
+    def logu(lo, hi):
+        return float(np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * torch.rand(1, generator=gen).item()))
+
+    out = []
+    for i in range(count):
+        m = int(round(logu(*users)))
+        n = int(round(logu(*items)))
+        d = logu(*density)
+        k = int(ks[int(torch.randint(0, len(ks), (1,), generator=gen).item())])
+        nnz = max(20, int(round(m * n * d)))
+        g2 = torch.Generator(device=device).manual_seed(seed * 100_003 + i)
+        u = torch.randint(0, m, (nnz,), generator=g2, device=device, dtype=torch.int32)
+        v = torch.randint(0, n, (nnz,), generator=g2, device=device, dtype=torch.int32)
+        s = _scores(g2, u, v, m, n, 16, device) / 4.0
+        r = torch.clamp(torch.round(3.6 + s + torch.randn(nnz, generator=g2, device=device) * 0.5), 1, 5)
+        tu_, tv_, tr_, eu, ev, er = _split(g2, u, v, r.to(torch.float32), int(round(nnz * test_frac)), device)
+        out.append(Problem(m, n, k, tu_, tv_, tr_, eu, ev, er))
+    return out

@@ -164,6 +164,30 @@ typedef struct cumf_launch_info {
 } cumf_launch_info;
 int cumf_last_launch(cumf_ctx* ctx, cumf_launch_info* info);
 
+/* ---- batched problems ----------------------------------------------------
+ * Many independent factorizations of uneven size and mixed k trained
+ * together, one launch per epoch (no counterpart in the reference, which
+ * trains one problem per process).  Each problem's random streams are keyed
+ * by (seed, problem index). */
+typedef struct cumf_batch cumf_batch;
+int cumf_batch_create(int device, int32_t precision, cumf_batch** out);
+int cumf_batch_destroy(cumf_batch* batch);
+/* Add a problem: training ratings (ids in [0,m) x [0,n)), optional test
+ * ratings, k in {32, 64, 128}, lambdas already divided by the problem's
+ * rating scale.  Factors are initialised as cumf_init_factors does. */
+int cumf_batch_add(cumf_batch* batch, const int32_t* u, const int32_t* v, const float* r, int64_t nnz,
+                   int32_t m, int32_t n, int32_t k, float lambda_p, float lambda_q, const int32_t* tu,
+                   const int32_t* tv, const float* tr, int64_t test_nnz, uint64_t seed, int32_t* index);
+/* Epochs [epoch_begin, epoch_end) of every problem.  opts: schedule
+ * (CUMF_SCHED_HOGWILD, or CUMF_SCHED_SERIAL = one worker per problem),
+ * num_workers = total waves (0 = 16 per CU, shared in proportion to each
+ * problem's bytes of work), alpha, beta, lrate, seed, batch_len,
+ * pipeline_depth; k and the lambdas come from each problem. */
+int cumf_batch_train(cumf_batch* batch, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end);
+int cumf_batch_test_rmse(cumf_batch* batch, int32_t index, float mult, float b, double* rmse);
+int cumf_batch_get_factors(cumf_batch* batch, int32_t index, int32_t which, void* data, float* scales);
+int cumf_batch_kernel_time(cumf_batch* batch, double* ms, int64_t* launches, int64_t* updates);
+
 /* ---- reference-seam entry points ---------------------------------------- */
 /* Mirror of Parameter (singleGPU/sgd.h:26-46). */
 typedef struct cumf_param {

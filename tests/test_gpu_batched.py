@@ -1,0 +1,91 @@
+"""GPU tests of batched training (many independent problems, mixed k, one
+launch per epoch) against per-problem serial CPU oracle solves."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(count, seed=11, scale=0.05):
+    from cumf_sgd_amd import datagen
+
+    return datagen.batched(seed=seed, count=count, users=(300, 3000), items=(100, 800), density=(0.01, 0.05))
+
+
+def _oracle_group(k, prec):
+    return k // 4 if prec == 3 else min(64, k // 2)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_batched_serial_matches_oracle(gpu, prec):
+    """One worker per problem: each problem's factors equal the serial oracle
+    keyed by its problem index, bit for bit."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    probs = _problems(6)
+    seed, lam = 21, 0.05
+    with cs.Batch(prec) as b:
+        init = []
+        for p in probs:
+            u, v, r = (t.numpy() for t in (p.u, p.v, p.r))
+            idx = b.add(u, v, r, p.m, p.n, p.k, lam, seed=seed)
+            init.append((b.get_factors(idx, 0), b.get_factors(idx, 1)))
+        b.train(cs.make_opts(schedule=cs.SERIAL, seed=seed), 0, 2)
+        lr = oracle.lr_table(0.08, 0.3, 0.0, 6)
+        for i, p in enumerate(probs):
+            (P0, ps0), (Q0, qs0) = init[i]
+            wantP0, wps = oracle.init_factors(p.m, p.k, prec, seed, 0, problem=i)
+            assert np.array_equal(P0.view(np.uint8), wantP0.view(np.uint8))
+            u, v, r = (t.numpy() for t in (p.u, p.v, p.r))
+            Pw, Qw, psw, qsw = P0, Q0, ps0, qs0
+            for ep in range(2):
+                Pw, Qw, psw, qsw = oracle.sgd_serial(u, v, r, Pw, Qw, lr, ep, ep + 1, k=p.k,
+                                                     G=_oracle_group(p.k, prec), prec=prec, lambda_p=lam,
+                                                     lambda_q=lam, seed=seed, ps=psw, qs=qsw, problem=i)
+                if prec == 3:
+                    Pw, psw = oracle.fp8_rescale(Pw, psw)
+                    Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+            P1, _ = b.get_factors(i, 0)
+            Q1, _ = b.get_factors(i, 1)
+            assert np.array_equal(P1.view(np.uint8), Pw.view(np.uint8)), (i, p.k)
+            assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8)), (i, p.k)
+
+
+def test_batched_hogwild_rmse_vs_serial_fp32(gpu):
+    """24 problems of mixed k in BF16, 20 Hogwild epochs in one launch per
+    epoch: each problem's test RMSE is within max(0.01, 1%) of its own serial
+    FP32 oracle solve at the same epoch count."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    probs = _problems(24, seed=12)
+    seed, epochs = 31, 20
+    with cs.Batch(cs.BF16) as b:
+        scales = []
+        for p in probs:
+            u, v, r, tu, tv, tr = (t.numpy() for t in (p.u, p.v, p.r, p.tu, p.tv, p.tr))
+            sd = np.float32(np.sqrt((r.astype(np.float64) ** 2).mean() - r.astype(np.float64).mean() ** 2))
+            sc = max(np.float32(1e-4), sd)
+            scales.append(sc)
+            rs = (r * np.float32(1.0 / sc)).astype(np.float32)
+            b.add(u, v, rs, p.m, p.n, p.k, np.float32(0.05) / sc, tu=tu, tv=tv, tr=tr, seed=seed)
+        b.train(cs.make_opts(schedule=cs.HOGWILD, num_workers=512, seed=seed), 0, epochs)
+        lr = oracle.lr_table(0.08, 0.3, 0.0, epochs + 4)
+        worst = 0.0
+        for i, p in enumerate(probs):
+            u, v, r, tu, tv, tr = (t.numpy() for t in (p.u, p.v, p.r, p.tu, p.tv, p.tr))
+            sc = scales[i]
+            fs = np.float32(np.sqrt(sc))
+            got = b.test_rmse(i, fs, 0.0)
+            rs = (r * np.float32(1.0 / sc)).astype(np.float32)
+            P, _ = oracle.init_factors(p.m, p.k, 0, seed, 0, problem=i)
+            Q, _ = oracle.init_factors(p.n, p.k, 0, seed, 1, problem=i)
+            P, Q, _, _ = oracle.sgd_serial(u, v, rs, P, Q, lr, 0, epochs, k=p.k, G=_oracle_group(p.k, 0), prec=0,
+                                           lambda_p=np.float32(0.05) / sc, lambda_q=np.float32(0.05) / sc, seed=seed,
+                                           problem=i)
+            want = oracle.rmse(tu, tv, tr, (P * fs).astype(np.float32), (Q * fs).astype(np.float32), 0.0)
+            worst = max(worst, got - want)
+            print(f"problem {i}: m={p.m} n={p.n} k={p.k} nnz={p.u.numel()} gpu={got:.4f} oracle={want:.4f}")
+            assert got <= want + max(0.01, 0.01 * want)
+        print(f"worst gap {worst:+.4f}")

@@ -287,7 +287,8 @@ def test_graph_epoch_loop_matches_eager(gpu, prec, sched):
                 opts = cs.make_opts(k=k, schedule=cs.STRATUM, num_workers=8 * 4 * 2, seed=5, use_graph=graph,
                                     lanes_per_sample=32)
             else:
-                opts = cs.make_opts(k=k, schedule=cs.SERIAL, seed=5, use_graph=graph, lanes_per_sample=32)
+                opts = cs.make_opts(k=k, schedule=cs.SERIAL, seed=5, use_graph=graph,
+                                    lanes_per_sample=16 if prec == 3 else 32)
             c.init_factors(k, prec, seed=5)
             out = []
             for ep in range(3):
