@@ -215,6 +215,13 @@ def main() -> int:
             }
         ttt["target_rmse"] = round(target, 5)
         ttt["target_rule"] = f"FP32 GPU test RMSE after {args.rmse_epochs} epochs + 0.005"
+        gold = os.path.join(ROOT, "tests", "golden", "medium_oracle_rmse.json")
+        if os.path.exists(gold) and args.seed == 2 and args.rmse_epochs <= 20:
+            g = json.load(open(gold))
+            ref64 = g["fp64"][args.rmse_epochs - 1]
+            ttt["fp64_oracle_rmse"] = round(ref64, 5)
+            for p in curves:
+                ttt[PREC_NAME[p]]["gap_to_fp64_oracle"] = round(curves[p][1][-1] - ref64, 5)
 
     # ---- CPU baseline (rank 0, N=1 only)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -102,6 +102,9 @@ def _load():
         "cumf_set_test": ([P, P, I64], ctypes.c_int),
         "cumf_set_test_soa": ([P, P, P, P, I64], ctypes.c_int),
         "cumf_get_ratings": ([P, P, P, P], ctypes.c_int),
+        "cumf_alloc_ratings": ([P, I64, I32, I32, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)],
+                               ctypes.c_int),
+        "cumf_commit_ratings": ([P], ctypes.c_int),
         "cumf_rating_moments": ([P, D, D], ctypes.c_int),
         "cumf_remap": ([P, P, P], ctypes.c_int),
         "cumf_scale_ratings": ([P, F32], ctypes.c_int),
@@ -220,6 +223,28 @@ class Context:
         nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
         _check(lib.cumf_set_ratings(self._h, _ptr(nodes), len(nodes), m, n), "cumf_set_ratings")
         self.m, self.n, self.nnz = m, n, len(nodes)
+
+    def alloc_ratings(self, nnz: int, m: int, n: int):
+        """The context's own device rating arrays, as torch tensors that alias
+        them (fill, then call commit_ratings)."""
+        import torch
+
+        pu, pv, pr = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib.cumf_alloc_ratings(self._h, nnz, m, n, ctypes.byref(pu), ctypes.byref(pv), ctypes.byref(pr)),
+               "cumf_alloc_ratings")
+        self.m, self.n, self.nnz = m, n, nnz
+
+        class _Dev:
+            def __init__(self, ptr, typestr):
+                self.__cuda_array_interface__ = {"shape": (nnz,), "typestr": typestr, "data": (ptr, False),
+                                                 "version": 3, "strides": None}
+
+        return (torch.as_tensor(_Dev(pu.value, "<i4"), device="cuda"),
+                torch.as_tensor(_Dev(pv.value, "<i4"), device="cuda"),
+                torch.as_tensor(_Dev(pr.value, "<f4"), device="cuda"))
+
+    def commit_ratings(self) -> None:
+        _check(lib.cumf_commit_ratings(self._h), "cumf_commit_ratings")
 
     def set_test(self, u, v, r) -> None:
         _check(lib.cumf_set_test_soa(self._h, _ptr(u), _ptr(v), _ptr(r), len(u)), "cumf_set_test_soa")

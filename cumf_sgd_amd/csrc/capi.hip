@@ -307,6 +307,33 @@ int cumf_set_ratings_soa(cumf_ctx* c, const int32_t* u, const int32_t* v, const 
     return after_ratings(c, nnz, m, n);
 }
 
+int cumf_alloc_ratings(cumf_ctx* c, int64_t nnz, int32_t m, int32_t n, int32_t** u, int32_t** v, float** r)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 0 || m <= 0 || n <= 0 || !u || !v || !r) return fail(CUMF_ERR_ARG, "bad arguments");
+    c->have_ratings = false;
+    if ((rc = ensure(&c->u, &c->u_cap, (size_t)nnz)) || (rc = ensure(&c->v, &c->v_cap, (size_t)nnz)) ||
+        (rc = ensure(&c->r, &c->r_cap, (size_t)nnz)))
+        return rc;
+    c->nnz = nnz;
+    c->m = m;
+    c->n = n;
+    *u = c->u;
+    *v = c->v;
+    *r = c->r;
+    return CUMF_OK;
+}
+
+int cumf_commit_ratings(cumf_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->u && c->nnz) return fail(CUMF_ERR_STATE, "no rating buffers allocated");
+    HIP_TRY(hipDeviceSynchronize());   // the caller may have filled them on any stream
+    return after_ratings(c, c->nnz, c->m, c->n);
+}
+
 int cumf_set_test(cumf_ctx* c, const cumf_node* R, int64_t nnz)
 {
     int rc = check_ctx(c);

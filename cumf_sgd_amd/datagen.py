@@ -196,3 +196,65 @@ def batched(seed: int = 4, count: int = 256, device="cpu", users=(1_000, 200_000
         tu_, tv_, tr_, eu, ev, er = _split(g2, u, v, r.to(torch.float32), int(round(nnz * test_frac)), device)
         out.append(Problem(m, n, k, tu_, tv_, tr_, eu, ev, er))
     return out
+
+
+# 11 bin masses of the large sparse config's values 0, 10, ..., 100: a mode at
+# 0 and a second mode at 90.
+BIMODAL_MASSES = (0.16, 0.04, 0.03, 0.03, 0.04, 0.06, 0.08, 0.11, 0.15, 0.18, 0.12)
+
+
+def large_sparse(seed: int = 3, device="cuda", m: int = 1_000_990, n: int = 624_961, nnz: int = 252_800_275,
+                 test_frac: float = 0.01, chunk: int = 1 << 25) -> Ratings:
+    """Large sparse: 1,000,990 x 624,961, 252,800,275 ratings, Zipf(1.0) ranks
+    on both axes over random id orders, float32 values in {0, 10, ..., 100}
+    mapped by quantiles of a rank-16 score + noise onto BIMODAL_MASSES, 1%
+    random test split.  Pairs may repeat."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    pu = torch.randperm(m, generator=gen, device=device).to(torch.int32)
+    pv = torch.randperm(n, generator=gen, device=device).to(torch.int32)
+    u = torch.empty(nnz, dtype=torch.int32, device=device)
+    v = torch.empty(nnz, dtype=torch.int32, device=device)
+    for s in range(0, nnz, chunk):
+        e = min(nnz, s + chunk)
+        u[s:e] = pu[_zipf_sample(gen, m, 1.0, e - s, device)]
+        v[s:e] = pv[_zipf_sample(gen, n, 1.0, e - s, device)]
+    sc = _scores(gen, u, v, m, n, 16, device) / 4.0
+    sc += torch.randn(nnz, generator=gen, device=device) * 0.5
+    probe = sc[torch.randint(0, nnz, (1 << 20,), generator=gen, device=device)]
+    cum = torch.tensor(np.cumsum(BIMODAL_MASSES)[:-1], dtype=torch.float64, device=device)
+    cuts = torch.quantile(probe.double(), cum)
+    r = torch.empty(nnz, dtype=torch.float32, device=device)
+    for s in range(0, nnz, chunk):
+        e = min(nnz, s + chunk)
+        r[s:e] = torch.bucketize(sc[s:e].double(), cuts).to(torch.float32) * 10.0
+    del sc
+    tu_, tv_, tr_, eu, ev, er = _split(gen, u, v, r, int(round(nnz * test_frac)), device)
+    return Ratings(m, n, tu_, tv_, tr_, eu, ev, er)
+
+
+def tall_into(u, v, r, m: int, n: int, seed: int = 5, rank: int = 32, chunk: int = 1 << 26, test=None):
+    """Fill device arrays u, v, r (length nnz) with the tall-skinny config:
+    users uniform over m, items Zipf(1.1) over a random order of n, values
+    exp(z) with z = U_u.V_v / sqrt(rank) (unit variance, U, V ~ N(0,1)^rank),
+    i.e. log-normal(0, 1).  `test` = (tu, tv, tr) arrays are filled with
+    further ratings of the same ground truth."""
+    device = u.device
+    gen = torch.Generator(device=device).manual_seed(seed)
+    pv = torch.randperm(n, generator=gen, device=device).to(torch.int32)
+    U = torch.randn(m, rank, generator=gen, device=device, dtype=torch.float32)
+    V = torch.randn(n, rank, generator=gen, device=device, dtype=torch.float32)
+    w = torch.arange(1, n + 1, dtype=torch.float64, device=device).pow(-1.1)
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    inv = 1.0 / np.sqrt(rank)
+    for (a, b, c) in ((u, v, r),) + ((tuple(test),) if test is not None else ()):
+        nnz = a.numel()
+        for s in range(0, nnz, chunk):
+            e = min(nnz, s + chunk)
+            uu = torch.randint(0, m, (e - s,), generator=gen, device=device, dtype=torch.int32)
+            x = torch.rand(e - s, generator=gen, dtype=torch.float64, device=device)
+            vv = pv[torch.searchsorted(cdf, x).clamp_(max=n - 1)]
+            a[s:e] = uu
+            b[s:e] = vv
+            c[s:e] = torch.exp((U[uu.long()] * V[vv.long()]).sum(1) * inv)
+    del U, V

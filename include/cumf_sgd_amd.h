@@ -98,6 +98,13 @@ int cumf_set_ratings_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, cons
 int cumf_set_test(cumf_ctx* ctx, const cumf_node* R, int64_t nnz);
 int cumf_set_test_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, const float* r, int64_t nnz);
 
+/* Zero-copy loading of very large rating sets: allocate the context's three
+ * rating arrays (nnz entries each) and return their device addresses; the
+ * caller fills them on the device (any stream), then commits, which waits for
+ * the device and checks every id like cumf_set_ratings_soa. */
+int cumf_alloc_ratings(cumf_ctx* ctx, int64_t nnz, int32_t m, int32_t n, int32_t** u, int32_t** v, float** r);
+int cumf_commit_ratings(cumf_ctx* ctx);
+
 /* Device copies of the ratings (3 arrays of nnz), e.g. for a checker. */
 int cumf_get_ratings(cumf_ctx* ctx, int32_t* u, int32_t* v, float* r);
 

@@ -3,8 +3,9 @@
 Runs the FP32 oracle (the kernel's arithmetic order, one worker) and the FP64
 oracle for 20 epochs on exactly the data and preprocessing the benchmark uses
 (host-generated with seed 2, srand(2) id maps, scale = std of the ratings),
-and writes the test RMSE after every epoch to medium_oracle_rmse_<fp32|fp64>.json.
-Takes about 40 minutes on one core per precision; run:
+and writes the test RMSE after every epoch into medium_oracle_rmse.json.
+Takes 20-35 minutes on one core per precision; run the two one after the
+other:
     python tests/golden/make_medium_oracle.py [fp32|fp64]
 """
 import json
@@ -63,8 +64,9 @@ def main(which: str) -> None:
             Pe, Qe = (P * float(fs)).astype(np.float32), (Q * float(fs)).astype(np.float32)
         curve.append(oracle.rmse(tu, tv, tr, Pe, Qe, b))
         print(f"{which} epoch {ep}: rmse {curve[-1]:.5f} ({time.time() - t0:.0f}s)", flush=True)
-    path = os.path.join(HERE, f"medium_oracle_rmse_{which}.json")
-    data = {which: curve}
+    path = os.path.join(HERE, "medium_oracle_rmse.json")
+    data = json.load(open(path)) if os.path.exists(path) else {}
+    data[which] = curve
     data["workload"] = ("medium: 480189 x 17770, 99072112 train + 1408395 test, seed 2, k=128, lambda 0.05, "
                         "alpha 0.08, beta 0.3, serialized single worker, windows of 128")
     json.dump(data, open(path, "w"), indent=1)

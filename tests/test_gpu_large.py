@@ -1,0 +1,152 @@
+"""Full-size GPU runs of the large configurations: properties that hold at
+any size (finite, decreasing test RMSE, every rating covered by the
+partition, throughput reported).  The CPU oracle cannot run these sizes; its
+bit-exact comparisons are made on the small and medium shapes."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _free():
+    import gc
+
+    import torch
+
+    gc.collect()
+    torch.cuda.empty_cache()
+
+
+def _prep(ctx, seed, m, n, nnz):
+    """Statistics, srand(seed) id maps and scaling on the device."""
+    import cumf_sgd_amd as cs
+
+    s1, s2 = ctx.rating_moments()
+    ex, ex2 = s1 / nnz, s2 / nnz
+    std = np.float32(np.sqrt(ex2 - ex * ex))
+    scale = max(np.float32(1e-4), std)
+    ctx.remap(cs.gen_random_map(seed, m), cs.gen_random_map(seed, n))
+    ctx.scale_ratings(np.float32(1.0 / scale))
+    return scale, np.float32(np.float32(ex) / std)
+
+
+def test_large_sparse_formats(gpu):
+    """1,000,990 x 624,961, 252,800,275 ratings (Zipf(1.0) on both axes,
+    bimodal 0..100 values): FP32, BF16 and FP8 storage under the stratum
+    schedule, 6 epochs each; every format trains, and the BF16 / FP8 gaps to
+    FP32 are printed."""
+    import torch
+
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    t0 = time.time()
+    R = datagen.large_sparse(seed=3, device="cuda")
+    torch.cuda.synchronize()
+    print(f"generated {R.nnz} ratings in {time.time() - t0:.1f}s")
+    res = {}
+    with cs.Context(0) as c:
+        c.set_ratings(R.u, R.v, R.r, R.m, R.n)
+        c.set_test(R.tu, R.tv, R.tr)
+        del R
+        _free()
+        scale, b = _prep(c, 3, 1_000_990, 624_961, c.nnz)
+        off = c.partition_strata(8)
+        assert off[-1] == c.nnz and np.all(np.diff(off) >= 0)
+        lam = float(np.float32(0.05) / scale)
+        for prec in (0, 2, 3):
+            c.init_factors(128, prec, 3)
+            o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=lam, seed=3)
+            curve, t = [], 0.0
+            for ep in range(6):
+                t1 = time.perf_counter()
+                c.train(o, ep, ep + 1)
+                t += time.perf_counter() - t1
+                curve.append(c.test_rmse(np.float32(np.sqrt(scale)), b))
+            res[prec] = curve
+            print(f"prec={prec} rmse={[round(x, 3) for x in curve]} {c.nnz * 6 / t / 1e9:.2f}e9 updates/s")
+            assert np.all(np.isfinite(curve)) and curve[-1] < curve[0]
+    print(f"BF16 gap to FP32 {res[2][-1] - res[0][-1]:+.4f}, FP8 gap {res[3][-1] - res[0][-1]:+.4f}")
+    assert res[2][-1] <= res[0][-1] * 1.02
+
+
+def test_tall_skinny_3_1_billion_ratings(gpu):
+    """50,082,604 x 39,780 with 3,101,144,313 ratings (37 GB, generated straight
+    into the context's buffers), FP16, stratum schedule: 64-bit indexing,
+    partition covers every rating, RMSE falls every epoch.
+
+    The learning rate is the reference's default (alpha = beta = 0.01,
+    singleGPU/sgd.h:45): with alpha = 0.08 plain SGD diverges on these
+    log-normal ratings (values up to ~260), in the serial CPU oracle as well."""
+    import torch
+
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    m, n, nnz, ntest = 50_082_604, 39_780, 3_101_144_313, 10_000_000
+    props = torch.cuda.get_device_properties(0)
+    if props.total_memory < 200 * 2 ** 30:
+        pytest.skip("needs a 288 GB device")
+    with cs.Context(0) as c:
+        t0 = time.time()
+        u, v, r = c.alloc_ratings(nnz, m, n)
+        tu = torch.empty(ntest, dtype=torch.int32, device="cuda")
+        tv = torch.empty(ntest, dtype=torch.int32, device="cuda")
+        tr = torch.empty(ntest, dtype=torch.float32, device="cuda")
+        datagen.tall_into(u, v, r, m, n, seed=5, test=(tu, tv, tr))
+        c.commit_ratings()
+        c.set_test(tu, tv, tr)
+        del u, v, r, tu, tv, tr
+        _free()
+        print(f"generated {nnz} ratings in {time.time() - t0:.1f}s")
+        scale, b = _prep(c, 5, m, n, nnz)
+        off = c.partition_strata(8)
+        assert off[-1] == nnz and off[0] == 0 and np.all(np.diff(off) >= 0)
+        c.init_factors(128, cs.FP16, 5)
+        o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=float(np.float32(0.05) / scale), seed=5, alpha=0.01,
+                         beta=0.01)
+        fs = np.float32(np.sqrt(scale))
+        curve = [c.test_rmse(fs, b)]
+        t = 0.0
+        for ep in range(3):
+            t1 = time.perf_counter()
+            c.train(o, ep, ep + 1)
+            t += time.perf_counter() - t1
+            curve.append(c.test_rmse(fs, b))
+        info = c.last_launch()
+        print(f"rmse {[round(x, 4) for x in curve]}, {nnz * 3 / t / 1e9:.2f}e9 updates/s, "
+              f"updates/epoch {info.updates_per_epoch}")
+        assert np.all(np.isfinite(curve)) and np.all(np.diff(curve) < 0) and curve[-1] < curve[0] - 0.1
+        assert info.updates_per_epoch >= nnz
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_medium_hogwild_reaches_serial_oracle_rmse(gpu, prec):
+    """Netflix-shaped 99M-rating workload, default Hogwild launch shape,
+    20 epochs: the test RMSE is within 0.005 of the serialized FP32 CPU
+    oracle's at 20 epochs (tests/golden/medium_oracle_rmse.json, produced on
+    exactly this data and preprocessing)."""
+    import json
+    import os
+
+    import torch
+
+    import bench
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "medium_oracle_rmse.json")))
+    R = datagen.medium(seed=2, device="cpu")
+    R = datagen.Ratings(R.m, R.n, *(t.to("cuda") for t in (R.u, R.v, R.r, R.tu, R.tv, R.tr)))
+    torch.cuda.synchronize()
+    with cs.Context(0) as c:
+        scale, b, lam = bench.prepare(c, R, 2)
+        times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.HOGWILD)
+    del R
+    _free()
+    want = gold["fp32"][19]
+    print(f"prec={prec} gpu={rmses[-1]:.5f} oracle_fp32={want:.5f} oracle_fp64={gold['fp64'][19]:.5f} "
+          f"gap={rmses[-1] - want:+.5f} train_s={sum(times):.3f}")
+    assert rmses[-1] <= want + 0.005
