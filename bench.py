@@ -130,6 +130,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--depth", type=int, default=0, help="pipeline depth (2, 4, 8; 0 = library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="pick lanes / depth / workers by timing trial epochs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -173,6 +174,12 @@ def main() -> int:
     ctx.init_factors(K, prec, args.seed)
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
                         lanes_per_sample=args.lanes, pipeline_depth=args.depth)
+    if args.autotune:
+        t1 = time.perf_counter()
+        opts, tuned_ms = ctx.autotune(opts)
+        log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
+            f"workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
+        args.workers, args.lanes, args.depth = opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth
     for ep in range(args.warmup):
         ctx.train(opts, ep, ep + 1)
     ctx.sync()

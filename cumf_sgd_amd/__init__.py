@@ -119,6 +119,7 @@ def _load():
         "cumf_test_rmse": ([P, F32, F32, D], ctypes.c_int),
         "cumf_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_reset_kernel_time": ([P], ctypes.c_int),
+        "cumf_autotune": ([P, ctypes.POINTER(TrainOpts), ctypes.POINTER(TrainOpts), D], ctypes.c_int),
         "cumf_last_launch": ([P, ctypes.POINTER(LaunchInfo)], ctypes.c_int),
         "cumf_sgd_update_k128": ([ctypes.POINTER(Param), I32, I32, I64, I64, P, P, P, P, P, P, U64], ctypes.c_int),
         "cumf_sgd_train": ([ctypes.POINTER(Param), ctypes.POINTER(TrainOpts), P, I64, I32, I32, I32, U64, P, I64,
@@ -321,6 +322,14 @@ class Context:
         la, up = ctypes.c_int64(), ctypes.c_int64()
         _check(lib.cumf_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)), "cumf_kernel_time")
         return KernelTime(ms.value, la.value, up.value)
+
+    def autotune(self, base: TrainOpts):
+        """Fastest launch shape for this problem (factors are left unchanged);
+        returns (opts, epoch milliseconds)."""
+        best = TrainOpts()
+        ms = ctypes.c_double()
+        _check(lib.cumf_autotune(self._h, ctypes.byref(base), ctypes.byref(best), ctypes.byref(ms)), "cumf_autotune")
+        return best, ms.value
 
     def reset_kernel_time(self) -> None:
         _check(lib.cumf_reset_kernel_time(self._h), "cumf_reset_kernel_time")

@@ -708,6 +708,20 @@ struct Shape {
     int G, S, L, W, blocks, D;
 };
 
+// waves_per_cu waves per CU (64 = two rounds of full residency), then the
+// fewest workers that keep the same windows per worker, so an epoch
+// over-samples the ratings as little as possible
+int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
+{
+    const int64_t w0 = (int64_t)c->cu_count * waves_per_cu * S;
+    const int64_t nnz = std::max<int64_t>(1, c->nnz);
+    const int64_t uc = std::max<int64_t>(1, (nnz + w0 * L - 1) / (w0 * L));
+    int64_t w = (nnz + uc * L - 1) / (uc * L);
+    const int64_t gran = (int64_t)kSgdWavesPerBlock * S;
+    w = std::max<int64_t>(gran, (w + gran - 1) / gran * gran);
+    return (int)std::min<int64_t>(w, 1 << 24);
+}
+
 int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
 {
     sh->G = o->lanes_per_sample ? o->lanes_per_sample : sgd_default_group(c->k, c->prec);
@@ -729,16 +743,7 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     } else if (o->num_workers > 0) {
         W = o->num_workers;
     } else {
-        // default: 64 waves per CU (two rounds of full residency), then the
-        // fewest workers that keep the same windows per worker, so an epoch
-        // over-samples the ratings as little as possible
-        const int64_t w0 = (int64_t)c->cu_count * 64 * sh->S;
-        const int64_t nnz = c->nnz;
-        const int64_t uc = std::max<int64_t>(1, (nnz + w0 * sh->L - 1) / (w0 * sh->L));
-        int64_t w = (nnz + uc * sh->L - 1) / (uc * sh->L);
-        const int64_t gran = (int64_t)kSgdWavesPerBlock * sh->S;
-        w = std::max<int64_t>(gran, (w + gran - 1) / gran * gran);
-        W = (int)std::min<int64_t>(w, 1 << 24);
+        W = default_workers(c, sh->S, sh->L, 64);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;
@@ -1009,4 +1014,88 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         }
     }
     return harvest_events(c);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: time one epoch of every candidate launch shape (lane group,
+// pipeline depth, waves per CU) on the context's own problem, keep the
+// fastest, and restore the factors the trials trained.
+// ---------------------------------------------------------------------------
+extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!base || !best) return fail(CUMF_ERR_ARG, "null options");
+    if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
+    if (base->schedule == CUMF_SCHED_SERIAL) {
+        *best = *base;
+        if (best_ms) *best_ms = 0;
+        return CUMF_OK;
+    }
+    const size_t eb = (size_t)prec_bytes(c->prec);
+    const size_t pb = (size_t)c->m * c->k * eb, qb = (size_t)c->n * c->k * eb;
+    DevTmp sp, sq, sps, sqs;
+    HIP_TRY(hipMalloc(&sp.p, pb));
+    HIP_TRY(hipMalloc(&sq.p, qb));
+    HIP_TRY(hipMemcpyAsync(sp.p, c->P, pb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(sq.p, c->Q, qb, hipMemcpyDeviceToDevice, c->stream));
+    if (c->prec == CUMF_FP8_E4M3) {
+        HIP_TRY(hipMalloc(&sps.p, 4 * (size_t)c->m));
+        HIP_TRY(hipMalloc(&sqs.p, 4 * (size_t)c->n));
+        HIP_TRY(hipMemcpyAsync(sps.p, c->ps, 4 * (size_t)c->m, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(sqs.p, c->qs, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    // keep the caller's timing totals out of the trials
+    if ((rc = harvest_events(c))) return rc;
+    const double keep_ms = c->kernel_ms;
+    const int64_t keep_l = c->launches, keep_u = c->updates;
+    const cumf_launch_info keep_last = c->last;
+
+    const int groups[4] = {64, 32, 16, 8};
+    const int depths[3] = {2, 4, 8};
+    const int waves_per_cu[4] = {16, 32, 64, 128};
+    const int L = base->batch_len ? base->batch_len : 128;
+    double best_t = 1e300;
+    cumf_train_opts bo = *base;
+    bool warm = false;
+    for (int G : groups) {
+        if (!sgd_shape_supported(c->k, c->prec, G)) continue;
+        if (base->coherent && (c->k / G) * (int)eb < 4) continue;
+        for (int D : depths)
+            for (int wpc : waves_per_cu) {
+                cumf_train_opts o = *base;
+                o.lanes_per_sample = G;
+                o.pipeline_depth = D;
+                o.num_workers = default_workers(c, 64 / G, L, wpc);
+                o.use_graph = 0;
+                if (!warm) {   // first launch of the code object is not timed
+                    if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
+                    warm = true;
+                }
+                c->kernel_ms = 0;
+                if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
+                // per-rating time: shapes differ in how much they over-sample
+                const int64_t upd = std::max<int64_t>(1, c->last.updates_per_epoch);
+                const double t = c->kernel_ms * (double)c->nnz / (double)upd;
+                if (t < best_t) {
+                    best_t = t;
+                    bo = o;
+                }
+            }
+    }
+    HIP_TRY(hipMemcpyAsync(c->P, sp.p, pb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Q, sq.p, qb, hipMemcpyDeviceToDevice, c->stream));
+    if (c->prec == CUMF_FP8_E4M3) {
+        HIP_TRY(hipMemcpyAsync(c->ps, sps.p, 4 * (size_t)c->m, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->qs, sqs.p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->kernel_ms = keep_ms;
+    c->launches = keep_l;
+    c->updates = keep_u;
+    c->last = keep_last;
+    bo.use_graph = base->use_graph;
+    *best = bo;
+    if (best_ms) *best_ms = best_t;
+    return CUMF_OK;
 }

@@ -153,6 +153,14 @@ int cumf_train(cumf_ctx* ctx, const cumf_train_opts* opts, int32_t epoch_begin, 
  * b for out-of-range ids or nan, FP64 sum of FP32 squared errors. */
 int cumf_test_rmse(cumf_ctx* ctx, float mult, float b, double* rmse);
 
+/* Pick the launch shape: time one epoch of every supported (lanes_per_sample,
+ * pipeline_depth, workers for {16, 32, 64, 128} waves per CU) for base's
+ * schedule on this context's problem, write the fastest per rating into *best
+ * (other fields copied from base) and its kernel time per nnz updates in ms
+ * into *best_ms, then restore the factors the trials trained.  Kernel-time
+ * totals and last_launch are not changed.  A serial schedule returns base. */
+int cumf_autotune(cumf_ctx* ctx, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms);
+
 /* Time spent in training kernels since the last reset (HIP events on the
  * context's stream), the number of training launches, and the number of
  * rating updates they performed. */

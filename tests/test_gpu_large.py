@@ -77,9 +77,12 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
     into the context's buffers), FP16, stratum schedule: 64-bit indexing,
     partition covers every rating, RMSE falls every epoch.
 
-    The learning rate is the reference's default (alpha = beta = 0.01,
-    singleGPU/sgd.h:45): with alpha = 0.08 plain SGD diverges on these
-    log-normal ratings (values up to ~260), in the serial CPU oracle as well."""
+    The ratings reach ~1300 (log-normal over 3.1e9 draws), and the
+    reference's update has no clipping: alpha = 0.005 (and the reference's
+    default 0.01, singleGPU/sgd.h:45) drive the factors to NaN within three
+    epochs under every schedule and in FP32 too, which the test RMSE hides
+    because mf_predict substitutes b for a NaN prediction (test/mf.cpp:3805).
+    alpha = 0.002 learns; the test also checks that Q stays finite."""
     import torch
 
     import cumf_sgd_amd as cs
@@ -105,7 +108,7 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
         off = c.partition_strata(8)
         assert off[-1] == nnz and off[0] == 0 and np.all(np.diff(off) >= 0)
         c.init_factors(128, cs.FP16, 5)
-        o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=float(np.float32(0.05) / scale), seed=5, alpha=0.01,
+        o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=float(np.float32(0.05) / scale), seed=5, alpha=0.002,
                          beta=0.01)
         fs = np.float32(np.sqrt(scale))
         curve = [c.test_rmse(fs, b)]
@@ -118,7 +121,9 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
         info = c.last_launch()
         print(f"rmse {[round(x, 4) for x in curve]}, {nnz * 3 / t / 1e9:.2f}e9 updates/s, "
               f"updates/epoch {info.updates_per_epoch}")
-        assert np.all(np.isfinite(curve)) and np.all(np.diff(curve) < 0) and curve[-1] < curve[0] - 0.1
+        Q, _ = c.get_factors(1)
+        assert np.isfinite(Q.view(np.float16)).all()
+        assert np.all(np.isfinite(curve)) and np.all(np.diff(curve) < 0) and curve[-1] < curve[0] - 0.3
         assert info.updates_per_epoch >= nnz
 
 

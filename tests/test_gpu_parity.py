@@ -306,3 +306,43 @@ def test_graph_epoch_loop_matches_eager(gpu, prec, sched):
         # Hogwild interleavings differ run to run; both must train
         for g in (0, 1):
             assert np.all(np.isfinite(snaps[g][-1][0]))
+
+
+@pytest.mark.parametrize("prec,sched", [(1, "hogwild"), (3, "hogwild"), (0, "stratum")])
+def test_autotune_restores_factors_and_picks_a_valid_shape(gpu, prec, sched):
+    """The autotuner trains trial epochs, so it must put the factors, the
+    kernel-time totals and the last-launch record back as they were; the shape
+    it returns must train."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=200_000, m=2000, n=900)
+    with _ctx(u, v, r, m, n) as c:
+        if sched == "stratum":
+            c.partition_strata(8)
+        c.init_factors(64, prec, seed=5)
+        schedule = cs.STRATUM if sched == "stratum" else cs.HOGWILD
+        base = cs.make_opts(k=64, schedule=schedule, seed=5)
+        c.train(base, 0, 1)
+        P0, ps0 = c.get_factors(0)
+        Q0, qs0 = c.get_factors(1)
+        ms0 = c.kernel_time()
+        last0 = c.last_launch()
+        best, ms = c.autotune(base)
+        assert ms > 0
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+        assert np.array_equal(P0.view(np.uint8), P1.view(np.uint8))
+        assert np.array_equal(Q0.view(np.uint8), Q1.view(np.uint8))
+        if prec == cs.FP8_E4M3:
+            assert np.array_equal(ps0, ps1) and np.array_equal(qs0, qs1)
+        assert c.kernel_time() == ms0
+        last1 = c.last_launch()
+        assert (last1.num_workers, last1.lanes_per_sample, last1.pipeline_depth) == \
+            (last0.num_workers, last0.lanes_per_sample, last0.pipeline_depth)
+        assert best.lanes_per_sample in (8, 16, 32, 64) and best.pipeline_depth in (2, 4, 8)
+        assert best.num_workers > 0 and best.schedule == schedule
+        c.train(best, 1, 3)
+        info = c.last_launch()
+        assert info.lanes_per_sample == best.lanes_per_sample and info.pipeline_depth == best.pipeline_depth
+        c.set_test(tu, tv, tr)
+        assert np.isfinite(c.test_rmse())
