@@ -708,9 +708,12 @@ struct Shape {
     int G, S, L, W, blocks, D;
 };
 
-// waves_per_cu waves per CU (64 = two rounds of full residency), then the
+// waves_per_cu waves per CU (the default 128 is four rounds of full
+// residency: short per-worker chains, little tail), then the
 // fewest workers that keep the same windows per worker, so an epoch
 // over-samples the ratings as little as possible
+constexpr int kDefaultWavesPerCu = 128;
+
 int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
 {
     const int64_t w0 = (int64_t)c->cu_count * waves_per_cu * S;
@@ -743,7 +746,7 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     } else if (o->num_workers > 0) {
         W = o->num_workers;
     } else {
-        W = default_workers(c, sh->S, sh->L, 64);
+        W = default_workers(c, sh->S, sh->L, kDefaultWavesPerCu);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;
@@ -1053,7 +1056,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
 
     const int groups[4] = {64, 32, 16, 8};
     const int depths[3] = {2, 4, 8};
-    const int waves_per_cu[4] = {16, 32, 64, 128};
+    const int waves_per_cu[5] = {32, 64, 128, 256, 512};
     const int L = base->batch_len ? base->batch_len : 128;
     double best_t = 1e300;
     cumf_train_opts bo = *base;

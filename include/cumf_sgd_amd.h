@@ -62,10 +62,10 @@ typedef struct cumf_node {
 /* Training options.  Field meanings follow Parameter (singleGPU/sgd.h:26-46). */
 typedef struct cumf_train_opts {
     int32_t k;                 /* factor length: 32, 64 or 128 */
-    int32_t num_workers;       /* Hogwild workers (-s); 0 = autotuned */
+    int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU, rounded for least over-sampling */
     int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128 */
     int32_t schedule;          /* enum cumf_schedule */
-    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = default for (k, precision) */
+    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane (FP8: 4 elements) */
     int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
     int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 4 */
     int32_t use_graph;         /* 1: run the epoch loop as a captured HIP graph, replayed per epoch */
@@ -154,7 +154,7 @@ int cumf_train(cumf_ctx* ctx, const cumf_train_opts* opts, int32_t epoch_begin, 
 int cumf_test_rmse(cumf_ctx* ctx, float mult, float b, double* rmse);
 
 /* Pick the launch shape: time one epoch of every supported (lanes_per_sample,
- * pipeline_depth, workers for {16, 32, 64, 128} waves per CU) for base's
+ * pipeline_depth, workers for {32, 64, 128, 256, 512} waves per CU) for base's
  * schedule on this context's problem, write the fastest per rating into *best
  * (other fields copied from base) and its kernel time per nnz updates in ms
  * into *best_ms, then restore the factors the trials trained.  Kernel-time

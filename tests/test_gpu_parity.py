@@ -235,7 +235,7 @@ def test_serial_gpu_rmse_equals_oracle(gpu, c1_oracle_rmse):
     oracle's, so the test RMSE agrees to the FP64 summation order."""
     import cumf_sgd_amd as cs
 
-    curve, _ = _c1_rmse_curve(0, cs.SERIAL, 20)
+    curve, _ = _c1_rmse_curve(0, cs.SERIAL, 20, lanes=64)   # the oracle's lane-group dot order
     assert abs(curve[-1] - c1_oracle_rmse) <= 1e-9 * c1_oracle_rmse
 
 
