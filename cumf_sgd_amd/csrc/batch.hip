@@ -102,6 +102,63 @@ void free_problem(Problem& p)
     p = Problem{};
 }
 
+constexpr int kBatchXcds = 8;   // MI355X: 8 XCDs of 32 CUs
+
+// XCD-aware placement of the problems' waves.  The kernel numbers waves
+// XCD-major, so XCD x runs waves [x*per_xcd, (x+1)*per_xcd).  Each XCD has its
+// own L2, which device memory does not keep coherent with the others inside
+// a kernel: a problem whose workers sit on two XCDs trains two copies of its
+// hot rows, and one of them is lost at write-back.  So every problem that is
+// at most an XCD's fair share of waves goes whole onto the least-loaded XCD
+// (longest first; the per-XCD size grows from the mean until everything
+// fits, and never past the mean plus the largest such problem, the longest-
+// first bound); a larger problem is laid across consecutive XCDs
+// first (its rows are spread over many more ratings, and it has to use more
+// than one XCD to finish in time).  On return the tasks are sorted by
+// worker_begin and *per_xcd is the waves per XCD, a multiple of a block.
+void place_on_xcds(std::vector<SgdTask>& tasks, int nx, int64_t* per_xcd)
+{
+    int64_t total = 0, biggest_small = 0;
+    for (const SgdTask& t : tasks) total += t.wave_count;
+    const int64_t fair = (total + nx - 1) / nx;
+    for (const SgdTask& t : tasks)
+        if (t.wave_count <= fair) biggest_small = std::max<int64_t>(biggest_small, t.wave_count);
+    std::vector<size_t> order(tasks.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return tasks[a].wave_count > tasks[b].wave_count; });
+    int64_t cap = fair;
+    for (;;) {
+        cap = (cap + kSgdWavesPerBlock - 1) / kSgdWavesPerBlock * kSgdWavesPerBlock;
+        std::vector<int64_t> load(nx, 0);
+        int64_t cursor = 0;   // end of the problems laid across XCDs
+        bool ok = true;
+        for (size_t i : order) {
+            SgdTask& t = tasks[i];
+            if (t.wave_count > fair) {
+                t.worker_begin = (int32_t)cursor;
+                cursor += t.wave_count;
+                continue;
+            }
+            for (int x = 0; x < nx; ++x) load[x] = std::max(load[x], std::min(cap, std::max<int64_t>(0, cursor - x * cap)));
+            int best = -1;
+            for (int x = 0; x < nx; ++x)
+                if (load[x] + t.wave_count <= cap && (best < 0 || load[x] < load[best])) best = x;
+            if (best < 0) {
+                ok = false;
+                break;
+            }
+            t.worker_begin = (int32_t)(best * cap + load[best]);
+            load[best] += t.wave_count;
+        }
+        if (ok && cursor <= (int64_t)nx * cap) break;
+        cap += std::max<int64_t>(cap / 32, kSgdWavesPerBlock);
+    }
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const SgdTask& a, const SgdTask& b) { return a.worker_begin < b.worker_begin; });
+    *per_xcd = cap;
+}
+
 }  // namespace
 
 extern "C" {
@@ -260,7 +317,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
         t.u = p.u; t.v = p.v; t.r = p.r; t.nnz = p.nnz;
         t.P = p.P; t.Q = p.Q; t.ps = p.ps; t.qs = p.qs; t.lr = b->lr;
         t.lambda_p = p.lp; t.lambda_q = p.lq;
-        t.worker_begin = (int32_t)wave;
+        t.worker_begin = 0;   // placed below
         t.wave_count = (int32_t)waves;
         t.worker_count = (int32_t)workers;
         t.update_count = (int32_t)((p.nnz + workers * L - 1) / (workers * L));
@@ -273,6 +330,9 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     }
     if (tasks.empty()) return CUMF_OK;
     if (wave > (int64_t)1 << 30) return fail(CUMF_ERR_ARG, "too many waves");
+    int64_t per_xcd = 0;
+    place_on_xcds(tasks, kBatchXcds, &per_xcd);
+    wave = per_xcd * kBatchXcds;
     if (tasks.size() > b->tasks_n) {
         dev_free(b->tasks);
         b->tasks = nullptr;
@@ -291,7 +351,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
         a.epoch_end = ep + 1;
         a.batch_len = L;
         a.seed = o->seed;
-        a.num_xcd = 1;
+        a.num_xcd = kBatchXcds;
         BHIP_TRY(launch_sgd_batched(a, blocks, b->prec, D, b->stream));
         if (b->prec == CUMF_FP8_E4M3)
             for (const Problem& p : b->probs) {

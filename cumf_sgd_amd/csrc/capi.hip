@@ -708,11 +708,12 @@ struct Shape {
     int G, S, L, W, blocks, D;
 };
 
-// waves_per_cu waves per CU (the default 128 is four rounds of full
-// residency: short per-worker chains, little tail), then the
+// waves_per_cu waves per CU (the default 128, 512 for FP8, measured
+// fastest: short per-worker chains, little tail), then the
 // fewest workers that keep the same windows per worker, so an epoch
 // over-samples the ratings as little as possible
 constexpr int kDefaultWavesPerCu = 128;
+constexpr int kDefaultWavesPerCuFp8 = 512;   // FP8 is ALU-heavier: more, shorter workers hide it best
 
 int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
 {
@@ -746,7 +747,7 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     } else if (o->num_workers > 0) {
         W = o->num_workers;
     } else {
-        W = default_workers(c, sh->S, sh->L, kDefaultWavesPerCu);
+        W = default_workers(c, sh->S, sh->L, c->prec == CUMF_FP8_E4M3 ? kDefaultWavesPerCuFp8 : kDefaultWavesPerCu);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;

@@ -123,24 +123,20 @@ __device__ inline uint32_t f32_to_e4m3_rne(float f)
     r = r > 0x43e00000u ? 0x43e00000u : r;
     return sign | e4m3_pack_normal(r);
 }
-// Stochastic rounding with 16 random bits.
-__device__ inline uint32_t f32_to_e4m3_sr(float f, uint32_t r16)
+// Stochastic rounding into byte SEL of `old`, with v_cvt_sr_fp8_f32.  The
+// instruction adds the top 20 bits of its random word into the 20-bit
+// fraction of the dropped part (in units of the E4M3 quantum, 2^(e-3) for
+// normal values, 2^-9 for subnormal ones) and truncates; rword carries 16
+// random bits in its top half, so |f| rounds up when
+// floor(frac * 2^20) + (rword >> 12) >= 2^20.  The instruction turns values
+// above 448 into NaN, so |f| is clamped to 448 first (NaN stays NaN, and
+// becomes 0xff).
+template <int SEL>
+__device__ inline uint32_t e4m3_sr_insert(float f, uint32_t rword, uint32_t old)
 {
-    uint32_t x = __float_as_uint(f);
-    uint32_t sign = (x >> 24) & 0x80u, ax = x & 0x7fffffffu;
-    r16 &= 0xffffu;
-    if (ax > 0x7f800000u) return sign | 0x7fu;
-    if (ax >= 0x43e00000u) return sign | 0x7eu;
-    if (ax < 0x3c800000u) {
-        float t = __uint_as_float(ax) * 0x1.0p9f;
-        float lo = floorf(t);
-        float frac = t - lo;
-        uint32_t q = (uint32_t)lo + (((float)r16 < frac * 0x1.0p16f) ? 1u : 0u);
-        return sign | q;
-    }
-    uint32_t r = (ax + (r16 << 4)) & 0xfff00000u;
-    r = r > 0x43e00000u ? 0x43e00000u : r;
-    return sign | e4m3_pack_normal(r);
+    const float c = __builtin_fminf(__builtin_fmaxf(f, -448.0f), 448.0f);
+    const float v = __builtin_isnan(f) ? f : c;
+    return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, SEL);
 }
 
 // Smallest power of two s with amax <= target * s.

@@ -206,7 +206,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 for (int e = 0; e < E; ++e) {
                     const float pn = __builtin_fmaf(lr, __builtin_fmaf(err, q[e], -(T.lambda_p * p[e])), p[e]);
                     const float qn = __builtin_fmaf(lr, __builtin_fmaf(err, p[e], -(T.lambda_q * q[e])), q[e]);
-                    uint32_t pb, qb;
+                    uint32_t pb = 0u, qb = 0u;
                     if constexpr (PREC == kF32) {
                         pb = __float_as_uint(pn);
                         qb = __float_as_uint(qn);
@@ -225,12 +225,22 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             rbits[2] = lowbias32(srk.z ^ t);
                             rbits[3] = lowbias32(srk.w ^ t);
                         }
+                        // element jj takes the 16-bit half (jj & 1) of word (jj >> 1) & 1
                         const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
-                        pb = f32_to_e4m3_sr(pn * pinv, (jj & 1) ? (bp >> 16) : bp);
-                        qb = f32_to_e4m3_sr(qn * qinv, (jj & 1) ? (bq >> 16) : bq);
+                        const uint32_t rp = (jj & 1) ? (bp & 0xffff0000u) : (bp << 16);
+                        const uint32_t rq = (jj & 1) ? (bq & 0xffff0000u) : (bq << 16);
+                        const uint32_t op = (e & 3) ? po.w[e >> 2] : 0u, oq = (e & 3) ? qo.w[e >> 2] : 0u;
+                        switch (e & 3) {
+                        case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn * qinv, rq, oq); break;
+                        case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn * qinv, rq, oq); break;
+                        case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn * qinv, rq, oq); break;
+                        default: po.w[e >> 2] = e4m3_sr_insert<3>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn * qinv, rq, oq); break;
+                        }
                     }
-                    pack<PREC>(po, e, pb);
-                    pack<PREC>(qo, e, qb);
+                    if constexpr (!FP8) {
+                        pack<PREC>(po, e, pb);
+                        pack<PREC>(qo, e, qb);
+                    }
                 }
                 store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
                 store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
@@ -281,8 +291,9 @@ __host__ __device__ constexpr int batch_group(int k)
 
 // Many independent problems in one launch.  Tasks own whole waves: task t
 // holds waves [worker_begin, worker_begin + wave_count) (worker_begin and
-// wave_count counted in waves here) and its first worker_count workers are
-// active; each wave splits into 64/G slots for the task's k.
+// wave_count counted in waves here, tasks sorted by worker_begin, gaps
+// allowed) and its first worker_count workers are active; each wave splits
+// into 64/G slots for the task's k.
 template <int PREC, bool COH, int D>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLaunch a)
 {
@@ -293,7 +304,11 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLau
 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * kWavesPerBlock + wave;
+    // waves numbered XCD-major (workgroups go round-robin over the XCDs), so
+    // the host can keep each problem's waves on one XCD and its L2
+    int blk = blockIdx.x;
+    if (a.num_xcd > 1) blk = (int)(blockIdx.x % a.num_xcd) * (int)(gridDim.x / a.num_xcd) + (int)(blockIdx.x / a.num_xcd);
+    const int gw = blk * kWavesPerBlock + wave;
     int t = 0;
     while (t < a.num_tasks && gw >= a.tasks[t].worker_begin + a.tasks[t].wave_count) ++t;
     if (t >= a.num_tasks || gw < a.tasks[t].worker_begin) return;

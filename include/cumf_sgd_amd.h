@@ -62,10 +62,10 @@ typedef struct cumf_node {
 /* Training options.  Field meanings follow Parameter (singleGPU/sgd.h:26-46). */
 typedef struct cumf_train_opts {
     int32_t k;                 /* factor length: 32, 64 or 128 */
-    int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU, rounded for least over-sampling */
+    int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU (FP8: 512), rounded for least over-sampling */
     int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128 */
     int32_t schedule;          /* enum cumf_schedule */
-    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane (FP8: 4 elements) */
+    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane */
     int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
     int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 4 */
     int32_t use_graph;         /* 1: run the epoch loop as a captured HIP graph, replayed per epoch */

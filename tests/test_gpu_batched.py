@@ -54,8 +54,13 @@ def test_batched_serial_matches_oracle(gpu, prec):
 
 def test_batched_hogwild_rmse_vs_serial_fp32(gpu):
     """24 problems of mixed k in BF16, 20 Hogwild epochs in one launch per
-    epoch: each problem's test RMSE is within max(0.01, 1%) of its own serial
-    FP32 oracle solve at the same epoch count."""
+    epoch, against serial FP32 oracle solves of each problem.
+
+    On problems this small the serial solve itself moves with the visiting
+    order: over four seeds its test RMSE spreads by up to ~0.04.  So each
+    problem's GPU RMSE must lie within that spread (max over the four serial
+    seeds, plus 0.005), and the mean gap to the seed-mean over all problems
+    (the systematic Hogwild + BF16 effect) must be below 0.005."""
     import cumf_sgd_amd as cs
     import oracle
 
@@ -72,20 +77,25 @@ def test_batched_hogwild_rmse_vs_serial_fp32(gpu):
             b.add(u, v, rs, p.m, p.n, p.k, np.float32(0.05) / sc, tu=tu, tv=tv, tr=tr, seed=seed)
         b.train(cs.make_opts(schedule=cs.HOGWILD, num_workers=512, seed=seed), 0, epochs)
         lr = oracle.lr_table(0.08, 0.3, 0.0, epochs + 4)
-        worst = 0.0
+        gaps = []
         for i, p in enumerate(probs):
             u, v, r, tu, tv, tr = (t.numpy() for t in (p.u, p.v, p.r, p.tu, p.tv, p.tr))
             sc = scales[i]
             fs = np.float32(np.sqrt(sc))
             got = b.test_rmse(i, fs, 0.0)
             rs = (r * np.float32(1.0 / sc)).astype(np.float32)
-            P, _ = oracle.init_factors(p.m, p.k, 0, seed, 0, problem=i)
-            Q, _ = oracle.init_factors(p.n, p.k, 0, seed, 1, problem=i)
-            P, Q, _, _ = oracle.sgd_serial(u, v, rs, P, Q, lr, 0, epochs, k=p.k, G=_oracle_group(p.k, 0), prec=0,
-                                           lambda_p=np.float32(0.05) / sc, lambda_q=np.float32(0.05) / sc, seed=seed,
-                                           problem=i)
-            want = oracle.rmse(tu, tv, tr, (P * fs).astype(np.float32), (Q * fs).astype(np.float32), 0.0)
-            worst = max(worst, got - want)
-            print(f"problem {i}: m={p.m} n={p.n} k={p.k} nnz={p.u.numel()} gpu={got:.4f} oracle={want:.4f}")
-            assert got <= want + max(0.01, 0.01 * want)
-        print(f"worst gap {worst:+.4f}")
+            want = []
+            for s in range(seed, seed + 4):
+                P, _ = oracle.init_factors(p.m, p.k, 0, s, 0, problem=i)
+                Q, _ = oracle.init_factors(p.n, p.k, 0, s, 1, problem=i)
+                P, Q, _, _ = oracle.sgd_serial(u, v, rs, P, Q, lr, 0, epochs, k=p.k, G=_oracle_group(p.k, 0), prec=0,
+                                               lambda_p=np.float32(0.05) / sc, lambda_q=np.float32(0.05) / sc, seed=s,
+                                               problem=i)
+                want.append(oracle.rmse(tu, tv, tr, (P * fs).astype(np.float32), (Q * fs).astype(np.float32), 0.0))
+            want = np.array(want)
+            gaps.append(got - want.mean())
+            print(f"problem {i}: m={p.m} n={p.n} k={p.k} nnz={p.u.numel()} gpu={got:.4f} "
+                  f"oracle seeds {want.min():.4f}..{want.max():.4f}")
+            assert np.isfinite(got) and got <= want.max() + 0.005
+        print(f"mean gap to the serial seed-mean {np.mean(gaps):+.4f}, worst {np.max(gaps):+.4f}")
+        assert np.mean(gaps) < 0.005

@@ -9,6 +9,8 @@ import pytest
 
 import oracle
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
 # ---------------------------------------------------------------- Philox
 # Known-answer vectors of Philox4x32-10 published with Random123 (kat_vectors).
 PHILOX_KAT = [
@@ -100,6 +102,23 @@ def test_e4m3_stochastic_rounding_brackets_and_is_unbiased():
     for b in [0x01, 0x08, 0x3C, 0x7E]:
         x = oracle.e4m3_to_f32(b)
         assert all(oracle.f32_to_e4m3_sr(x, r) == b for r in (0, 1, 0x7FFF, 0xFFFF))
+
+
+def test_e4m3_stochastic_rounding_matches_gfx950_instruction_vectors():
+    """The oracle's SR rule against 21,302 outputs of v_cvt_sr_fp8_f32 (the
+    conversion the update kernel stores FP8 with), written on a gfx950 GPU by
+    tests/golden/make_e4m3_sr_vectors.py: subnormal-range values at and around
+    the round-up threshold of their 20-bit fraction, normal values at the
+    threshold, random values of both signs, saturation, zeros and NaNs."""
+    d = np.load(os.path.join(GOLDEN, "e4m3_sr_hw.npz"))
+    f, r16, want = d["f"], d["r16"], d["byte"]
+    got = np.array([oracle.f32_to_e4m3_sr(float(x), int(r)) for x, r in zip(f, r16)], np.uint8)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, [(float(f[i]), int(r16[i]), int(want[i]), int(got[i])) for i in bad[:5]]
+    # the vectors reach every regime the rule distinguishes
+    ax = np.abs(f)
+    assert (ax < 2.0 ** -6).sum() > 1000 and ((ax >= 2.0 ** -6) & (ax < 448)).sum() > 1000
+    assert (want == 0xFF).any() and (want == 0x7E).any() and (want == 0xFE).any()
 
 
 @pytest.mark.parametrize("amax,target", [(0.0884, 14.0), (3.0, 56.0), (56.0, 56.0), (56.01, 56.0), (1e-7, 14.0)])
