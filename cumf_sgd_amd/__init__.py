@@ -25,10 +25,13 @@ except ImportError:  # the library itself does not need torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcumf_sgd_amd.so")
 CLI_PATH = os.path.join(_HERE, "cumf_sgd")
+PREDICT_PATH = os.path.join(_HERE, "cumf_predict")
 
 FP32, FP16, BF16, FP8_E4M3 = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3}
 HOGWILD, SERIAL, GRID, STRATUM = 0, 1, 2, 3
+# mf-predict -e criteria (test/mf.h:30-40)
+EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
 SCHEDULES = {"hogwild": HOGWILD, "serial": SERIAL, "grid": GRID, "stratum": STRATUM}
 ELEM_BYTES = {FP32: 4, FP16: 2, BF16: 2, FP8_E4M3: 1}
 STORAGE_DTYPE = {FP32: np.float32, FP16: np.uint16, BF16: np.uint16, FP8_E4M3: np.uint8}
@@ -117,6 +120,8 @@ def _load():
         "cumf_export_factors": ([P, I32, F32, P, P], ctypes.c_int),
         "cumf_train": ([P, ctypes.POINTER(TrainOpts), I32, I32, P, I64], ctypes.c_int),
         "cumf_test_rmse": ([P, F32, F32, D], ctypes.c_int),
+        "cumf_test_eval": ([P, I32, F32, F32, D], ctypes.c_int),
+        "cumf_set_model": ([P, I32, I32, I32, P, P], ctypes.c_int),
         "cumf_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_reset_kernel_time": ([P], ctypes.c_int),
         "cumf_autotune": ([P, ctypes.POINTER(TrainOpts), ctypes.POINTER(TrainOpts), D], ctypes.c_int),
@@ -132,12 +137,15 @@ def _load():
                                ctypes.POINTER(I32)], ctypes.c_int),
         "cumf_free_host": ([P], None),
         "cumf_save_model": ([ctypes.c_char_p, I32, I32, I32, F32, P, P], ctypes.c_int),
+        "cumf_load_model": ([ctypes.c_char_p, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                             ctypes.POINTER(F32), ctypes.POINTER(P), ctypes.POINTER(P)], ctypes.c_int),
         "cumf_batch_create": ([ctypes.c_int, I32, ctypes.POINTER(P)], ctypes.c_int),
         "cumf_batch_destroy": ([P], ctypes.c_int),
         "cumf_batch_add": ([P, P, P, P, I64, I32, I32, I32, F32, F32, P, P, P, I64, U64, ctypes.POINTER(I32)],
                            ctypes.c_int),
         "cumf_batch_train": ([P, ctypes.POINTER(TrainOpts), I32, I32], ctypes.c_int),
         "cumf_batch_test_rmse": ([P, I32, F32, F32, D], ctypes.c_int),
+        "cumf_batch_test_eval": ([P, I32, I32, F32, F32, D], ctypes.c_int),
         "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
         "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
     }
@@ -317,6 +325,19 @@ class Context:
         _check(lib.cumf_test_rmse(self._h, float(mult), float(b), ctypes.byref(out)), "cumf_test_rmse")
         return out.value
 
+    def set_model(self, P: np.ndarray, Q: np.ndarray) -> None:
+        """FP32 factors of any k for evaluation (cumf_set_model)."""
+        P = np.ascontiguousarray(P, np.float32)
+        Q = np.ascontiguousarray(Q, np.float32)
+        _check(lib.cumf_set_model(self._h, P.shape[0], Q.shape[0], P.shape[1], _ptr(P), _ptr(Q)), "cumf_set_model")
+        self.m, self.n, self.k, self.precision = P.shape[0], Q.shape[0], P.shape[1], FP32
+
+    def test_eval(self, criterion: int, mult: float = 1.0, b: float = 0.0) -> float:
+        """mf-predict's -e criterion (EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC)."""
+        out = ctypes.c_double()
+        _check(lib.cumf_test_eval(self._h, int(criterion), float(mult), float(b), ctypes.byref(out)), "cumf_test_eval")
+        return out.value
+
     def kernel_time(self) -> KernelTime:
         ms = ctypes.c_double()
         la, up = ctypes.c_int64(), ctypes.c_int64()
@@ -386,6 +407,12 @@ class Batch:
         out = ctypes.c_double()
         _check(lib.cumf_batch_test_rmse(self._h, index, float(mult), float(b), ctypes.byref(out)),
                "cumf_batch_test_rmse")
+        return out.value
+
+    def test_eval(self, index: int, criterion: int, mult: float = 1.0, b: float = 0.0) -> float:
+        out = ctypes.c_double()
+        _check(lib.cumf_batch_test_eval(self._h, index, int(criterion), float(mult), float(b), ctypes.byref(out)),
+               "cumf_batch_test_eval")
         return out.value
 
     def get_factors(self, index: int, which: int):
@@ -479,6 +506,24 @@ def save_model(path: str, P: np.ndarray, Q: np.ndarray, b: float) -> None:
     Q = np.ascontiguousarray(Q, np.float32)
     _check(lib.cumf_save_model(path.encode(), P.shape[0], Q.shape[0], P.shape[1], float(b), _ptr(P), _ptr(Q)),
            "cumf_save_model")
+
+
+def load_model(path: str):
+    """The reference's model file (test/mf.cpp:3706-3777): returns (P, Q, b)."""
+    m, n, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    b = ctypes.c_float()
+    pp, qp = ctypes.c_void_p(), ctypes.c_void_p()
+    _check(lib.cumf_load_model(path.encode(), ctypes.byref(m), ctypes.byref(n), ctypes.byref(k), ctypes.byref(b),
+                               ctypes.byref(pp), ctypes.byref(qp)), "cumf_load_model")
+    try:
+        P = np.ctypeslib.as_array((ctypes.c_float * (m.value * k.value)).from_address(pp.value)).copy() \
+            if m.value else np.zeros(0, np.float32)
+        Q = np.ctypeslib.as_array((ctypes.c_float * (n.value * k.value)).from_address(qp.value)).copy() \
+            if n.value else np.zeros(0, np.float32)
+    finally:
+        lib.cumf_free_host(pp)
+        lib.cumf_free_host(qp)
+    return P.reshape(m.value, k.value), Q.reshape(n.value, k.value), np.float32(b.value)
 
 
 def sgd_train(param: Param, nodes: np.ndarray, m: int, n: int, precision: int = FP16, seed: int = 1,

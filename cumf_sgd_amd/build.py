@@ -3,6 +3,7 @@
 Produces, in-tree:
   cumf_sgd_amd/libcumf_sgd_amd.so   HIP kernels + C ABI + host pipeline
   cumf_sgd_amd/cumf_sgd             command-line trainer (singleGPU/main.cpp's CLI)
+  cumf_sgd_amd/cumf_predict         GPU evaluator (test/mf-predict's CLI)
   oracle/liboracle.so, oracle/_ref/ checker only (tests and benchmark baseline)
 """
 from __future__ import annotations
@@ -18,6 +19,7 @@ ROOT = os.path.dirname(PKG)
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libcumf_sgd_amd.so")
 CLI = os.path.join(PKG, "cumf_sgd")
+PREDICT = os.path.join(PKG, "cumf_predict")
 ARCH = os.environ.get("CUMF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -55,8 +57,9 @@ def build_library() -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
           *[os.path.join(BUILD, f"{u}.o") for u in hip_units], os.path.join(BUILD, "host.o")])
     os.replace(tmp, LIB)
-    _run(["g++", *HOST_FLAGS, "-o", CLI, os.path.join(host, "main.cpp"), f"-L{PKG}", "-lcumf_sgd_amd",
-          "-Wl,-rpath,$ORIGIN"])
+    for exe, src in ((CLI, "main.cpp"), (PREDICT, "predict.cpp")):
+        _run(["g++", *HOST_FLAGS, "-o", exe, os.path.join(host, src), f"-L{PKG}", "-lcumf_sgd_amd",
+              "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

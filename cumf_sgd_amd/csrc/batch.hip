@@ -369,15 +369,17 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     return CUMF_OK;
 }
 
-int cumf_batch_test_rmse(cumf_batch* b, int32_t index, float mult, float bias, double* rmse)
+int cumf_batch_test_eval(cumf_batch* b, int32_t index, int32_t criterion, float mult, float bias, double* value)
 {
     int rc = check_batch(b);
     if (rc) return rc;
     Problem* p;
     if ((rc = problem_of(b, index, &p))) return rc;
-    if (!rmse) return fail(CUMF_ERR_ARG, "null output");
+    if (!eval_supported(criterion))
+        return fail(CUMF_ERR_ARG, "criterion %d not supported (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)", criterion);
+    if (!value) return fail(CUMF_ERR_ARG, "null output");
     if (p->tnnz == 0) {
-        *rmse = 0.0;
+        *value = 0.0;
         return CUMF_OK;
     }
     const size_t need = (size_t)rmse_blocks(p->tnnz) + 1;
@@ -389,13 +391,18 @@ int cumf_batch_test_rmse(cumf_batch* b, int32_t index, float mult, float bias, d
         b->partials_n = need;
     }
     double* out = b->partials + need - 1;
-    BHIP_TRY(launch_rmse(p->tu, p->tv, p->tr, p->tnnz, p->P, p->ps, p->Q, p->qs, p->m, p->n, p->k, b->prec, mult,
-                         bias, b->partials, out, b->stream));
+    BHIP_TRY(launch_eval(criterion, p->tu, p->tv, p->tr, p->tnnz, p->P, p->ps, p->Q, p->qs, p->m, p->n, p->k, b->prec,
+                         mult, bias, b->partials, out, b->stream));
     double h = 0;
     BHIP_TRY(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, b->stream));
     BHIP_TRY(hipStreamSynchronize(b->stream));
-    *rmse = std::sqrt(h / (double)p->tnnz);
+    *value = criterion == 0 ? std::sqrt(h / (double)p->tnnz) : h / (double)p->tnnz;
     return CUMF_OK;
+}
+
+int cumf_batch_test_rmse(cumf_batch* b, int32_t index, float mult, float bias, double* rmse)
+{
+    return cumf_batch_test_eval(b, index, 0, mult, bias, rmse);
 }
 
 int cumf_batch_get_factors(cumf_batch* b, int32_t index, int32_t which, void* data, float* scales)

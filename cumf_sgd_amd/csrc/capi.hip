@@ -582,6 +582,28 @@ int cumf_init_factors(cumf_ctx* c, int32_t k, int32_t precision, uint64_t seed)
     return CUMF_OK;
 }
 
+int cumf_set_model(cumf_ctx* c, int32_t m, int32_t n, int32_t k, const float* P, const float* Q)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (m < 0 || n < 0 || k < 1 || k > 4096 || (m && !P) || (n && !Q)) return fail(CUMF_ERR_ARG, "bad model arguments");
+    if (c->have_ratings && (c->m != m || c->n != n))
+        return fail(CUMF_ERR_ARG, "model is %d x %d but the ratings are %d x %d", m, n, c->m, c->n);
+    char** Pp = (char**)&c->P;
+    char** Qp = (char**)&c->Q;
+    if ((rc = ensure(Pp, &c->P_cap, (size_t)m * k * 4))) return rc;
+    if ((rc = ensure(Qp, &c->Q_cap, (size_t)n * k * 4))) return rc;
+    c->m = m;
+    c->n = n;
+    c->k = k;
+    c->prec = CUMF_FP32;
+    if (m && (rc = copy_in(c->P, P, (size_t)m * k * 4, c->stream))) return rc;
+    if (n && (rc = copy_in(c->Q, Q, (size_t)n * k * 4, c->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_factors = true;
+    return CUMF_OK;
+}
+
 static int factor_ref(cumf_ctx* c, int32_t which, void** F, float** S, int64_t* rows)
 {
     if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
@@ -645,27 +667,34 @@ int cumf_export_factors(cumf_ctx* c, int32_t which, float mult, const int32_t* m
     return CUMF_OK;
 }
 
-int cumf_test_rmse(cumf_ctx* c, float mult, float b, double* rmse)
+int cumf_test_eval(cumf_ctx* c, int32_t criterion, float mult, float b, double* value)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (!eval_supported(criterion))
+        return fail(CUMF_ERR_ARG, "criterion %d not supported (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)", criterion);
     if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
     if (!c->have_test) return fail(CUMF_ERR_STATE, "no test ratings");
-    if (!rmse) return fail(CUMF_ERR_ARG, "null output");
+    if (!value) return fail(CUMF_ERR_ARG, "null output");
     if (c->tnnz == 0) {
-        *rmse = 0.0;   // test/mf.cpp:3818-3819
+        *value = 0.0;   // test/mf.cpp:3818-3819
         return CUMF_OK;
     }
     const size_t need = (size_t)rmse_blocks(c->tnnz) + 1;
     if ((rc = ensure(&c->partials, &c->partials_cap, need))) return rc;
     double* out = c->partials + need - 1;
-    HIP_TRY(launch_rmse(c->tu, c->tv, c->tr, c->tnnz, c->P, c->ps, c->Q, c->qs, c->m, c->n, c->k, c->prec, mult, b,
-                        c->partials, out, c->stream));
+    HIP_TRY(launch_eval(criterion, c->tu, c->tv, c->tr, c->tnnz, c->P, c->ps, c->Q, c->qs, c->m, c->n, c->k, c->prec,
+                        mult, b, c->partials, out, c->stream));
     double h = 0.0;
     HIP_TRY(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    *rmse = std::sqrt(h / (double)c->tnnz);
+    *value = criterion == 0 ? std::sqrt(h / (double)c->tnnz) : h / (double)c->tnnz;
     return CUMF_OK;
+}
+
+int cumf_test_rmse(cumf_ctx* c, float mult, float b, double* rmse)
+{
+    return cumf_test_eval(c, 0, mult, b, rmse);
 }
 
 int cumf_kernel_time(cumf_ctx* c, double* ms, int64_t* launches, int64_t* updates)

@@ -62,9 +62,11 @@ hipError_t launch_fp8_rescale(uint8_t* F, float* scales, int64_t rows, int k, hi
 hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k, int prec, float mult,
                          const int32_t* map, float* out, hipStream_t st);
 int rmse_blocks(int64_t nnz);
-hipError_t launch_rmse(const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz, const void* P,
-                       const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k, int prec,
-                       float mult, float b, double* partials, double* out, hipStream_t st);
+bool eval_supported(int criterion);
+// criterion: 0 RMSE (sum of squares), 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC (sums of terms)
+hipError_t launch_eval(int criterion, const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz,
+                       const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,
+                       int prec, float mult, float b, double* partials, double* out, hipStream_t st);
 hipError_t launch_split(const void* aos, int64_t nnz, int32_t* u, int32_t* v, float* r, hipStream_t st);
 hipError_t launch_check_ids(const int32_t* u, const int32_t* v, int64_t nnz, int32_t m, int32_t n,
                             unsigned long long* bad, hipStream_t st);

@@ -6,7 +6,8 @@
 //  export_kernel       dequantize * mult + inverse permutation gather
 //                      (transform_half, sgd_kernel.cu:29-38; scale_model and
 //                      shuffle_model, sgd.cu:593-635)
-//  rmse_kernel         test RMSE (mf_predict + calc_rmse, test/mf.cpp:3794-3839)
+//  eval_kernel         test RMSE and the other mf-predict criteria (mf_predict +
+//                      calc_rmse/mae/gkl/logloss/accuracy, test/mf.cpp:3794-3912)
 //  split/remap/scale/moments/partition kernels: on-device preprocessing
 //                      (sgd.cu:78-134, 411-487)
 #include "kernels.h"
@@ -245,14 +246,39 @@ hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k
 }
 
 // ---------------------------------------------------------------------------
-// Test RMSE: one thread per test rating, sequential FP32 dot of the exported
-// values with separate multiply and add, as std::inner_product in
-// test/mf.cpp:3803; FP64 sums, reduced per block and then in block order.
+// Test evaluation: one thread per test rating, mf_predict's sequential FP32
+// dot of the exported values with separate multiply and add (std::inner_product,
+// test/mf.cpp:3794-3812, b for out-of-range ids or a NaN dot), then the
+// criterion's per-rating term with the reference's arithmetic; FP64 sums,
+// reduced per block and then in block order.
+//   CRIT 0 RMSE     float e = r - z; e*e            (calc_rmse, mf.cpp:3814-3839)
+//        1 MAE      fabsf(r - z)                    (calc_mae, mf.cpp:3841-3855)
+//        2 GKL      float r*logf(r/z) - r + z       (calc_gkl, mf.cpp:3857-3872)
+//        5 LOGLOSS  log(1.0 + double(expf(-+z)))    (calc_logloss, mf.cpp:3874-3892)
+//        6 ACC      r > 0 ? z > 0 : z < 0           (calc_accuracy, mf.cpp:3894-3912)
 // ---------------------------------------------------------------------------
 constexpr int kRmseThreads = 256;
 
-template <int PREC>
-__global__ __launch_bounds__(kRmseThreads) void rmse_kernel(const int32_t* tu, const int32_t* tv, const float* tr,
+template <int CRIT>
+__device__ inline double eval_term(float r, float z)
+{
+    if constexpr (CRIT == 0) {
+        const float e = __fsub_rn(r, z);
+        return (double)__fmul_rn(e, e);
+    } else if constexpr (CRIT == 1) {
+        return (double)fabsf(__fsub_rn(r, z));
+    } else if constexpr (CRIT == 2) {
+        const float t = __fmul_rn(r, logf(__fdiv_rn(r, z)));
+        return (double)__fadd_rn(__fsub_rn(t, r), z);
+    } else if constexpr (CRIT == 5) {
+        return r > 0.0f ? log(1.0 + (double)expf(-z)) : log(1.0 + (double)expf(z));
+    } else {
+        return r > 0.0f ? (z > 0.0f ? 1.0 : 0.0) : (z < 0.0f ? 1.0 : 0.0);
+    }
+}
+
+template <int PREC, int CRIT>
+__global__ __launch_bounds__(kRmseThreads) void eval_kernel(const int32_t* tu, const int32_t* tv, const float* tr,
                                                             int64_t nnz, const void* P, const float* ps,
                                                             const void* Q, const float* qs, int32_t m, int32_t n,
                                                             int k, float mult, float b, double* partials)
@@ -273,8 +299,7 @@ __global__ __launch_bounds__(kRmseThreads) void rmse_kernel(const int32_t* tu, c
             }
             if (__builtin_isnan(z)) z = b;
         }
-        const float err = __fsub_rn(tr[i], z);
-        acc += (double)__fmul_rn(err, err);
+        acc += eval_term<CRIT>(tr[i], z);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -308,19 +333,46 @@ int rmse_blocks(int64_t nnz)
     return (int)b;
 }
 
-hipError_t launch_rmse(const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz, const void* P,
-                       const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k, int prec,
-                       float mult, float b, double* partials, double* out, hipStream_t st)
+bool eval_supported(int criterion)
 {
-    const int blocks = rmse_blocks(nnz);
-    switch (prec) {
-    case kF32: hipLaunchKernelGGL(rmse_kernel<kF32>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
-    case kF16: hipLaunchKernelGGL(rmse_kernel<kF16>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
-    case kBF16: hipLaunchKernelGGL(rmse_kernel<kBF16>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
-    case kE4M3: hipLaunchKernelGGL(rmse_kernel<kE4M3>, dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials); break;
+    return criterion == 0 || criterion == 1 || criterion == 2 || criterion == 5 || criterion == 6;
+}
+
+template <int PREC>
+static hipError_t launch_eval_prec(int crit, int blocks, const int32_t* tu, const int32_t* tv, const float* tr,
+                                   int64_t nnz, const void* P, const float* ps, const void* Q, const float* qs,
+                                   int32_t m, int32_t n, int k, float mult, float b, double* partials, hipStream_t st)
+{
+#define CUMF_EVAL(C)                                                                                          \
+    case C:                                                                                                   \
+        hipLaunchKernelGGL((eval_kernel<PREC, C>), dim3(blocks), dim3(kRmseThreads), 0, st, tu, tv, tr, nnz, P, ps, Q, \
+                           qs, m, n, k, mult, b, partials);                                                    \
+        break;
+    switch (crit) {
+        CUMF_EVAL(0)
+        CUMF_EVAL(1)
+        CUMF_EVAL(2)
+        CUMF_EVAL(5)
+        CUMF_EVAL(6)
     default: return hipErrorInvalidValue;
     }
-    hipError_t e = hipGetLastError();
+#undef CUMF_EVAL
+    return hipGetLastError();
+}
+
+hipError_t launch_eval(int criterion, const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz,
+                       const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,
+                       int prec, float mult, float b, double* partials, double* out, hipStream_t st)
+{
+    const int blocks = rmse_blocks(nnz);
+    hipError_t e;
+    switch (prec) {
+    case kF32: e = launch_eval_prec<kF32>(criterion, blocks, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials, st); break;
+    case kF16: e = launch_eval_prec<kF16>(criterion, blocks, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials, st); break;
+    case kBF16: e = launch_eval_prec<kBF16>(criterion, blocks, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials, st); break;
+    case kE4M3: e = launch_eval_prec<kE4M3>(criterion, blocks, tu, tv, tr, nnz, P, ps, Q, qs, m, n, k, mult, b, partials, st); break;
+    default: return hipErrorInvalidValue;
+    }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sum_doubles_kernel, dim3(1), dim3(256), 0, st, partials, (int64_t)blocks, out);
     return hipGetLastError();

@@ -255,6 +255,44 @@ int cumf_save_model(const char* path, int32_t m, int32_t n, int32_t k, float b, 
     return CUMF_OK;
 }
 
+int cumf_load_model(const char* path, int32_t* m, int32_t* n, int32_t* k, float* b, float** P, float** Q)
+{
+    if (!path || !m || !n || !k || !b || !P || !Q) return fail(CUMF_ERR_ARG, "null argument");
+    *P = *Q = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CUMF_ERR_IO, "%s open failed", path);
+    int32_t hdr[4];
+    float bb = 0.0f;
+    bool ok = std::fread(hdr, sizeof(int32_t), 4, f) == 4 && std::fread(&bb, sizeof(float), 1, f) == 1;
+    if (!ok || hdr[1] < 0 || hdr[2] < 0 || hdr[3] < 1) {
+        std::fclose(f);
+        return fail(CUMF_ERR_IO, "%s: bad model header", path);
+    }
+    const size_t np = (size_t)hdr[1] * hdr[3], nq = (size_t)hdr[2] * hdr[3];
+    float* p = (float*)std::malloc(sizeof(float) * (np ? np : 1));
+    float* q = (float*)std::malloc(sizeof(float) * (nq ? nq : 1));
+    if (!p || !q) {
+        std::free(p);
+        std::free(q);
+        std::fclose(f);
+        return fail(CUMF_ERR_NOMEM, "model of %zu floats does not fit in host memory", np + nq);
+    }
+    ok = std::fread(p, sizeof(float), np, f) == np && std::fread(q, sizeof(float), nq, f) == nq;
+    std::fclose(f);
+    if (!ok) {
+        std::free(p);
+        std::free(q);
+        return fail(CUMF_ERR_IO, "%s: file shorter than its %d x %d x %d model", path, hdr[1], hdr[2], hdr[3]);
+    }
+    *m = hdr[1];
+    *n = hdr[2];
+    *k = hdr[3];
+    *b = bb;
+    *P = p;
+    *Q = q;
+    return CUMF_OK;
+}
+
 int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u_seg, int64_t v_seg,
                          const cumf_node* const* R2D, const int64_t* grid_size, uint16_t* halfp, uint16_t* halfq,
                          float* floatp, float* floatq, uint64_t seed)

@@ -153,6 +153,19 @@ int cumf_train(cumf_ctx* ctx, const cumf_train_opts* opts, int32_t epoch_begin, 
  * b for out-of-range ids or nan, FP64 sum of FP32 squared errors. */
 int cumf_test_rmse(cumf_ctx* ctx, float mult, float b, double* rmse);
 
+/* Evaluation criteria of the reference's mf-predict -e (test/mf.h:30-40,
+ * test/mf-predict.cpp:129-162), replacing calc_rmse/calc_mae/calc_gkl/
+ * calc_logloss/calc_accuracy (test/mf.cpp:3814-3912) on the same predictions
+ * as cumf_test_rmse.  Row/column MPR and AUC (10-13) are not provided. */
+enum cumf_eval { CUMF_EVAL_RMSE = 0, CUMF_EVAL_MAE = 1, CUMF_EVAL_GKL = 2, CUMF_EVAL_LOGLOSS = 5, CUMF_EVAL_ACC = 6 };
+int cumf_test_eval(cumf_ctx* ctx, int32_t criterion, float mult, float b, double* value);
+
+/* Load a trained FP32 model (any k up to 4096, as mf_load_model reads it,
+ * test/mf.cpp:3706-3777) for evaluation: sets m, n, k and FP32 factors.
+ * With ratings present, m and n must match them; training still needs k in
+ * {32, 64, 128}. */
+int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* P, const float* Q);
+
 /* Pick the launch shape: time one epoch of every supported (lanes_per_sample,
  * pipeline_depth, workers for {32, 64, 128, 256, 512} waves per CU) for base's
  * schedule on this context's problem, write the fastest per rating into *best
@@ -200,6 +213,7 @@ int cumf_batch_add(cumf_batch* batch, const int32_t* u, const int32_t* v, const 
  * pipeline_depth; k and the lambdas come from each problem. */
 int cumf_batch_train(cumf_batch* batch, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end);
 int cumf_batch_test_rmse(cumf_batch* batch, int32_t index, float mult, float b, double* rmse);
+int cumf_batch_test_eval(cumf_batch* batch, int32_t index, int32_t criterion, float mult, float b, double* value);
 int cumf_batch_get_factors(cumf_batch* batch, int32_t index, int32_t which, void* data, float* scales);
 int cumf_batch_kernel_time(cumf_batch* batch, double* ms, int64_t* launches, int64_t* updates);
 
@@ -254,6 +268,10 @@ int cumf_grid_problem(cumf_node* R, int64_t nnz, int32_t m, int32_t n, int32_t u
  * read in parallel.  *R_out is malloc'd (free with cumf_free_host). */
 int cumf_read_problem(const char* path, cumf_node** R_out, int64_t* nnz, int32_t* m, int32_t* n);
 void cumf_free_host(void* p);
+/* mf_load_model (test/mf.cpp:3706-3777): the 20-byte header {fun, m, n, k, b}
+ * and the P and Q rows; *P and *Q are malloc'd (free with cumf_free_host).
+ * A missing or short file is CUMF_ERR_IO (the reference exits). */
+int cumf_load_model(const char* path, int32_t* m, int32_t* n, int32_t* k, float* b, float** P, float** Q);
 /* save_model (singleGPU/main.cpp:231-273): 20-byte header + P rows + Q rows. */
 int cumf_save_model(const char* path, int32_t m, int32_t n, int32_t k, float b,
                     const float* P, const float* Q);

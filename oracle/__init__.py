@@ -62,6 +62,7 @@ _sig = {
     "orc_scale_model": ([P, I64, ctypes.c_int, F], None),
     "orc_inv_shuffle": ([P, P, I32, ctypes.c_int], None),
     "orc_rmse": ([P, P, P, I64, P, P, I32, I32, ctypes.c_int, F], ctypes.c_double),
+    "orc_eval": ([ctypes.c_int, P, P, P, I64, P, P, I32, I32, ctypes.c_int, F], ctypes.c_double),
     "orc_save_model": ([ctypes.c_char_p, I32, I32, I32, F, P, P], ctypes.c_int),
 }
 for _n, (_a, _r) in _sig.items():
@@ -235,6 +236,20 @@ def rmse(u, v, r, P_, Q_, b):
     return _lib.orc_rmse(_p(u), _p(v), _p(r), len(u), _p(P_), _p(Q_), P_.shape[0], Q_.shape[0], P_.shape[1], b)
 
 
+RMSE, MAE, GKL, LOGLOSS, ACC = 0, 1, 2, 5, 6
+
+
+def evaluate(criterion, u, v, r, P_, Q_, b):
+    """mf-predict's -e criterion on float factors (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)."""
+    P_ = np.ascontiguousarray(P_, np.float32)
+    Q_ = np.ascontiguousarray(Q_, np.float32)
+    u = np.ascontiguousarray(u, np.int32)
+    v = np.ascontiguousarray(v, np.int32)
+    r = np.ascontiguousarray(r, np.float32)
+    return _lib.orc_eval(criterion, _p(u), _p(v), _p(r), len(u), _p(P_), _p(Q_), P_.shape[0], Q_.shape[0],
+                         P_.shape[1], b)
+
+
 def save_model(path, P_, Q_, b):
     P_ = np.ascontiguousarray(P_, np.float32)
     Q_ = np.ascontiguousarray(Q_, np.float32)
@@ -247,8 +262,18 @@ def have_mf_predict() -> bool:
 
 def mf_predict_rmse(test_path: str, model_path: str) -> str:
     """Run the reference's compiled evaluator: returns the 'RMSE = x.xxxx' value text."""
-    out = subprocess.run([MF_PREDICT, "-e", "0", test_path, model_path], capture_output=True, text=True, check=True)
+    return mf_predict_eval(test_path, model_path, RMSE)
+
+
+_EVAL_LABEL = {RMSE: "RMSE = ", MAE: "MAE = ", GKL: "GKL = ", LOGLOSS: "LOGLOSS = ", ACC: "ACCURACY = "}
+
+
+def mf_predict_eval(test_path: str, model_path: str, criterion: int) -> str:
+    """Run the reference's compiled evaluator with -e criterion: returns the
+    printed value text (4 decimals)."""
+    out = subprocess.run([MF_PREDICT, "-e", str(criterion), test_path, model_path], capture_output=True, text=True,
+                         check=True)
     for line in out.stdout.splitlines():
-        if line.startswith("RMSE = "):
+        if line.startswith(_EVAL_LABEL[criterion]):
             return line.split("=", 1)[1].strip()
-    raise RuntimeError(f"no RMSE line in mf-predict output:\n{out.stdout}")
+    raise RuntimeError(f"no {_EVAL_LABEL[criterion]!r} line in mf-predict output:\n{out.stdout}")

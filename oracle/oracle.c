@@ -691,6 +691,64 @@ EXPORT double orc_rmse(const int32_t* tu, const int32_t* tv, const float* tr, in
     return sqrt(loss / nnz);
 }
 
+/* mf_predict (test/mf.cpp:3794-3812) for one test rating. */
+static float predict_one(const float* P, const float* Q, int32_t m, int32_t n, int k, float b, int32_t u, int32_t v)
+{
+    if (u < 0 || u >= m || v < 0 || v >= n) return b;
+    const float* p = P + (int64_t)u * k;
+    const float* q = Q + (int64_t)v * k;
+    float z = 0.0f;
+    for (int j = 0; j < k; ++j) {
+        float t = p[j] * q[j];
+        z = z + t;
+    }
+    return isnan(z) ? b : z;
+}
+
+/* The other evaluation criteria of mf-predict -e (test/mf.h:30-40,
+ * test/mf-predict.cpp:129-162), each a double sum of a per-rating term whose
+ * arithmetic is the reference's:
+ *   1 MAE      abs(float(r - z))                          (mf.cpp:3841-3855)
+ *   2 GKL      float: r*logf(r/z) - r + z                 (mf.cpp:3857-3872)
+ *   5 LOGLOSS  log(1.0 + double(expf(-+z))), by r > 0     (mf.cpp:3874-3892)
+ *   6 ACC      r > 0 ? z > 0 : z < 0                      (mf.cpp:3894-3912)
+ * and the mean over nnz; 0 = RMSE as orc_rmse.  Returns NAN for others. */
+EXPORT double orc_eval(int criterion, const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz,
+                       const float* P, const float* Q, int32_t m, int32_t n, int k, float b)
+{
+    if (criterion == 0) return orc_rmse(tu, tv, tr, nnz, P, Q, m, n, k, b);
+    if (criterion != 1 && criterion != 2 && criterion != 5 && criterion != 6) return NAN;
+    if (nnz == 0) return 0;
+    double acc = 0;
+    for (int64_t i = 0; i < nnz; ++i) {
+        const float z = predict_one(P, Q, m, n, k, b, tu[i], tv[i]);
+        const float r = tr[i];
+        switch (criterion) {
+        case 1: {
+            float e = r - z;
+            acc += fabsf(e);
+            break;
+        }
+        case 2: {
+            float q = r / z;
+            float l = logf(q);
+            float t = r * l;
+            t = t - r;
+            t = t + z;
+            acc += t;
+            break;
+        }
+        case 5:
+            acc += r > 0 ? log(1.0 + (double)expf(-z)) : log(1.0 + (double)expf(z));
+            break;
+        default:
+            acc += r > 0 ? (z > 0 ? 1 : 0) : (z < 0 ? 1 : 0);
+            break;
+        }
+    }
+    return acc / nnz;
+}
+
 /* save_model, singleGPU/main.cpp:231-273: header {int 0, int m, int n, int k,
  * float b}, then m rows of P and n rows of Q as float32. */
 EXPORT int orc_save_model(const char* path, int32_t m, int32_t n, int32_t k, float b,

@@ -98,6 +98,12 @@ def test_serial_epoch_matches_oracle(gpu, k, G, prec, D):
     assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))
 
 
+def _close(got, want, rel):
+    if np.isnan(want):
+        return bool(np.isnan(got))
+    return abs(got - want) <= rel * abs(want) + 1e-15
+
+
 @pytest.mark.parametrize("prec", [0, 1, 3])
 def test_export_and_rmse_match_oracle(gpu, prec):
     import cumf_sgd_amd as cs
@@ -121,6 +127,10 @@ def test_export_and_rmse_match_oracle(gpu, prec):
         gotP = c.export_factors(0, mult, pmap)
         gotQ = c.export_factors(1, mult)
         got_rmse = c.test_rmse(mult, b)
+        got_eval = {crit: c.test_eval(crit, mult, b) for crit in (0, 1, 2, 5, 6)}
+        trs = np.where(np.arange(len(tr)) % 3 == 0, -1.0, 1.0).astype(np.float32)   # +-1 labels
+        c.set_test(tu, tv, trs)
+        got_eval_pm = {crit: c.test_eval(crit, mult, b) for crit in (5, 6)}
     wantP = oracle.widen(raw_p, sp, prec)
     wantQ = oracle.widen(raw_q, sq, prec)
     wantP_s = (wantP * mult).astype(np.float32)   # FP32 multiply, as __fmul_rn
@@ -129,6 +139,15 @@ def test_export_and_rmse_match_oracle(gpu, prec):
     assert np.array_equal(gotQ, wantQ_s)
     want_rmse = oracle.rmse(tu, tv, tr, wantP_s, wantQ_s, b)
     assert abs(got_rmse - want_rmse) <= 1e-12 * want_rmse
+    # mf-predict's other criteria: RMSE/MAE/ACC differ from the oracle only in
+    # the FP64 summation order; GKL and LOGLOSS also in the device logf/expf
+    # (a few ulp per term), so 1e-6 relative
+    for crit, tol in ((0, 1e-12), (1, 1e-12), (2, 1e-6), (5, 1e-6), (6, 1e-12)):
+        want = oracle.evaluate(crit, tu, tv, tr, wantP_s, wantQ_s, b)
+        assert _close(got_eval[crit], want, tol), (crit, got_eval[crit], want)
+    for crit, tol in ((5, 1e-6), (6, 1e-12)):
+        want = oracle.evaluate(crit, tu, tv, trs, wantP_s, wantQ_s, b)
+        assert _close(got_eval_pm[crit], want, tol), (crit, got_eval_pm[crit], want)
 
 
 def test_remap_scale_moments(gpu):

@@ -129,3 +129,45 @@ def test_cli_serial_mode_is_deterministic(gpu, tmp_path):
         assert res.returncode == 0, res.stderr
         outs.append(open(model, "rb").read())
     assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("k", [80, 128])
+def test_predict_cli_matches_reference_mf_predict(gpu, tmp_path, k):
+    """cumf_predict (GPU) prints what the reference's compiled mf-predict
+    prints for every supported -e, on model files of the reference's default
+    k=80 and k=128, with out-of-range test ids; set_model + test_eval agree
+    with the oracle."""
+    import cumf_sgd_amd as cs
+    import oracle
+    from cumf_sgd_amd import datagen
+
+    rng = np.random.default_rng(k)
+    m, n = 700, 300
+    P = np.abs(rng.normal(0.3, 0.15, (m, k))).astype(np.float32)
+    Q = np.abs(rng.normal(0.3, 0.15, (n, k))).astype(np.float32)
+    b = np.float32(3.4)
+    tu = rng.integers(-1, m + 2, 20_000).astype(np.int32)
+    tv = rng.integers(0, n, 20_000).astype(np.int32)
+    tr = rng.integers(1, 6, 20_000).astype(np.float32)
+    model, test = str(tmp_path / "model"), str(tmp_path / "test.bin")
+    cs.save_model(model, P, Q, b)
+    datagen.write_triplets(test, tu, tv, tr)
+    label = {0: "RMSE", 1: "MAE", 2: "GKL", 5: "LOGLOSS", 6: "ACCURACY"}
+    with cs.Context(0) as c:
+        c.set_model(P, Q)
+        c.set_test(tu, tv, tr)
+        for crit in (0, 1, 2, 5, 6):
+            got = c.test_eval(crit, 1.0, b)
+            want = oracle.evaluate(crit, tu, tv, tr, P, Q, b)
+            assert abs(got - want) <= (1e-6 if crit in (2, 5) else 1e-12) * abs(want), (crit, got, want)
+    for crit in (0, 1, 2, 5, 6):
+        out = subprocess.run([cs.PREDICT_PATH, "-e", str(crit), test, model], capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        line = out.stdout.strip().splitlines()[-1]
+        assert line.startswith(label[crit] + " = ")
+        value = line.split("=", 1)[1].strip()
+        want = f"{oracle.evaluate(crit, tu, tv, tr, P, Q, b):.4f}"
+        assert value == want, (crit, value, want)
+        if oracle.have_mf_predict():
+            assert value == oracle.mf_predict_eval(test, model, crit)

@@ -161,3 +161,38 @@ def test_cli_prints_reference_argument_block(tmp_path):
     assert "x_grid     :1" in out.stdout and "lambda_q   :0.02000" in out.stdout
     assert "model_path :missing.bin.model" in out.stdout
     assert out.returncode == 2 and "open failed" in out.stderr
+
+
+def test_load_model_round_trip_and_errors(tmp_path):
+    rng = np.random.default_rng(8)
+    P = rng.normal(size=(23, 80)).astype(np.float32)   # any k: the reference's default is 80
+    Q = rng.normal(size=(11, 80)).astype(np.float32)
+    path = str(tmp_path / "m")
+    cs.save_model(path, P, Q, 2.75)
+    P2, Q2, b = cs.load_model(path)
+    assert np.array_equal(P2, P) and np.array_equal(Q2, Q) and b == np.float32(2.75)
+    with pytest.raises(cs.CumfError) as e:
+        cs.load_model(str(tmp_path / "missing"))
+    assert e.value.code == 5
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[:-4])
+    with pytest.raises(cs.CumfError) as e:
+        cs.load_model(path)
+    assert e.value.code == 5 and "shorter" in str(e.value)
+
+
+def _predict(*args):
+    return subprocess.run([cs.PREDICT_PATH, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_predict_cli_usage_and_read_errors(tmp_path):
+    out = _predict()
+    assert out.returncode == 1 and "usage: cumf_predict" in out.stdout
+    out = _predict("-e", "10", "a", "b")
+    assert out.returncode == 1 and "unknown evaluation criterion" in out.stdout
+    out = _predict("-e")
+    assert out.returncode == 1 and "after -e" in out.stdout
+    out = _predict("only_one")
+    assert out.returncode == 1
+    out = _predict(str(tmp_path / "t.bin"), str(tmp_path / "missing_model"))
+    assert out.returncode == 2 and "cannot load model" in out.stderr

@@ -1,6 +1,7 @@
 """Pins of the CPU oracle against facts fixed outside it: published known-answer
 vectors, independent library implementations of the number formats, the
 reference's own compiled evaluator, closed forms and brute force."""
+import json
 import math
 import os
 
@@ -332,16 +333,41 @@ def test_rmse_and_model_file_match_reference_mf_predict(tmp_path):
     assert np.frombuffer(raw[16:20], "<f4")[0] == b
 
 
-def test_golden_fixtures():
-    """Committed vectors produced by the reference's compiled tools."""
-    here = os.path.join(os.path.dirname(__file__), "golden")
-    path = os.path.join(here, "mf_predict_rmse.json")
-    if not os.path.exists(path):
-        pytest.skip("golden fixtures not generated")
-    import json
+def _fmt4(x):
+    return "nan" if np.isnan(x) else f"{x:.4f}"
 
-    g = json.load(open(path))
+
+def test_evaluation_criteria_match_committed_mf_predict_vectors():
+    """RMSE, MAE, GKL, LOGLOSS and ACC of the oracle against the 4-decimal
+    values the reference's compiled mf-predict printed for the committed
+    cases (tests/golden/make_mf_predict_vectors.py): ratings 1-5 with
+    out-of-range ids, +-1 labels with a NaN factor row, a wide value range."""
+    g = json.load(open(os.path.join(GOLDEN, "mf_predict_vectors.json")))
+    assert len(g["cases"]) == 3
     for case in g["cases"]:
-        d = np.load(os.path.join(here, case["file"]))
-        mine = oracle.rmse(d["tu"], d["tv"], d["tr"], d["P"], d["Q"], np.float32(d["b"]))
-        assert f"{mine:.4f}" == case["rmse"]
+        d = np.load(os.path.join(GOLDEN, case["file"]))
+        for key, crit in (("rmse", 0), ("mae", 1), ("gkl", 2), ("logloss", 5), ("acc", 6)):
+            mine = oracle.evaluate(crit, d["tu"], d["tv"], d["tr"], d["P"], d["Q"], np.float32(d["b"]))
+            want = case[key].replace("-nan", "nan")
+            assert _fmt4(mine) == want, (case["file"], key, mine, want)
+
+
+@pytest.mark.skipif(not oracle.have_mf_predict(), reason="reference mf-predict not built")
+def test_evaluation_criteria_match_reference_mf_predict(tmp_path):
+    """Fresh random models against the live compiled evaluator, every -e."""
+    from cumf_sgd_amd import datagen
+
+    rng = np.random.default_rng(17)
+    for trial in range(3):
+        m, n, k = 40 + trial, 25 + trial, (8, 16, 32)[trial]
+        P = np.abs(rng.normal(0.5, 0.3, (m, k))).astype(np.float32)
+        Q = np.abs(rng.normal(0.5, 0.3, (n, k))).astype(np.float32)
+        tu = rng.integers(-1, m + 2, 2500).astype(np.int32)
+        tv = rng.integers(0, n, 2500).astype(np.int32)
+        tr = rng.integers(1, 6, 2500).astype(np.float32)
+        model, test = str(tmp_path / "m"), str(tmp_path / "t.bin")
+        oracle.save_model(model, P, Q, np.float32(2.5))
+        datagen.write_triplets(test, tu, tv, tr)
+        for crit in (0, 1, 2, 5, 6):
+            assert oracle.mf_predict_eval(test, model, crit) == _fmt4(
+                oracle.evaluate(crit, tu, tv, tr, P, Q, np.float32(2.5))), (trial, crit)
