@@ -121,6 +121,9 @@ def _load():
         "cumf_train": ([P, ctypes.POINTER(TrainOpts), I32, I32, P, I64], ctypes.c_int),
         "cumf_test_rmse": ([P, F32, F32, D], ctypes.c_int),
         "cumf_test_eval": ([P, I32, F32, F32, D], ctypes.c_int),
+        "cumf_read_ratings": ([P, ctypes.c_char_p, ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32)],
+                              ctypes.c_int),
+        "cumf_read_test": ([P, ctypes.c_char_p, ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_set_model": ([P, I32, I32, I32, P, P], ctypes.c_int),
         "cumf_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_reset_kernel_time": ([P], ctypes.c_int),
@@ -324,6 +327,20 @@ class Context:
         out = ctypes.c_double()
         _check(lib.cumf_test_rmse(self._h, float(mult), float(b), ctypes.byref(out)), "cumf_test_rmse")
         return out.value
+
+    def read_ratings(self, path: str):
+        """Stream the training file (or shards path0..path79) into the device
+        arrays; returns (nnz, m, n) as read_problem computes them."""
+        nnz, m, n = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.cumf_read_ratings(self._h, path.encode(), ctypes.byref(nnz), ctypes.byref(m), ctypes.byref(n)),
+               "cumf_read_ratings")
+        self.m, self.n, self.nnz = m.value, n.value, nnz.value
+        return nnz.value, m.value, n.value
+
+    def read_test(self, path: str) -> int:
+        nnz = ctypes.c_int64()
+        _check(lib.cumf_read_test(self._h, path.encode(), ctypes.byref(nnz)), "cumf_read_test")
+        return nnz.value
 
     def set_model(self, P: np.ndarray, Q: np.ndarray) -> None:
         """FP32 factors of any k for evaluation (cumf_set_model)."""

@@ -47,14 +47,14 @@ def build_library() -> str:
     csrc = os.path.join(PKG, "csrc")
     host = os.path.join(PKG, "host")
     hip_units = ["sgd_kernels", "sgd_update_f32", "sgd_update_f16", "sgd_update_bf16", "sgd_update_e4m3", "capi",
-                 "batch"]
+                 "batch", "loader"]
     jobs = [[HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, f"{u}.hip"), "-o", os.path.join(BUILD, f"{u}.o")]
             for u in hip_units]
     jobs.append(["g++", *HOST_FLAGS, "-c", os.path.join(host, "host.cpp"), "-o", os.path.join(BUILD, "host.o")])
     with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp,
           *[os.path.join(BUILD, f"{u}.o") for u in hip_units], os.path.join(BUILD, "host.o")])
     os.replace(tmp, LIB)
     for exe, src in ((CLI, "main.cpp"), (PREDICT, "predict.cpp")):

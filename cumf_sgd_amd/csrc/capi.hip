@@ -14,6 +14,7 @@
 
 #include "../../include/cumf_sgd_amd.h"
 #include "errors.h"
+#include "loader.h"
 #include "kernels.h"
 
 using namespace cumf;
@@ -332,6 +333,54 @@ int cumf_commit_ratings(cumf_ctx* c)
     if (!c->u && c->nnz) return fail(CUMF_ERR_STATE, "no rating buffers allocated");
     HIP_TRY(hipDeviceSynchronize());   // the caller may have filled them on any stream
     return after_ratings(c, c->nnz, c->m, c->n);
+}
+
+int cumf_read_ratings(cumf_ctx* c, const char* path, int64_t* nnz_out, int32_t* m_out, int32_t* n_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::vector<std::string> files;
+    std::vector<int64_t> counts;
+    bool sharded = false;
+    if ((rc = triplet_files(path, &files, &counts, &sharded))) return rc;
+    int64_t nnz = 0;
+    for (int64_t x : counts) nnz += x;
+    c->have_ratings = false;
+    if ((rc = ensure(&c->u, &c->u_cap, (size_t)nnz)) || (rc = ensure(&c->v, &c->v_cap, (size_t)nnz)) ||
+        (rc = ensure(&c->r, &c->r_cap, (size_t)nnz)))
+        return rc;
+    int32_t mm[3];
+    if ((rc = stream_triplets(files, counts, c->u, c->v, c->r, c->stream, mm))) return rc;
+    if (nnz > 0 && mm[2] < 0) return fail(CUMF_ERR_ARG, "%s holds negative ids", path);
+    // m, n = largest id + 1; a single file starts them at 1 (singleGPU/sgd.cu:202-203)
+    const int32_t m = std::max(sharded ? -1 : 1, mm[0] + 1), n = std::max(sharded ? -1 : 1, mm[1] + 1);
+    if ((rc = after_ratings(c, nnz, m, n))) return rc;
+    if (nnz_out) *nnz_out = nnz;
+    if (m_out) *m_out = m;
+    if (n_out) *n_out = n;
+    return CUMF_OK;
+}
+
+int cumf_read_test(cumf_ctx* c, const char* path, int64_t* nnz_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::vector<std::string> files;
+    std::vector<int64_t> counts;
+    bool sharded = false;
+    if ((rc = triplet_files(path, &files, &counts, &sharded))) return rc;
+    int64_t nnz = 0;
+    for (int64_t x : counts) nnz += x;
+    c->have_test = false;
+    if ((rc = ensure(&c->tu, &c->tu_cap, (size_t)nnz)) || (rc = ensure(&c->tv, &c->tv_cap, (size_t)nnz)) ||
+        (rc = ensure(&c->tr, &c->tr_cap, (size_t)nnz)))
+        return rc;
+    int32_t mm[3];
+    if ((rc = stream_triplets(files, counts, c->tu, c->tv, c->tr, c->stream, mm))) return rc;
+    c->tnnz = nnz;
+    c->have_test = true;
+    if (nnz_out) *nnz_out = nnz;
+    return CUMF_OK;
 }
 
 int cumf_set_test(cumf_ctx* c, const cumf_node* R, int64_t nnz)

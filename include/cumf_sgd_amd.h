@@ -93,6 +93,16 @@ int cumf_sync(cumf_ctx* ctx);
 int cumf_set_ratings(cumf_ctx* ctx, const cumf_node* R, int64_t nnz, int32_t m, int32_t n);
 int cumf_set_ratings_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, const float* r,
                          int64_t nnz, int32_t m, int32_t n);
+/* read_problem (singleGPU/sgd.cu:190-335) fused with the upload: reads the
+ * file, or the shards path0..path79 when any exist, straight into the
+ * context's device arrays through a ring of pinned buffers and reader
+ * threads (no full host copy).  A file's trailing partial record is dropped;
+ * m and n are the largest ids + 1 (at least 1 for a single file).  Negative
+ * ids are an error; a missing or unreadable file is CUMF_ERR_IO. */
+int cumf_read_ratings(cumf_ctx* ctx, const char* path, int64_t* nnz, int32_t* m, int32_t* n);
+/* The same for the test ratings (any ids; those outside the model predict b). */
+int cumf_read_test(cumf_ctx* ctx, const char* path, int64_t* nnz);
+
 /* Test ratings for cumf_test_rmse (ids outside [0,m) x [0,n) predict b,
  * test/mf.cpp:3796-3797). */
 int cumf_set_test(cumf_ctx* ctx, const cumf_node* R, int64_t nnz);

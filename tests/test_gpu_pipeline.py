@@ -171,3 +171,85 @@ def test_predict_cli_matches_reference_mf_predict(gpu, tmp_path, k):
         assert value == want, (crit, value, want)
         if oracle.have_mf_predict():
             assert value == oracle.mf_predict_eval(test, model, crit)
+
+
+def test_streaming_reader_matches_read_problem(gpu, tmp_path):
+    """cumf_read_ratings / cumf_read_test: the records of a single file (with
+    a trailing partial record) and of shards path0..path3 land in the device
+    arrays in file order, bit for bit, with read_problem's m and n; negative
+    ids and missing files are errors."""
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    rng = np.random.default_rng(21)
+    # single file spanning several reader chunks (4M records each)
+    nnz = 9_000_001
+    u = rng.integers(0, 70_000, nnz).astype(np.int32)
+    v = rng.integers(0, 5_000, nnz).astype(np.int32)
+    r = rng.random(nnz).astype(np.float32)
+    single = str(tmp_path / "single.bin")
+    datagen.write_triplets(single, u, v, r)
+    with open(single, "ab") as f:
+        f.write(b"\x07" * 7)
+    want_nodes, wm, wn = cs.read_problem(single)
+    with cs.Context(0) as c:
+        got_nnz, m, n = c.read_ratings(single)
+        assert (got_nnz, m, n) == (nnz, wm, wn)
+        gu, gv, gr = c.get_ratings()
+        assert np.array_equal(gu, u) and np.array_equal(gv, v) and np.array_equal(gr.view(np.uint32), r.view(np.uint32))
+        assert c.read_test(single) == nnz
+    # shards of uneven sizes, one empty
+    parts = []
+    for i, k in enumerate((1000, 0, 4_200_000, 37)):
+        pu = rng.integers(0, 900, k).astype(np.int32)
+        pv = rng.integers(0, 300, k).astype(np.int32)
+        pr = rng.random(k).astype(np.float32)
+        datagen.write_triplets(str(tmp_path / f"sh{i}"), pu, pv, pr)
+        parts.append((pu, pv, pr))
+    want_nodes, wm, wn = cs.read_problem(str(tmp_path / "sh"))
+    with cs.Context(0) as c:
+        got_nnz, m, n = c.read_ratings(str(tmp_path / "sh"))
+        assert (got_nnz, m, n) == (len(want_nodes), wm, wn)
+        gu, gv, gr = c.get_ratings()
+        assert np.array_equal(gu, want_nodes["u"]) and np.array_equal(gv, want_nodes["v"])
+        assert np.array_equal(gr, want_nodes["rate"])
+        # negative ids
+        bad = str(tmp_path / "bad.bin")
+        datagen.write_triplets(bad, np.array([0, -3], np.int32), np.array([1, 2], np.int32), np.ones(2, np.float32))
+        with pytest.raises(cs.CumfError) as e:
+            c.read_ratings(bad)
+        assert e.value.code == 1 and "negative" in str(e.value)
+        with pytest.raises(cs.CumfError) as e:
+            c.read_ratings(str(tmp_path / "nope.bin"))
+        assert e.value.code == 5
+
+
+def test_streaming_reader_throughput(gpu, tmp_path):
+    """The 99M-rating workload's training file (1.19 GB) from the page cache
+    to the device: streaming reader vs read_problem + upload; both give the
+    same arrays.  Rates are printed (this measures the host side)."""
+    import time
+
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    R = datagen.medium(seed=2, device="cpu")
+    u, v, r = R.u.numpy(), R.v.numpy(), R.r.numpy()
+    path = str(tmp_path / "medium.bin")
+    datagen.write_triplets(path, u, v, r)
+    del R
+    with cs.Context(0) as c:
+        c.read_ratings(path)          # warm the page cache and the runtime
+        t0 = time.perf_counter()
+        nnz, m, n = c.read_ratings(path)
+        t_stream = time.perf_counter() - t0
+        gu, gv, gr = c.get_ratings()
+        assert np.array_equal(gu, u) and np.array_equal(gv, v) and np.array_equal(gr, r)
+        t0 = time.perf_counter()
+        nodes, m2, n2 = cs.read_problem(path)
+        c.set_ratings_nodes(nodes, m2, n2)
+        t_host = time.perf_counter() - t0
+    gb = 12 * nnz / 1e9
+    print(f"streaming reader {gb / t_stream:.2f} GB/s ({t_stream:.2f} s), read_problem + upload "
+          f"{gb / t_host:.2f} GB/s ({t_host:.2f} s)")
+    assert (m, n) == (m2, n2)
