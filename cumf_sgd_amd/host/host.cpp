@@ -103,6 +103,10 @@ cumf_train_opts opts_from(const cumf_param& p, const cumf_train_opts* extra, flo
     return o;
 }
 
+int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extra, const std::vector<int32_t>& order,
+                 float ave, float std_dev, int32_t precision, uint64_t seed, bool eval, const int32_t* pmap,
+                 const int32_t* qmap, float* P_out, float* Q_out, float* b_out, double* rmse_out);
+
 }  // namespace
 
 extern "C" {
@@ -359,7 +363,6 @@ int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* extra, const c
     // collect_data + scale (singleGPU/sgd.cu:644-649)
     float ave = 0.0f, std_dev = 0.0f;
     CUMF_TRY(cumf_collect_data(R, nnz, &ave, &std_dev));
-    const float scale = std::max(1e-4f, std_dev);
     // id permutations (sgd.cu:656-659): both maps restart srand(seed)
     std::vector<int32_t> pmap((size_t)m), qmap((size_t)n);
     CUMF_TRY(cumf_gen_random_map((uint32_t)seed, m, pmap.data()));
@@ -397,8 +400,79 @@ int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* extra, const c
         }
         order = reference_block_order(p, p.num_iters);
     }
-    CUMF_TRY(cumf_scale_ratings(g.c, (float)(1.0 / scale)));   // sgd.cu:672
-    CUMF_TRY(cumf_init_factors(g.c, p.k, precision, seed));
+    return train_loaded(g.c, p, extra, order, ave, std_dev, precision, seed, test && test_nnz > 0 && rmse_out,
+                        pmap.data(), qmap.data(), P_out, Q_out, b_out, rmse_out);
+}
+
+int cumf_sgd_train_file(const cumf_param* para, const cumf_train_opts* extra, const char* train_path,
+                        const char* test_path, int32_t precision, uint64_t seed, float** P_out, float** Q_out,
+                        int32_t* m_out, int32_t* n_out, int64_t* nnz_out, float* b_out, double* rmse_out)
+{
+    if (!para || !train_path || !P_out || !Q_out || !m_out || !n_out || !b_out)
+        return fail(CUMF_ERR_ARG, "bad sgd_train_file arguments");
+    *P_out = *Q_out = nullptr;
+    const cumf_param& p = *para;
+    if (p.ux * p.vy != 1) return fail(CUMF_ERR_ARG, "the file pipeline trains the 1x1 grid only (use cumf_sgd_train)");
+    for (const char* f : {train_path, test_path}) {   // file errors before device errors, as read_problem
+        if (!f || !*f) continue;
+        bool found = access(f, F_OK) != -1;
+        for (int i = 0; i < 80 && !found; ++i) found = access((std::string(f) + std::to_string(i)).c_str(), F_OK) != -1;
+        if (!found) return fail(CUMF_ERR_IO, "file %s open failed", f);
+    }
+    CtxGuard g;
+    CUMF_TRY(cumf_create(p.gpu, &g.c));
+    int64_t nnz = 0;
+    int32_t m = 0, n = 0;
+    CUMF_TRY(cumf_read_ratings(g.c, train_path, &nnz, &m, &n));
+    if (nnz <= 0) return fail(CUMF_ERR_ARG, "%s holds no ratings", train_path);
+    const bool eval = test_path && *test_path && rmse_out;
+    if (test_path && *test_path) CUMF_TRY(cumf_read_test(g.c, test_path, nullptr));
+    // collect_data (sgd.cu:78-96) on the device sums (FP64, another order)
+    double s1 = 0, s2 = 0;
+    CUMF_TRY(cumf_rating_moments(g.c, &s1, &s2));
+    const double ex = s1 / (double)nnz, ex2 = s2 / (double)nnz;
+    const float ave = (float)ex, std_dev = (float)std::sqrt(ex2 - ex * ex);
+    std::vector<int32_t> pmap((size_t)m), qmap((size_t)n);
+    CUMF_TRY(cumf_gen_random_map((uint32_t)seed, m, pmap.data()));
+    CUMF_TRY(cumf_gen_random_map((uint32_t)seed, n, qmap.data()));
+    CUMF_TRY(cumf_remap(g.c, pmap.data(), qmap.data()));
+    float* P = (float*)std::malloc(sizeof(float) * (size_t)m * p.k);
+    float* Q = (float*)std::malloc(sizeof(float) * (size_t)n * p.k);
+    if (!P || !Q) {
+        std::free(P);
+        std::free(Q);
+        return fail(CUMF_ERR_NOMEM, "model does not fit in host memory");
+    }
+    const std::vector<int32_t> no_order;
+    int rc = train_loaded(g.c, p, extra, no_order, ave, std_dev, precision, seed, eval, pmap.data(), qmap.data(), P,
+                          Q, b_out, rmse_out);
+    if (rc != CUMF_OK) {
+        std::free(P);
+        std::free(Q);
+        return rc;
+    }
+    *P_out = P;
+    *Q_out = Q;
+    *m_out = m;
+    *n_out = n;
+    if (nnz_out) *nnz_out = nnz;
+    return CUMF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scale, init, train, export: the part of sgd_train (singleGPU/sgd.cu:670-690)
+// after the ratings are on the device in their permuted ids.
+int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extra, const std::vector<int32_t>& order,
+                 float ave, float std_dev, int32_t precision, uint64_t seed, bool eval, const int32_t* pmap,
+                 const int32_t* qmap, float* P_out, float* Q_out, float* b_out, double* rmse_out)
+{
+    const int ux = p.ux, vy = p.vy;
+    const float scale = std::max(1e-4f, std_dev);
+    CUMF_TRY(cumf_scale_ratings(ctx, (float)(1.0 / scale)));   // sgd.cu:672
+    CUMF_TRY(cumf_init_factors(ctx, p.k, precision, seed));
     const float b = ave / std_dev;   // init_model(tr, k, ave/std_dev), sgd.cu:679
     const float fs = std::sqrt(scale);   // scale_model, sgd.cu:598
 
@@ -406,9 +480,8 @@ int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* extra, const c
     if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID;
     if (o.schedule == CUMF_SCHED_STRATUM) {
         const int s = ux == vy && ux > 1 ? ux : 8;
-        CUMF_TRY(cumf_partition_strata(g.c, s, nullptr));
+        CUMF_TRY(cumf_partition_strata(ctx, s, nullptr));
     }
-    const bool eval = test && test_nnz > 0 && rmse_out;
     for (int ep = 0; ep < p.num_iters; ++ep) {
         const int32_t* ord = nullptr;
         int64_t olen = 0;
@@ -416,15 +489,15 @@ int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* extra, const c
             ord = order.data() + (size_t)ep * ux * vy;
             olen = (int64_t)ux * vy;
         }
-        CUMF_TRY(cumf_train(g.c, &o, ep, ep + 1, ord, olen));
-        if (eval) CUMF_TRY(cumf_test_rmse(g.c, fs, b, &rmse_out[ep]));
+        CUMF_TRY(cumf_train(ctx, &o, ep, ep + 1, ord, olen));
+        if (eval) CUMF_TRY(cumf_test_rmse(ctx, fs, b, &rmse_out[ep]));
     }
     // export in original id order: row u of the model is shuffled row pmap[u]
     // (scale_model + shuffle_model, sgd.cu:593-635)
-    CUMF_TRY(cumf_export_factors(g.c, 0, fs, pmap.data(), P_out));
-    CUMF_TRY(cumf_export_factors(g.c, 1, fs, qmap.data(), Q_out));
+    CUMF_TRY(cumf_export_factors(ctx, 0, fs, pmap, P_out));
+    CUMF_TRY(cumf_export_factors(ctx, 1, fs, qmap, Q_out));
     *b_out = b;
     return CUMF_OK;
 }
 
-}  // extern "C"
+}  // namespace

@@ -11,7 +11,9 @@
 //   --schedule S    hogwild | grid | stratum
 //   --lanes G       lanes per update (8..64)
 //   --coherent      device-scope coherent factor accesses
-// With -p the test RMSE is printed after every epoch.
+// With -p the test RMSE is printed after every epoch.  A 1x1 grid streams the
+// files straight into device memory (cumf_sgd_train_file); gridded modes keep
+// the reference's host shuffle and cycle-swap partition (cumf_sgd_train).
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -162,6 +164,39 @@ int main(int argc, char** argv)
         std::printf("%s\n", e.what());
         return 1;
     }
+    cumf_train_opts o;
+    std::memset(&o, 0, sizeof(o));
+    o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID);
+    o.lanes_per_sample = a.lanes;
+    o.coherent = a.coherent;
+    std::vector<double> rmse((size_t)(a.p.num_iters > 0 ? a.p.num_iters : 1), 0.0);
+    float b = 0.0f;
+    if (a.p.ux * a.p.vy == 1) {
+        // 1x1 grid: the files stream straight into HBM, everything else on the device
+        float *P = nullptr, *Q = nullptr;
+        int32_t m = 0, n = 0;
+        int64_t nnz = 0;
+        const bool test = !a.va_path.empty();
+        int rc = cumf_sgd_train_file(&a.p, &o, a.tr_path.c_str(), test ? a.va_path.c_str() : nullptr, a.precision,
+                                     a.seed, &P, &Q, &m, &n, &nnz, &b, test ? rmse.data() : nullptr);
+        if (rc != CUMF_OK) {
+            std::fprintf(stderr, "%s%s\n", rc == CUMF_ERR_IO ? "" : "training failed: ", cumf_last_error());
+            return rc == CUMF_ERR_IO ? 2 : 3;
+        }
+        std::printf("m:%d, n:%d, nnz:%lld\n", m, n, (long long)nnz);
+        if (test)
+            for (int ep = 0; ep < a.p.num_iters; ++ep) std::printf("iter %3d  test RMSE %.4f\n", ep, rmse[ep]);
+        std::printf("save_model() is called\n");
+        rc = cumf_save_model(a.model_path.c_str(), m, n, a.p.k, b, P, Q);
+        cumf_free_host(P);
+        cumf_free_host(Q);
+        if (rc != CUMF_OK) {
+            std::fprintf(stderr, "%s\n", cumf_last_error());
+            return 4;
+        }
+        std::printf("\ntraining application finished...\n\n\n");
+        return 0;
+    }
     cumf_node* R = nullptr;
     int64_t nnz = 0;
     int32_t m = 0, n = 0;
@@ -179,13 +214,6 @@ int main(int argc, char** argv)
         return 2;
     }
     std::vector<float> P((size_t)m * a.p.k), Q((size_t)n * a.p.k);
-    std::vector<double> rmse((size_t)(a.p.num_iters > 0 ? a.p.num_iters : 1), 0.0);
-    float b = 0.0f;
-    cumf_train_opts o;
-    std::memset(&o, 0, sizeof(o));
-    o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID);
-    o.lanes_per_sample = a.lanes;
-    o.coherent = a.coherent;
     int rc = cumf_sgd_train(&a.p, &o, R, nnz, m, n, a.precision, a.seed, T, tnnz, P.data(), Q.data(), &b,
                             T ? rmse.data() : nullptr);
     if (rc != CUMF_OK) {

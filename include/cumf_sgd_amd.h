@@ -265,6 +265,16 @@ int cumf_sgd_train(const cumf_param* para, const cumf_train_opts* opts, const cu
                    const cumf_node* test, int64_t test_nnz, float* P_out, float* Q_out,
                    float* b_out, double* rmse_out);
 
+/* sgd_train for a 1x1 grid straight from the files: read_problem's inputs
+ * streamed into HBM (cumf_read_ratings / cumf_read_test), collect_data on
+ * device FP64 sums, the same maps, scaling, training and export as
+ * cumf_sgd_train.  *P_out and *Q_out (m x k, n x k, original ids, times
+ * sqrt(scale)) are malloc'd; free with cumf_free_host.  rmse_out, when not
+ * NULL and test_path is given, receives num_iters test RMSEs. */
+int cumf_sgd_train_file(const cumf_param* para, const cumf_train_opts* opts, const char* train_path,
+                        const char* test_path, int32_t precision, uint64_t seed, float** P_out, float** Q_out,
+                        int32_t* m, int32_t* n, int64_t* nnz, float* b_out, double* rmse_out);
+
 /* ---- host helpers (the reference's host pipeline, bit-exact) ------------- */
 /* gen_random_map (singleGPU/sgd.cu:39-53) with srand(seed) in place of
  * srand(time(NULL)). */

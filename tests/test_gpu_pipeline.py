@@ -253,3 +253,32 @@ def test_streaming_reader_throughput(gpu, tmp_path):
     print(f"streaming reader {gb / t_stream:.2f} GB/s ({t_stream:.2f} s), read_problem + upload "
           f"{gb / t_host:.2f} GB/s ({t_host:.2f} s)")
     assert (m, n) == (m2, n2)
+
+
+def test_cli_file_pipeline_equals_host_pipeline_in_serial_mode(gpu, tmp_path):
+    """The CLI's 1x1 path (files streamed into HBM, device statistics) and
+    cumf_sgd_train (host read_problem, host collect_data) give the same model
+    file bit for bit in serial mode: integer ratings make the FP64 sums
+    order-independent, so every later step sees the same numbers."""
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    R = _data(nnz=40_000, n_test=4_000)
+    u, v, r, tu, tv, tr = R.numpy()
+    tr_path, te_path = str(tmp_path / "train.bin"), str(tmp_path / "test.bin")
+    datagen.write_triplets(tr_path, u, v, r)
+    datagen.write_triplets(te_path, tu, tv, tr)
+    model = str(tmp_path / "model")
+    out = subprocess.run([cs.CLI_PATH, "-k", "64", "-t", "3", "-l", "0.05", "-a", "0.08", "-b", "0.3", "--serial",
+                          "--seed", "6", "-p", te_path, tr_path, model], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rm = [l.split()[-1] for l in out.stdout.splitlines() if "test RMSE" in l]
+    nodes, m, n = cs.read_problem(tr_path)
+    tnodes, _, _ = cs.read_problem(te_path)
+    p = cs.make_param(k=64, num_iters=3, lambda_p=0.05, lambda_q=0.05, alpha=0.08, beta=0.3)
+    P, Q, b, rmse = cs.sgd_train(p, nodes, m, n, precision=cs.FP16, seed=6, test=tnodes,
+                                 opts=cs.make_opts(k=64, schedule=cs.SERIAL))
+    host_model = str(tmp_path / "host_model")
+    cs.save_model(host_model, P, Q, b)
+    assert open(model, "rb").read() == open(host_model, "rb").read()
+    assert rm == [f"{x:.4f}" for x in rmse]
