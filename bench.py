@@ -5,7 +5,10 @@ Workload (one GPU): the medium Netflix-shaped problem, 480,189 users x
 k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated (seeded) on the host
 and copied to HBM.  One
 step = one epoch of batch-Hogwild SGD over all training ratings (the
-reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244).
+reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
+by default under the stratum schedule (8 x 8 nnz-balanced blocks, the 8
+conflict-free blocks of a diagonal on the 8 XCDs, one launch per diagonal;
+--schedule hogwild runs the reference's 1x1 mode).
 Inputs are resident in HBM before the timed region starts.
 
 Printed: one JSON line with value (rating updates/s of the timed epochs),
@@ -101,11 +104,12 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
-def measured_traffic(prec: int, info):
+def measured_traffic(prec: int, info, schedule: str):
     """HBM bytes per launch of the update kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), when it was
-    measured on the same kernel shape; None otherwise."""
-    path = os.path.join(ROOT, "profiles", f"sgd_update_{PREC_NAME[prec]}_medium.json")
+    FETCH_SIZE/WRITE_SIZE summary of this workload and schedule (profiles/),
+    when it was measured on the same kernel shape; None otherwise."""
+    name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8"}[prec]
+    path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
         return None
     d = json.load(open(path))
@@ -121,7 +125,10 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "fp8"])
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--schedule", default="hogwild", choices=["hogwild", "stratum"])
+    ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum"],
+                    help="stratum: s x s conflict-free blocks, block i on XCD i mod 8 (default); hogwild: the "
+                         "reference's 1x1 mode")
+    ap.add_argument("--strata", type=int, default=8, help="s for the stratum schedule (s x s blocks)")
     ap.add_argument("--rmse-epochs", type=int, default=20)
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
@@ -148,6 +155,7 @@ def main() -> int:
         return 2
     prec = cs.PRECISIONS[args.precision]
     sched = cs.SCHEDULES[args.schedule]
+    sched_tag = f"stratum{args.strata}" if sched == cs.STRATUM else "hogwild"
     t0 = time.perf_counter()
     # generated on the host (deterministic across machines), then made resident in HBM
     torch.set_num_threads(min(16, os.cpu_count() or 1))
@@ -166,7 +174,7 @@ def main() -> int:
     ctx = cs.Context(0)
     scale, b, lam = prepare(ctx, R, args.seed)
     if sched == cs.STRATUM:
-        ctx.partition_strata(8)
+        ctx.partition_strata(args.strata)
     torch.cuda.synchronize()
     log(f"preprocessed: scale={scale:.5f} b={b:.5f}")
 
@@ -179,7 +187,7 @@ def main() -> int:
         opts, tuned_ms = ctx.autotune(opts)
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
             f"workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
-        args.workers, args.lanes, args.depth = opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth
+    tuned = (opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth)
     for ep in range(args.warmup):
         ctx.train(opts, ep, ep + 1)
     ctx.sync()
@@ -207,8 +215,10 @@ def main() -> int:
         ttt = {}
         curves = {}
         for p in (0, 1, 2, 3):
-            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, args.workers, args.lanes,
-                                      sched, args.depth)
+            # the headline format trains with the measured shape, the others
+            # with their defaults (or what the command line fixed)
+            w, l, d = tuned if p == prec else (args.workers, args.lanes, args.depth)
+            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d)
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005
@@ -254,11 +264,11 @@ def main() -> int:
         "data": "synthetic (seeded, generated on the host, resident in HBM)",
         "config": {"workload": "medium Netflix-shaped MF: 480189 x 17770, 99072112 train + 1408395 test, k=128, "
                                "lambda 0.05, alpha 0.08, beta 0.3, batch-Hogwild",
-                   "precision": args.precision, "schedule": args.schedule, "workers": info.num_workers,
+                   "precision": args.precision, "schedule": sched_tag, "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info, sched_tag),
                      "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upd_per_launch},
         "time_to_target": ttt,

@@ -1156,9 +1156,9 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
                 }
                 c->kernel_ms = 0;
                 if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
-                // per-rating time: shapes differ in how much they over-sample
-                const int64_t upd = std::max<int64_t>(1, c->last.updates_per_epoch);
-                const double t = c->kernel_ms * (double)c->nnz / (double)upd;
+                // the epoch's time, over-sampling included: that is what an
+                // epoch costs the caller
+                const double t = c->kernel_ms;
                 if (t < best_t) {
                     best_t = t;
                     bo = o;

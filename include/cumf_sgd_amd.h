@@ -178,9 +178,9 @@ int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* 
 
 /* Pick the launch shape: time one epoch of every supported (lanes_per_sample,
  * pipeline_depth, workers for {32, 64, 128, 256, 512} waves per CU) for base's
- * schedule on this context's problem, write the fastest per rating into *best
- * (other fields copied from base) and its kernel time per nnz updates in ms
- * into *best_ms, then restore the factors the trials trained.  Kernel-time
+ * schedule on this context's problem, write the one with the shortest epoch
+ * into *best (other fields copied from base) and its epoch's kernel time in
+ * ms into *best_ms, then restore the factors the trials trained.  Kernel-time
  * totals and last_launch are not changed.  A serial schedule returns base. */
 int cumf_autotune(cumf_ctx* ctx, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms);
 
