@@ -36,8 +36,9 @@ HBM_PEAK_GBS = 8000.0
 K = 128
 # algorithmic bytes per update at k=128: 12-byte rating + P and Q rows read and
 # written (+ the two FP32 row scales read, FP8)
-BYTES_PER_UPDATE = {0: 12 + 4 * K * 4, 1: 12 + 4 * K * 2, 2: 12 + 4 * K * 2, 3: 12 + 4 * K * 1 + 2 * 4}
-PREC_NAME = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8_e4m3"}
+BYTES_PER_UPDATE = {0: 12 + 4 * K * 4, 1: 12 + 4 * K * 2, 2: 12 + 4 * K * 2, 3: 12 + 4 * K * 1 + 2 * 4,
+                    4: 12 + 4 * K * 2}
+PREC_NAME = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8_e4m3", 4: "bf16_sr"}
 
 
 def log(msg: str) -> None:
@@ -122,7 +123,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "fp8"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "bf16sr", "fp8"])
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum"],
@@ -214,7 +215,7 @@ def main() -> int:
     if not args.no_rmse:
         ttt = {}
         curves = {}
-        for p in (0, 1, 2, 3):
+        for p in (0, 1, 2, 4, 3):
             # the headline format trains with the measured shape, the others
             # with their defaults (or what the command line fixed)
             w, l, d = tuned if p == prec else (args.workers, args.lanes, args.depth)
@@ -260,7 +261,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3"}[prec] + " storage / f32 math",
+        "dtype": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3", 4: "bf16 (stochastic rounding)"}[prec]
+                 + " storage / f32 math",
         "data": "synthetic (seeded, generated on the host, resident in HBM)",
         "config": {"workload": "medium Netflix-shaped MF: 480189 x 17770, 99072112 train + 1408395 test, k=128, "
                                "lambda 0.05, alpha 0.08, beta 0.3, batch-Hogwild",

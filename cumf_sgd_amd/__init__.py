@@ -27,14 +27,14 @@ LIB_PATH = os.path.join(_HERE, "libcumf_sgd_amd.so")
 CLI_PATH = os.path.join(_HERE, "cumf_sgd")
 PREDICT_PATH = os.path.join(_HERE, "cumf_predict")
 
-FP32, FP16, BF16, FP8_E4M3 = 0, 1, 2, 3
-PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3}
+FP32, FP16, BF16, FP8_E4M3, BF16_SR = 0, 1, 2, 3, 4
+PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3, "bf16sr": BF16_SR}
 HOGWILD, SERIAL, GRID, STRATUM = 0, 1, 2, 3
 # mf-predict -e criteria (test/mf.h:30-40)
 EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
 SCHEDULES = {"hogwild": HOGWILD, "serial": SERIAL, "grid": GRID, "stratum": STRATUM}
 ELEM_BYTES = {FP32: 4, FP16: 2, BF16: 2, FP8_E4M3: 1}
-STORAGE_DTYPE = {FP32: np.float32, FP16: np.uint16, BF16: np.uint16, FP8_E4M3: np.uint8}
+STORAGE_DTYPE = {FP32: np.float32, FP16: np.uint16, BF16: np.uint16, FP8_E4M3: np.uint8, BF16_SR: np.uint16}
 
 NODE_DTYPE = np.dtype([("u", "<i4"), ("v", "<i4"), ("rate", "<f4")])
 
@@ -149,6 +149,7 @@ def _load():
         "cumf_batch_train": ([P, ctypes.POINTER(TrainOpts), I32, I32], ctypes.c_int),
         "cumf_batch_test_rmse": ([P, I32, F32, F32, D], ctypes.c_int),
         "cumf_batch_test_eval": ([P, I32, I32, F32, F32, D], ctypes.c_int),
+        "cumf_batch_autotune": ([P, ctypes.POINTER(TrainOpts), ctypes.POINTER(TrainOpts), D], ctypes.c_int),
         "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
         "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
     }
@@ -425,6 +426,15 @@ class Batch:
         _check(lib.cumf_batch_test_rmse(self._h, index, float(mult), float(b), ctypes.byref(out)),
                "cumf_batch_test_rmse")
         return out.value
+
+    def autotune(self, base: "TrainOpts"):
+        """Fastest total waves and pipeline depth for this batch (factors left
+        unchanged); returns (opts, epoch milliseconds)."""
+        best = TrainOpts()
+        ms = ctypes.c_double()
+        _check(lib.cumf_batch_autotune(self._h, ctypes.byref(base), ctypes.byref(best), ctypes.byref(ms)),
+               "cumf_batch_autotune")
+        return best, ms.value
 
     def test_eval(self, index: int, criterion: int, mult: float = 1.0, b: float = 0.0) -> float:
         out = ctypes.c_double()

@@ -167,7 +167,7 @@ int cumf_batch_create(int device, int32_t precision, cumf_batch** out)
 {
     if (!out) return fail(CUMF_ERR_ARG, "null output pointer");
     *out = nullptr;
-    if (precision < CUMF_FP32 || precision > CUMF_FP8_E4M3) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
+    if (precision < CUMF_FP32 || precision > CUMF_BF16_SR) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count == 0) return fail(CUMF_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
@@ -297,7 +297,8 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     // waves per problem in proportion to its bytes of work; no more workers
     // than windows
     const bool serial = o->schedule == CUMF_SCHED_SERIAL;
-    const int64_t total_waves = o->num_workers > 0 ? o->num_workers : (int64_t)b->cu_count * 16;
+    // default 64 waves per CU: the batch autotuner's pick on the 256-problem configuration
+    const int64_t total_waves = o->num_workers > 0 ? o->num_workers : (int64_t)b->cu_count * 64;
     double cost_sum = 0;
     for (const Problem& p : b->probs) cost_sum += (double)p.nnz * (12.0 + 4.0 * p.k * eb_of(b->prec));
     std::vector<SgdTask> tasks;
@@ -366,6 +367,78 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     float ms = 0;
     BHIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->kernel_ms += ms;
+    return CUMF_OK;
+}
+
+int cumf_batch_autotune(cumf_batch* b, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!base || !best) return fail(CUMF_ERR_ARG, "null options");
+    if (base->schedule == CUMF_SCHED_SERIAL || b->probs.empty()) {
+        *best = *base;
+        if (best_ms) *best_ms = 0;
+        return CUMF_OK;
+    }
+    // keep every problem's factors (and FP8 scales) to put them back
+    const size_t eb = (size_t)eb_of(b->prec);
+    std::vector<void*> keep;
+    struct Free {
+        std::vector<void*>* v;
+        ~Free() { for (void* p : *v) dev_free(p); }
+    } guard{&keep};
+    auto save = [&](const void* src, size_t bytes) -> int {
+        void* d = nullptr;
+        int r = dev_alloc(&d, bytes);
+        if (r) return r;
+        keep.push_back(d);
+        if (bytes) BHIP_TRY(hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, b->stream));
+        return CUMF_OK;
+    };
+    for (const Problem& p : b->probs) {
+        if ((rc = save(p.P, (size_t)p.m * p.k * eb)) || (rc = save(p.Q, (size_t)p.n * p.k * eb))) return rc;
+        if (b->prec == CUMF_FP8_E4M3 && ((rc = save(p.ps, 4 * (size_t)p.m)) || (rc = save(p.qs, 4 * (size_t)p.n))))
+            return rc;
+    }
+    const double keep_ms = b->kernel_ms;
+    const int64_t keep_l = b->launches, keep_u = b->updates;
+    const int waves_per_cu[5] = {4, 8, 16, 32, 64};
+    const int depths[3] = {2, 4, 8};
+    double best_t = 1e300;
+    cumf_train_opts bo = *base;
+    bool warm = false;
+    for (int wpc : waves_per_cu)
+        for (int D : depths) {
+            cumf_train_opts o = *base;
+            o.num_workers = b->cu_count * wpc;
+            o.pipeline_depth = D;
+            if (!warm) {
+                if ((rc = cumf_batch_train(b, &o, 0, 1))) return rc;
+                warm = true;
+            }
+            const double before = b->kernel_ms;
+            if ((rc = cumf_batch_train(b, &o, 0, 1))) return rc;
+            const double t = b->kernel_ms - before;
+            if (t < best_t) {
+                best_t = t;
+                bo = o;
+            }
+        }
+    size_t j = 0;
+    for (const Problem& p : b->probs) {
+        BHIP_TRY(hipMemcpyAsync(p.P, keep[j++], (size_t)p.m * p.k * eb, hipMemcpyDeviceToDevice, b->stream));
+        BHIP_TRY(hipMemcpyAsync(p.Q, keep[j++], (size_t)p.n * p.k * eb, hipMemcpyDeviceToDevice, b->stream));
+        if (b->prec == CUMF_FP8_E4M3) {
+            BHIP_TRY(hipMemcpyAsync(p.ps, keep[j++], 4 * (size_t)p.m, hipMemcpyDeviceToDevice, b->stream));
+            BHIP_TRY(hipMemcpyAsync(p.qs, keep[j++], 4 * (size_t)p.n, hipMemcpyDeviceToDevice, b->stream));
+        }
+    }
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    b->kernel_ms = keep_ms;
+    b->launches = keep_l;
+    b->updates = keep_u;
+    *best = bo;
+    if (best_ms) *best_ms = best_t;
     return CUMF_OK;
 }
 

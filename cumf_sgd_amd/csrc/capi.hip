@@ -611,7 +611,7 @@ int cumf_init_factors(cumf_ctx* c, int32_t k, int32_t precision, uint64_t seed)
     if (rc) return rc;
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "set ratings (m, n) before factors");
     if (k != 32 && k != 64 && k != 128) return fail(CUMF_ERR_ARG, "k=%d unsupported (32, 64 or 128)", k);
-    if (precision < CUMF_FP32 || precision > CUMF_FP8_E4M3) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
+    if (precision < CUMF_FP32 || precision > CUMF_BF16_SR) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
     const size_t eb = (size_t)prec_bytes(precision);
     char** P = (char**)&c->P;
     char** Q = (char**)&c->Q;

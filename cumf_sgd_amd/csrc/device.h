@@ -7,7 +7,8 @@
 
 namespace cumf {
 
-enum Prec : int { kF32 = 0, kF16 = 1, kBF16 = 2, kE4M3 = 3 };
+// kBF16SR: BF16 storage whose update write-back rounds stochastically
+enum Prec : int { kF32 = 0, kF16 = 1, kBF16 = 2, kE4M3 = 3, kBF16SR = 4 };
 
 template <int PREC> struct PrecBytes { static constexpr int value = PREC == kF32 ? 4 : (PREC == kE4M3 ? 1 : 2); };
 
@@ -96,6 +97,15 @@ __device__ inline float bf16_to_f32(uint32_t b) { return __uint_as_float(b << 16
 __device__ inline uint32_t f32_to_bf16(float f)
 {
     return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+// Stochastic rounding with 16 random bits r16 (low half): adding them to the
+// 16 bits that truncation drops carries into the kept part with probability
+// equal to their value / 2^16.  NaN stays a quiet NaN.
+__device__ inline uint32_t f32_to_bf16_sr(float f, uint32_t r16)
+{
+    const uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (x >> 16) | 0x40u;
+    return (x + (r16 & 0xffffu)) >> 16;
 }
 __device__ inline float e4m3_to_f32(uint32_t b)
 {
@@ -246,7 +256,7 @@ __device__ inline float unpack(const Words<NW>& r, int i)
 {
     if constexpr (PREC == kF32) return __uint_as_float(r.w[i]);
     else if constexpr (PREC == kF16) return f16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
-    else if constexpr (PREC == kBF16) return bf16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
+    else if constexpr (PREC == kBF16 || PREC == kBF16SR) return bf16_to_f32((r.w[i >> 1] >> ((i & 1) * 16)) & 0xffffu);
     else {
         switch (i & 3) {
         case 0: return e4m3_byte_to_f32<0>(r.w[i >> 2]);
@@ -274,7 +284,7 @@ template <int PREC, int NW>
 __device__ inline void pack(Words<NW>& r, int i, uint32_t bits)
 {
     if constexpr (PREC == kF32) r.w[i] = bits;
-    else if constexpr (PREC == kF16 || PREC == kBF16) {
+    else if constexpr (PREC == kF16 || PREC == kBF16 || PREC == kBF16SR) {
         if ((i & 1) == 0) r.w[i >> 1] = bits;
         else r.w[i >> 1] |= bits << 16;
     } else {

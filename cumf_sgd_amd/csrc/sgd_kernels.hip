@@ -49,6 +49,7 @@ hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bo
     case kF16: return launch_sgd_prec<kF16>(a, blocks, k, G, coh, D, s);
     case kBF16: return launch_sgd_prec<kBF16>(a, blocks, k, G, coh, D, s);
     case kE4M3: return launch_sgd_prec<kE4M3>(a, blocks, k, G, coh, D, s);
+    case kBF16SR: return launch_sgd_prec<kBF16SR>(a, blocks, k, G, coh, D, s);
     }
     return hipErrorInvalidValue;
 }
@@ -60,6 +61,7 @@ hipError_t launch_sgd_batched(const SgdLaunch& a, int blocks, int prec, int D, h
     case kF16: return launch_batched_for<kF16>(a, blocks, D, s);
     case kBF16: return launch_batched_for<kBF16>(a, blocks, D, s);
     case kE4M3: return launch_batched_for<kE4M3>(a, blocks, D, s);
+    case kBF16SR: return launch_batched_for<kBF16SR>(a, blocks, D, s);
     }
     return hipErrorInvalidValue;
 }
@@ -70,6 +72,7 @@ int sgd_batch_group(int k, int prec)
     case kF32: return batch_group<kF32>(k);
     case kF16: return batch_group<kF16>(k);
     case kBF16: return batch_group<kBF16>(k);
+    case kBF16SR: return batch_group<kBF16SR>(k);
     default: return batch_group<kE4M3>(k);
     }
 }
@@ -147,6 +150,7 @@ __global__ __launch_bounds__(256) void init_factors_kernel(void* F, float* scale
 hipError_t launch_init(void* F, float* scales, int64_t rows, int k, int prec, float s, uint64_t seed,
                        uint32_t problem, uint32_t stream_tag, hipStream_t st)
 {
+    if (prec == kBF16SR) prec = kBF16;   // same storage; rounding differs only in the update
     if (rows <= 0) return hipSuccess;
     int64_t blocks64 = (rows + 3) / 4;
     int blocks = (int)(blocks64 > 65536 ? 65536 : blocks64);
@@ -232,6 +236,7 @@ __global__ __launch_bounds__(256) void export_kernel(const void* F, const float*
 hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k, int prec, float mult,
                          const int32_t* map, float* out, hipStream_t st)
 {
+    if (prec == kBF16SR) prec = kBF16;   // same storage; rounding differs only in the update
     if (rows <= 0) return hipSuccess;
     int64_t total = rows * k;
     int blocks = (int)((total + 255) / 256 > 65536 ? 65536 : (total + 255) / 256);
@@ -364,6 +369,7 @@ hipError_t launch_eval(int criterion, const int32_t* tu, const int32_t* tv, cons
                        const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,
                        int prec, float mult, float b, double* partials, double* out, hipStream_t st)
 {
+    if (prec == kBF16SR) prec = kBF16;   // same storage; rounding differs only in the update
     const int blocks = rmse_blocks(nnz);
     hipError_t e;
     switch (prec) {

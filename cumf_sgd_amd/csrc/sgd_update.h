@@ -77,6 +77,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     constexpr int TB = E * EB;
     constexpr int NW = (TB + 3) / 4;
     constexpr bool FP8 = PREC == kE4M3;
+    constexpr bool SR = FP8 || PREC == kBF16SR;   // stochastic rounding on the write-back
     constexpr int RPL = (kMaxBatch + G - 1) / G;   // staged records per lane
     static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
     static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
@@ -149,7 +150,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         const int b = w % T.update_count;
         const float lr = LR[ep];
         U4 srk = U4{0u, 0u, 0u, 0u};
-        if constexpr (FP8) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
+        if constexpr (SR) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
         stage_window();
         if (w + 1 < nwin) fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
 
@@ -217,6 +218,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         pb = f32_to_bf16(pn);
                         qb = f32_to_bf16(qn);
                     } else {
+                        // element jj (of the row) takes the 16-bit half (jj & 1) of word
+                        // (jj >> 1) & 1 of lowbias32(K ^ (j*0x9E3779B9 + (jj>>2)*0x85EBCA77)),
+                        // words 0-1 for P and 2-3 for Q
                         const int jj = ls * E + e;
                         if ((e & 3) == 0) {
                             const uint32_t t = (uint32_t)j * 0x9E3779B9u + (uint32_t)(jj >> 2) * 0x85EBCA77u;
@@ -225,16 +229,20 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             rbits[2] = lowbias32(srk.z ^ t);
                             rbits[3] = lowbias32(srk.w ^ t);
                         }
-                        // element jj takes the 16-bit half (jj & 1) of word (jj >> 1) & 1
                         const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
-                        const uint32_t rp = (jj & 1) ? (bp & 0xffff0000u) : (bp << 16);
-                        const uint32_t rq = (jj & 1) ? (bq & 0xffff0000u) : (bq << 16);
-                        const uint32_t op = (e & 3) ? po.w[e >> 2] : 0u, oq = (e & 3) ? qo.w[e >> 2] : 0u;
-                        switch (e & 3) {
-                        case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn * qinv, rq, oq); break;
-                        case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn * qinv, rq, oq); break;
-                        case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn * qinv, rq, oq); break;
-                        default: po.w[e >> 2] = e4m3_sr_insert<3>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn * qinv, rq, oq); break;
+                        if constexpr (PREC == kBF16SR) {
+                            pb = f32_to_bf16_sr(pn, (jj & 1) ? (bp >> 16) : bp);
+                            qb = f32_to_bf16_sr(qn, (jj & 1) ? (bq >> 16) : bq);
+                        } else {
+                            const uint32_t rp = (jj & 1) ? (bp & 0xffff0000u) : (bp << 16);
+                            const uint32_t rq = (jj & 1) ? (bq & 0xffff0000u) : (bq << 16);
+                            const uint32_t op = (e & 3) ? po.w[e >> 2] : 0u, oq = (e & 3) ? qo.w[e >> 2] : 0u;
+                            switch (e & 3) {
+                            case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn * qinv, rq, oq); break;
+                            case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn * qinv, rq, oq); break;
+                            case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn * qinv, rq, oq); break;
+                            default: po.w[e >> 2] = e4m3_sr_insert<3>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn * qinv, rq, oq); break;
+                            }
                         }
                     }
                     if constexpr (!FP8) {

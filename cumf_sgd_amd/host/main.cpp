@@ -6,7 +6,7 @@
 //   -p test_file  train_file [model_file]
 // Additions (long options, no reference counterpart):
 //   --seed N        Philox key and srand seed (reference: time(NULL))
-//   --precision P   fp32 | fp16 (default, the reference's) | bf16 | fp8
+//   --precision P   fp32 | fp16 (default, the reference's) | bf16 | bf16sr | fp8
 //   --serial        one worker, deterministic window order
 //   --schedule S    hogwild | grid | stratum
 //   --lanes G       lanes per update (8..64)
@@ -92,12 +92,13 @@ Args parse(int argc, char** argv)
         else if (s == "-p") { need("path after -p"); a.va_path = argv[i]; }
         else if (s == "--seed") { need("a number after --seed"); num("--seed"); a.seed = std::strtoull(argv[i], nullptr, 10); }
         else if (s == "--precision") {
-            need("fp32|fp16|bf16|fp8 after --precision");
+            need("fp32|fp16|bf16|bf16sr|fp8 after --precision");
             std::string v = argv[i];
             if (v == "fp32") a.precision = CUMF_FP32;
             else if (v == "fp16") a.precision = CUMF_FP16;
             else if (v == "bf16") a.precision = CUMF_BF16;
             else if (v == "fp8") a.precision = CUMF_FP8_E4M3;
+            else if (v == "bf16sr") a.precision = CUMF_BF16_SR;
             else throw std::invalid_argument("unknown precision " + v);
         }
         else if (s == "--serial") a.schedule = CUMF_SCHED_SERIAL;

@@ -41,7 +41,8 @@ enum cumf_precision {
     CUMF_FP32 = 0,
     CUMF_FP16 = 1,        /* the reference's native format (singleGPU/sgd.h:124-133) */
     CUMF_BF16 = 2,
-    CUMF_FP8_E4M3 = 3     /* OCP E4M3 + one power-of-two FP32 scale per row */
+    CUMF_FP8_E4M3 = 3,    /* OCP E4M3 + one power-of-two FP32 scale per row */
+    CUMF_BF16_SR = 4      /* BF16 whose update write-back rounds stochastically */
 };
 
 /* Which rating blocks run in one launch. */
@@ -218,10 +219,15 @@ int cumf_batch_add(cumf_batch* batch, const int32_t* u, const int32_t* v, const 
                    const int32_t* tv, const float* tr, int64_t test_nnz, uint64_t seed, int32_t* index);
 /* Epochs [epoch_begin, epoch_end) of every problem.  opts: schedule
  * (CUMF_SCHED_HOGWILD, or CUMF_SCHED_SERIAL = one worker per problem),
- * num_workers = total waves (0 = 16 per CU, shared in proportion to each
+ * num_workers = total waves (0 = 64 per CU, shared in proportion to each
  * problem's bytes of work), alpha, beta, lrate, seed, batch_len,
  * pipeline_depth; k and the lambdas come from each problem. */
 int cumf_batch_train(cumf_batch* batch, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end);
+/* The batch's launch shape: times one epoch for total waves in
+ * {4, 8, 16, 32, 64} per CU and pipeline depth {2, 4, 8}, writes the fastest
+ * into *best (other fields from base) and its epoch time (ms) into *best_ms,
+ * and puts every problem's factors back.  A serial schedule returns base. */
+int cumf_batch_autotune(cumf_batch* batch, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms);
 int cumf_batch_test_rmse(cumf_batch* batch, int32_t index, float mult, float b, double* rmse);
 int cumf_batch_test_eval(cumf_batch* batch, int32_t index, int32_t criterion, float mult, float b, double* value);
 int cumf_batch_get_factors(cumf_batch* batch, int32_t index, int32_t which, void* data, float* scales);

@@ -17,8 +17,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 MF_PREDICT = os.path.join(REF_DIR, "mf-predict")
 MF_TRAIN = os.path.join(REF_DIR, "mf-train")
 
-F32, F16, BF16, E4M3 = 0, 1, 2, 3
-STORAGE = {F32: np.float32, F16: np.uint16, BF16: np.uint16, E4M3: np.uint8}
+F32, F16, BF16, E4M3, BF16_SR = 0, 1, 2, 3, 4
+STORAGE = {F32: np.float32, F16: np.uint16, BF16: np.uint16, E4M3: np.uint8, BF16_SR: np.uint16}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
@@ -43,6 +43,7 @@ _sig = {
     "orc_e4m3_to_f32": ([ctypes.c_uint8], F),
     "orc_f32_to_e4m3_rne": ([F], ctypes.c_uint8),
     "orc_f32_to_e4m3_sr": ([F, U32], ctypes.c_uint8),
+    "orc_f32_to_bf16_sr": ([F, U32], ctypes.c_uint16),
     "orc_pow2_scale": ([F, F], F),
     "orc_init_factors": ([P, P, I64, ctypes.c_int, ctypes.c_int, U64, U32, ctypes.c_int], None),
     "orc_fp8_rescale": ([P, P, I64, ctypes.c_int], None),
@@ -99,6 +100,7 @@ f32_to_bf16 = _lib.orc_f32_to_bf16
 e4m3_to_f32 = _lib.orc_e4m3_to_f32
 f32_to_e4m3_rne = _lib.orc_f32_to_e4m3_rne
 f32_to_e4m3_sr = _lib.orc_f32_to_e4m3_sr
+f32_to_bf16_sr = _lib.orc_f32_to_bf16_sr
 pow2_scale = _lib.orc_pow2_scale
 
 

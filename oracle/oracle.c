@@ -145,6 +145,19 @@ EXPORT uint16_t orc_f32_to_bf16(float f)
     return (uint16_t)(x >> 16);
 }
 
+/* BF16 with stochastic rounding by 16 random bits r16: adding r16 to the 16
+ * bits that truncation drops carries into the kept part with probability
+ * equal to their value / 2^16, so |f| rounds up to the next BF16 with that
+ * probability (a carry out of the largest finite value gives inf).  NaN stays
+ * a quiet NaN. */
+EXPORT uint16_t orc_f32_to_bf16_sr(float f, uint32_t r16)
+{
+    uint32_t x = fbits(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u)
+        return (uint16_t)((x >> 16) | 0x40u);
+    return (uint16_t)((x + (r16 & 0xffffu)) >> 16);
+}
+
 EXPORT float orc_e4m3_to_f32(uint8_t b)
 {
     uint32_t sign = b >> 7;
@@ -217,9 +230,10 @@ EXPORT uint8_t orc_f32_to_e4m3_sr(float f, uint32_t r16)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Factor storage.  prec: 0 FP32, 1 FP16, 2 BF16, 3 FP8-E4M3 + per-row scale. */
+/* Factor storage.  prec: 0 FP32, 1 FP16, 2 BF16, 3 FP8-E4M3 + per-row scale, */
+/* 4 BF16 with stochastic rounding on the update's write-back.              */
 /* ------------------------------------------------------------------------ */
-enum { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2, PREC_E4M3 = 3 };
+enum { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2, PREC_E4M3 = 3, PREC_BF16_SR = 4 };
 
 /* FP8 row scales are powers of two.  A row is (re)scaled so that its largest
  * magnitude maps to at most FP8_INIT_TARGET (at init) or FP8_EPOCH_TARGET
@@ -234,7 +248,8 @@ static inline float load_elem(const void* base, int prec, int64_t i)
     switch (prec) {
     case PREC_F32:  return ((const float*)base)[i];
     case PREC_F16:  return orc_f16_to_f32(((const uint16_t*)base)[i]);
-    case PREC_BF16: return orc_bf16_to_f32(((const uint16_t*)base)[i]);
+    case PREC_BF16:
+    case PREC_BF16_SR: return orc_bf16_to_f32(((const uint16_t*)base)[i]);
     default:        return orc_e4m3_to_f32(((const uint8_t*)base)[i]);
     }
 }
@@ -255,6 +270,11 @@ static inline float store_elem(void* base, int prec, int64_t i, float x, float i
     }
     case PREC_BF16: {
         uint16_t h = orc_f32_to_bf16(x);
+        ((uint16_t*)base)[i] = h;
+        return orc_bf16_to_f32(h);
+    }
+    case PREC_BF16_SR: {
+        uint16_t h = orc_f32_to_bf16_sr(x, rnd);
         ((uint16_t*)base)[i] = h;
         return orc_bf16_to_f32(h);
     }
@@ -315,7 +335,8 @@ EXPORT void orc_init_factors(void* out, float* row_scale, int64_t rows, int k, i
             switch (prec) {
             case PREC_F32:  ((float*)out)[i] = tmp[e]; break;
             case PREC_F16:  ((uint16_t*)out)[i] = orc_f32_to_f16(tmp[e]); break;
-            case PREC_BF16: ((uint16_t*)out)[i] = orc_f32_to_bf16(tmp[e]); break;
+            case PREC_BF16:
+            case PREC_BF16_SR: ((uint16_t*)out)[i] = orc_f32_to_bf16(tmp[e]); break;
             default:        ((uint8_t*)out)[i] = orc_f32_to_e4m3_rne(tmp[e] * inv); break;
             }
         }
@@ -502,7 +523,7 @@ typedef struct {
     float lambda_p, lambda_q;
 } orc_sgd_cfg;
 
-/* Random bits for stochastic rounding of the FP8 write-back of sample j
+/* Random bits for stochastic rounding of the FP8 (and BF16-SR) write-back of sample j
  * (0..L-1) of window `batch`: one Philox block per window,
  * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}), then for element
  * group c = e >> 2 the words W_i = lowbias32(K_i ^ (j*0x9E3779B9 +
@@ -545,7 +566,7 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
         float pn = fmaf(lr, fmaf(e, q[j], -(c->lambda_p * p[j])), p[j]);
         float qn = fmaf(lr, fmaf(e, p[j], -(c->lambda_q * q[j])), q[j]);
         uint32_t rp = 0, rq = 0;
-        if (c->prec == PREC_E4M3) {
+        if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
             rp = sr_bits(K, sj, 0, j);
             rq = sr_bits(K, sj, 1, j);
         }
@@ -574,7 +595,7 @@ EXPORT int orc_sgd_serial(const orc_sgd_cfg* c, const int32_t* ru, const int32_t
         for (int64_t b = 0; b < nb; ++b) {
             int64_t start = orc_window_start(c->seed, c->problem, 0, 0, (uint32_t)ep, (uint32_t)b, nnz);
             uint32_t K[4] = {0, 0, 0, 0};
-            if (c->prec == PREC_E4M3) {
+            if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
                 uint32_t ctr[4] = {(uint32_t)b, (uint32_t)ep, 0u, STREAM_ROUND};
                 uint32_t key[2];
                 key_of(c->seed, c->problem, key);

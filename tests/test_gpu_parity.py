@@ -29,7 +29,7 @@ def _ctx(u, v, r, m, n):
 
 
 @pytest.mark.parametrize("k", [32, 64, 128])
-@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3, 4])
 def test_init_bit_exact(gpu, k, prec):
     import cumf_sgd_amd as cs
     import oracle
@@ -52,6 +52,8 @@ SERIAL_CASES = [
     (128, 64, 0, 8), (128, 64, 1, 8), (128, 64, 2, 8), (128, 64, 1, 2), (128, 64, 0, 4), (128, 32, 0, 8),
     (128, 16, 1, 8), (128, 32, 3, 8), (128, 16, 3, 4), (64, 32, 0, 8), (64, 16, 1, 2), (64, 16, 3, 8),
     (32, 16, 0, 8), (32, 8, 2, 4), (32, 8, 3, 8),
+    # BF16 with stochastic rounding, including two elements per lane (G=64)
+    (128, 16, 4, 4), (128, 64, 4, 4), (64, 8, 4, 2), (32, 8, 4, 8),
 ]
 
 

@@ -105,6 +105,32 @@ def test_e4m3_stochastic_rounding_brackets_and_is_unbiased():
         assert all(oracle.f32_to_e4m3_sr(x, r) == b for r in (0, 1, 0x7FFF, 0xFFFF))
 
 
+def test_bf16_stochastic_rounding_counts_exactly():
+    """BF16-SR over all 65536 random words: the result is the truncation or
+    the next BF16 away from zero, and it rounds up for exactly (low 16 bits
+    of x) words, i.e. with probability equal to the dropped fraction.
+    Truncation itself is torch's bfloat16 of the value with its low bits
+    cleared; representable values are fixed points; NaN stays NaN."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([rng.normal(0, 3, 6).astype(np.float32), np.float32([1e-30, -2.5e-3, 3.0e38])])
+    for x in xs:
+        bits = int(np.float32(x).view(np.uint32))
+        trunc = bits >> 16
+        want_trunc = int(torch.tensor(np.uint32(bits & 0xFFFF0000).view(np.float32)).to(torch.bfloat16)
+                         .view(torch.int16).item()) & 0xFFFF
+        assert want_trunc == trunc
+        outs = np.array([oracle.f32_to_bf16_sr(float(x), r) for r in range(65536)])
+        assert set(np.unique(outs)) <= {trunc, trunc + 1}
+        assert int((outs == trunc + 1).sum()) == bits & 0xFFFF
+    for b in (0x3F80, 0x0001, 0xC2F7, 0x7F7F):
+        x = oracle.bf16_to_f32(b)
+        assert all(oracle.f32_to_bf16_sr(x, r) == b for r in (0, 1, 0x8000, 0xFFFF))
+    assert oracle.f32_to_bf16_sr(float("nan"), 0xFFFF) & 0x7FC0 == 0x7FC0
+    assert oracle.f32_to_bf16_sr(float("inf"), 0xFFFF) == 0x7F80
+
+
 def test_e4m3_stochastic_rounding_matches_gfx950_instruction_vectors():
     """The oracle's SR rule against 21,302 outputs of v_cvt_sr_fp8_f32 (the
     conversion the update kernel stores FP8 with), written on a gfx950 GPU by
