@@ -109,7 +109,7 @@ def measured_traffic(prec: int, info, schedule: str):
     """HBM bytes per launch of the update kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload and schedule (profiles/),
     when it was measured on the same kernel shape; None otherwise."""
-    name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8"}[prec]
+    name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8", 4: "bf16sr"}[prec]
     path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
         return None

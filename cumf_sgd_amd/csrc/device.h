@@ -107,6 +107,16 @@ __device__ inline uint32_t f32_to_bf16_sr(float f, uint32_t r16)
     if ((x & 0x7fffffffu) > 0x7f800000u) return (x >> 16) | 0x40u;
     return (x + (r16 & 0xffffu)) >> 16;
 }
+// The same rounding by v_cvt_sr_bf16_f32, which adds the top half of its
+// random word (rword = r16 << 16) and quiets NaN the same way, written into
+// half HI of the packed pair `old`.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+template <bool HI>
+__device__ inline uint32_t bf16_sr_insert(float f, uint32_t rword, uint32_t old)
+{
+    const bf16x2_t o = __builtin_bit_cast(bf16x2_t, old);
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(o, f, rword, HI));
+}
 __device__ inline float e4m3_to_f32(uint32_t b)
 {
     uint32_t sign = (b >> 7) & 1u, exp = (b >> 3) & 0xfu, man = b & 7u;

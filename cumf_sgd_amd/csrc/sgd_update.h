@@ -230,7 +230,19 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             rbits[3] = lowbias32(srk.w ^ t);
                         }
                         const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
-                        if constexpr (PREC == kBF16SR) {
+                        if constexpr (PREC == kBF16SR && E % 2 == 0) {
+                            // (jj & 1) == (e & 1): the instruction writes that half directly
+                            const uint32_t rp = (e & 1) ? (bp & 0xffff0000u) : (bp << 16);
+                            const uint32_t rq = (e & 1) ? (bq & 0xffff0000u) : (bq << 16);
+                            const uint32_t op = (e & 1) ? po.w[e >> 1] : 0u, oq = (e & 1) ? qo.w[e >> 1] : 0u;
+                            if (e & 1) {
+                                po.w[e >> 1] = bf16_sr_insert<true>(pn, rp, op);
+                                qo.w[e >> 1] = bf16_sr_insert<true>(qn, rq, oq);
+                            } else {
+                                po.w[e >> 1] = bf16_sr_insert<false>(pn, rp, op);
+                                qo.w[e >> 1] = bf16_sr_insert<false>(qn, rq, oq);
+                            }
+                        } else if constexpr (PREC == kBF16SR) {
                             pb = f32_to_bf16_sr(pn, (jj & 1) ? (bp >> 16) : bp);
                             qb = f32_to_bf16_sr(qn, (jj & 1) ? (bq >> 16) : bq);
                         } else {
@@ -245,7 +257,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             }
                         }
                     }
-                    if constexpr (!FP8) {
+                    if constexpr (!FP8 && !(PREC == kBF16SR && E % 2 == 0)) {
                         pack<PREC>(po, e, pb);
                         pack<PREC>(qo, e, qb);
                     }
