@@ -124,6 +124,7 @@ def _load():
         "cumf_read_ratings": ([P, ctypes.c_char_p, ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32)],
                               ctypes.c_int),
         "cumf_read_test": ([P, ctypes.c_char_p, ctypes.POINTER(I64)], ctypes.c_int),
+        "cumf_stream_ratings": ([P, P, I64, I32, I32, I32, P], ctypes.c_int),
         "cumf_set_model": ([P, I32, I32, I32, P, P], ctypes.c_int),
         "cumf_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_reset_kernel_time": ([P], ctypes.c_int),
@@ -328,6 +329,17 @@ class Context:
         out = ctypes.c_double()
         _check(lib.cumf_test_rmse(self._h, float(mult), float(b), ctypes.byref(out)), "cumf_test_rmse")
         return out.value
+
+    def stream_ratings(self, nodes: np.ndarray, m: int, n: int, s: int) -> np.ndarray:
+        """Out-of-core ratings for the stratum schedule: the 12-byte records
+        (ids permuted, values scaled) stay in pinned host memory in s x s
+        blocks and stream to the device one diagonal at a time during
+        training.  Returns the block bounds."""
+        nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        off = np.zeros(s * s + 1, np.int64)
+        _check(lib.cumf_stream_ratings(self._h, _ptr(nodes), len(nodes), m, n, s, _ptr(off)), "cumf_stream_ratings")
+        self.m, self.n, self.nnz = m, n, len(nodes)
+        return off
 
     def read_ratings(self, path: str):
         """Stream the training file (or shards path0..path79) into the device

@@ -115,6 +115,19 @@ struct cumf_ctx {
     uint64_t graph_key = 0;
     double* partials = nullptr; size_t partials_cap = 0;
 
+    // streamed ratings (cumf_stream_ratings): block-ordered SoA in pinned host
+    // memory, two device staging buffers of one stratum diagonal each, filled
+    // on a copy stream while the other diagonal trains
+    bool streamed = false;
+    int32_t* hu = nullptr;
+    int32_t* hv = nullptr;
+    float* hr = nullptr;
+    int32_t* su[2] = {nullptr, nullptr};
+    int32_t* sv[2] = {nullptr, nullptr};
+    float* sr[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_done[2] = {nullptr, nullptr}, kern_done[2] = {nullptr, nullptr};
+
     std::vector<hipEvent_t> ev;   // start/stop pairs of training launches
     size_t ev_used = 0;
     double kernel_ms = 0.0;
@@ -190,8 +203,31 @@ int validate_ids(cumf_ctx* c)
     return CUMF_OK;
 }
 
+void drop_streamed(cumf_ctx* c)
+{
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : {(void*)c->hu, (void*)c->hv, (void*)c->hr})
+        if (p) (void)hipHostFree(p);
+    c->hu = c->hv = nullptr;
+    c->hr = nullptr;
+    for (int b = 0; b < 2; ++b) {
+        for (void* p : {(void*)c->su[b], (void*)c->sv[b], (void*)c->sr[b]})
+            if (p) (void)hipFree(p);
+        c->su[b] = c->sv[b] = nullptr;
+        c->sr[b] = nullptr;
+        if (c->copy_done[b]) (void)hipEventDestroy(c->copy_done[b]);
+        if (c->kern_done[b]) (void)hipEventDestroy(c->kern_done[b]);
+        c->copy_done[b] = c->kern_done[b] = nullptr;
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    c->copy_stream = nullptr;
+    c->streamed = false;
+}
+
 int after_ratings(cumf_ctx* c, int64_t nnz, int32_t m, int32_t n)
 {
+    drop_streamed(c);
     c->nnz = nnz;
     c->m = m;
     c->n = n;
@@ -269,6 +305,7 @@ int cumf_destroy(cumf_ctx* c)
     release(&c->P, &c->P_cap); release(&c->Q, &c->Q_cap);
     release(&c->ps, &c->ps_cap); release(&c->qs, &c->qs_cap);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    drop_streamed(c);
     release(&c->lr, &c->lr_cap); release(&c->tasks, &c->tasks_cap); release(&c->partials, &c->partials_cap);
     release(&c->epoch_dev, &c->epoch_dev_cap);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -313,6 +350,7 @@ int cumf_alloc_ratings(cumf_ctx* c, int64_t nnz, int32_t m, int32_t n, int32_t**
     int rc = check_ctx(c);
     if (rc) return rc;
     if (nnz < 0 || m <= 0 || n <= 0 || !u || !v || !r) return fail(CUMF_ERR_ARG, "bad arguments");
+    drop_streamed(c);
     c->have_ratings = false;
     if ((rc = ensure(&c->u, &c->u_cap, (size_t)nnz)) || (rc = ensure(&c->v, &c->v_cap, (size_t)nnz)) ||
         (rc = ensure(&c->r, &c->r_cap, (size_t)nnz)))
@@ -412,6 +450,12 @@ int cumf_get_ratings(cumf_ctx* c, int32_t* u, int32_t* v, float* r)
     if (rc) return rc;
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (c->nnz == 0) return CUMF_OK;
+    if (c->streamed) {
+        if (u) HIP_TRY(hipMemcpy(u, c->hu, sizeof(int32_t) * c->nnz, hipMemcpyDefault));
+        if (v) HIP_TRY(hipMemcpy(v, c->hv, sizeof(int32_t) * c->nnz, hipMemcpyDefault));
+        if (r) HIP_TRY(hipMemcpy(r, c->hr, sizeof(float) * c->nnz, hipMemcpyDefault));
+        return CUMF_OK;
+    }
     if (u) HIP_TRY(hipMemcpyAsync(u, c->u, sizeof(int32_t) * c->nnz, hipMemcpyDefault, c->stream));
     if (v) HIP_TRY(hipMemcpyAsync(v, c->v, sizeof(int32_t) * c->nnz, hipMemcpyDefault, c->stream));
     if (r) HIP_TRY(hipMemcpyAsync(r, c->r, sizeof(float) * c->nnz, hipMemcpyDefault, c->stream));
@@ -423,6 +467,7 @@ int cumf_rating_moments(cumf_ctx* c, double* sum, double* sum_sq)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     const size_t need = 2 * (size_t)moments_blocks(c->nnz) + 2;
     if ((rc = ensure(&c->partials, &c->partials_cap, need))) return rc;
@@ -440,6 +485,7 @@ int cumf_remap(cumf_ctx* c, const int32_t* p_map, const int32_t* q_map)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (!p_map || !q_map) return fail(CUMF_ERR_ARG, "null map");
     DevTmp dp, dq;
@@ -477,6 +523,7 @@ int cumf_scale_ratings(cumf_ctx* c, float scale)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     HIP_TRY(launch_scale(c->r, c->nnz, scale, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -556,6 +603,7 @@ int cumf_partition(cumf_ctx* c, int32_t ux, int32_t vy, int64_t* offsets)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (ux < 1 || vy < 1) return fail(CUMF_ERR_ARG, "bad grid %d x %d", ux, vy);
     // segment lengths of grid_problem (singleGPU/sgd.cu:419-423)
@@ -573,6 +621,7 @@ int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (s < 1 || s > 64) return fail(CUMF_ERR_ARG, "strata %d out of range 1..64", s);
     DevTmp du, dv;
@@ -589,10 +638,97 @@ int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
     return partition_with(c, balanced_bounds(hu, s, c->nnz), balanced_bounds(hv, s, c->nnz), offsets);
 }
 
+int cumf_stream_ratings(cumf_ctx* c, const cumf_node* R, int64_t nnz, int32_t m, int32_t n, int32_t s,
+                        int64_t* offsets)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nnz < 1 || m <= 0 || n <= 0 || !R) return fail(CUMF_ERR_ARG, "bad ratings (nnz=%lld m=%d n=%d)", (long long)nnz, m, n);
+    if (s < 2 || s > 64 || s % 2) return fail(CUMF_ERR_ARG, "streamed strata need an even s in 2..64 (got %d)", s);
+    drop_streamed(c);
+    c->have_ratings = false;
+    // the same blocks as cumf_partition_strata: nnz-balanced id ranges from
+    // the degree histograms, then a stable counting sort by block
+    std::vector<int32_t> du((size_t)m, 0), dv((size_t)n, 0);
+    for (int64_t i = 0; i < nnz; ++i) {
+        const cumf_node& x = R[i];
+        if (x.u < 0 || x.u >= m || x.v < 0 || x.v >= n)
+            return fail(CUMF_ERR_ARG, "rating %lld has ids (%d, %d) outside [0,%d) x [0,%d)", (long long)i, x.u, x.v, m, n);
+        ++du[(size_t)x.u];
+        ++dv[(size_t)x.v];
+    }
+    const std::vector<int64_t> bu = balanced_bounds(du, s, nnz), bv = balanced_bounds(dv, s, nnz);
+    std::vector<uint8_t> useg((size_t)m), vseg((size_t)n);
+    for (int i = 0; i < s; ++i) {
+        for (int64_t x = bu[i]; x < bu[i + 1]; ++x) useg[(size_t)x] = (uint8_t)i;
+        for (int64_t x = bv[i]; x < bv[i + 1]; ++x) vseg[(size_t)x] = (uint8_t)i;
+    }
+    const int nb = s * s;
+    std::vector<int64_t> off((size_t)nb + 1, 0);
+    for (int64_t i = 0; i < nnz; ++i) ++off[(size_t)(useg[(size_t)R[i].u] * s + vseg[(size_t)R[i].v]) + 1];
+    for (int b = 0; b < nb; ++b) off[b + 1] += off[b];
+    struct Pinned {
+        void* p = nullptr;
+        ~Pinned() { if (p) (void)hipHostFree(p); }
+    } pu, pv, pr;
+    HIP_TRY(hipHostMalloc(&pu.p, sizeof(int32_t) * (size_t)nnz, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&pv.p, sizeof(int32_t) * (size_t)nnz, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&pr.p, sizeof(float) * (size_t)nnz, hipHostMallocDefault));
+    {
+        std::vector<int64_t> pos(off.begin(), off.end() - 1);
+        int32_t* hu = (int32_t*)pu.p;
+        int32_t* hv = (int32_t*)pv.p;
+        float* hr = (float*)pr.p;
+        for (int64_t i = 0; i < nnz; ++i) {
+            const cumf_node& x = R[i];
+            const int64_t d = pos[(size_t)(useg[(size_t)x.u] * s + vseg[(size_t)x.v])]++;
+            hu[d] = x.u;
+            hv[d] = x.v;
+            hr[d] = x.rate;
+        }
+    }
+    // staging: the largest diagonal
+    int64_t cap = 1;
+    for (int d = 0; d < s; ++d) {
+        int64_t t = 0;
+        for (int i = 0; i < s; ++i) {
+            const int blk = i * s + (i + d) % s;
+            t += off[(size_t)blk + 1] - off[(size_t)blk];
+        }
+        cap = std::max(cap, t);
+    }
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMalloc(&c->su[b], sizeof(int32_t) * (size_t)cap));
+        HIP_TRY(hipMalloc(&c->sv[b], sizeof(int32_t) * (size_t)cap));
+        HIP_TRY(hipMalloc(&c->sr[b], sizeof(float) * (size_t)cap));
+        HIP_TRY(hipEventCreateWithFlags(&c->copy_done[b], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->kern_done[b], hipEventDisableTiming));
+    }
+    HIP_TRY(hipStreamCreate(&c->copy_stream));
+    c->hu = (int32_t*)pu.p;
+    c->hv = (int32_t*)pv.p;
+    c->hr = (float*)pr.p;
+    pu.p = pv.p = pr.p = nullptr;
+    // the device copies of earlier ratings are not needed any more
+    release(&c->u, &c->u_cap);
+    release(&c->v, &c->v_cap);
+    release(&c->r, &c->r_cap);
+    c->nnz = nnz;
+    c->m = m;
+    c->n = n;
+    c->ux = c->vy = s;
+    c->offsets = off;
+    c->streamed = true;
+    c->have_ratings = true;
+    if (offsets) std::copy(off.begin(), off.end(), offsets);
+    return CUMF_OK;
+}
+
 int cumf_set_blocks(cumf_ctx* c, int32_t ux, int32_t vy, const int64_t* offsets)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (ux < 1 || vy < 1 || (int64_t)ux * vy > 4096 || !offsets) return fail(CUMF_ERR_ARG, "bad block grid");
     const int nb = ux * vy;
@@ -947,10 +1083,10 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
 // Queue one epoch's launches (and the FP8 rescale) on the context stream.
 // epoch_dev non-null: the kernels read the epoch from device memory and the
 // counter is advanced at the end (the form captured into a graph).
-int queue_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const std::vector<LaunchRec>& ls, int ep,
-                int32_t* epoch_dev, bool timed)
+int queue_one(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const LaunchRec& l, int ep, int32_t* epoch_dev,
+              bool timed)
 {
-    for (const LaunchRec& l : ls) {
+    {
         SgdLaunch a{};
         a.tasks = c->tasks + l.task_off;
         a.num_tasks = l.task_cnt;
@@ -970,11 +1106,85 @@ int queue_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const st
         HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D, c->stream));
         if (timed) HIP_TRY(hipEventRecord(ev1, c->stream));
     }
+    return CUMF_OK;
+}
+
+// After an epoch's launches: the FP8 rescale and, for graph replays, the
+// device epoch counter.
+int queue_epoch_tail(cumf_ctx* c, int32_t* epoch_dev)
+{
     if (c->prec == CUMF_FP8_E4M3) {
         HIP_TRY(launch_fp8_rescale((uint8_t*)c->P, c->ps, c->m, c->k, c->stream));
         HIP_TRY(launch_fp8_rescale((uint8_t*)c->Q, c->qs, c->n, c->k, c->stream));
     }
     if (epoch_dev) HIP_TRY(launch_advance_int(epoch_dev, c->stream));
+    return CUMF_OK;
+}
+
+int queue_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const std::vector<LaunchRec>& ls, int ep,
+                int32_t* epoch_dev, bool timed)
+{
+    int rc;
+    for (const LaunchRec& l : ls)
+        if ((rc = queue_one(c, o, sh, l, ep, epoch_dev, timed))) return rc;
+    return queue_epoch_tail(c, epoch_dev);
+}
+
+// Streamed ratings: each stratum diagonal is copied from pinned host memory
+// into staging buffer (d & 1) on the copy stream while the previous diagonal
+// trains; a diagonal's launch waits for its copy, a copy waits for the
+// launch that last read its buffer.  soff[blk] is the block's place in its
+// diagonal's staging buffer; the tasks already point there.
+int train_streamed(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const std::vector<LaunchRec>& ls,
+                   const std::vector<SgdTask>& tasks, const std::vector<int64_t>& soff, int e0, int e1)
+{
+    const int s = c->ux;
+    auto diag_of = [&](const LaunchRec& l) {
+        const int blk = (int)tasks[(size_t)l.task_off].tag;
+        return (blk % s - blk / s + s) % s;
+    };
+    auto copy_diag = [&](int d) -> int {
+        const int b = d & 1;
+        HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->kern_done[b], 0));
+        for (int i = 0; i < s; ++i) {
+            const int blk = i * s + (i + d) % s;
+            const int64_t beg = c->offsets[(size_t)blk], sz = c->offsets[(size_t)blk + 1] - beg;
+            if (sz == 0) continue;
+            const int64_t at = soff[(size_t)blk];
+            HIP_TRY(hipMemcpyAsync(c->su[b] + at, c->hu + beg, sizeof(int32_t) * sz, hipMemcpyHostToDevice, c->copy_stream));
+            HIP_TRY(hipMemcpyAsync(c->sv[b] + at, c->hv + beg, sizeof(int32_t) * sz, hipMemcpyHostToDevice, c->copy_stream));
+            HIP_TRY(hipMemcpyAsync(c->sr[b] + at, c->hr + beg, sizeof(float) * sz, hipMemcpyHostToDevice, c->copy_stream));
+        }
+        HIP_TRY(hipEventRecord(c->copy_done[b], c->copy_stream));
+        return CUMF_OK;
+    };
+    int rc;
+    // both buffers are free at the start
+    HIP_TRY(hipEventRecord(c->kern_done[0], c->stream));
+    HIP_TRY(hipEventRecord(c->kern_done[1], c->stream));
+    if (ls.empty()) return CUMF_OK;
+    if ((rc = copy_diag(diag_of(ls[0])))) return rc;
+    for (int ep = e0; ep < e1; ++ep) {
+        for (size_t li = 0; li < ls.size(); ++li) {
+            const int b = diag_of(ls[li]) & 1;
+            HIP_TRY(hipStreamWaitEvent(c->stream, c->copy_done[b], 0));
+            if ((rc = queue_one(c, o, sh, ls[li], ep, nullptr, true))) return rc;
+            HIP_TRY(hipEventRecord(c->kern_done[b], c->stream));
+            if (li + 1 < ls.size()) {
+                if ((rc = copy_diag(diag_of(ls[li + 1])))) return rc;
+            } else if (ep + 1 < e1) {
+                if ((rc = copy_diag(diag_of(ls[0])))) return rc;
+            }
+            const LaunchRec& l = ls[li];
+            for (int32_t t = 0; t < l.task_cnt; ++t) {
+                const SgdTask& x = tasks[(size_t)(l.task_off + t)];
+                c->updates += (int64_t)x.worker_count * x.update_count * sh.L;
+            }
+        }
+        if ((rc = queue_epoch_tail(c, nullptr))) return rc;
+        c->launches += (int64_t)ls.size();
+    }
+    HIP_TRY(hipStreamSynchronize(c->copy_stream));
     return CUMF_OK;
 }
 
@@ -1039,12 +1249,36 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     }
     c->last.updates_per_epoch = upd;
     if (tasks.empty()) return CUMF_OK;
+    std::vector<int64_t> soff;
+    if (c->streamed) {
+        if (!stratum) return fail(CUMF_ERR_STATE, "streamed ratings train with the stratum schedule");
+        // point every task at its block's place in its diagonal's staging buffer
+        const int s = c->ux;
+        soff.assign((size_t)s * s, 0);
+        for (int d = 0; d < s; ++d) {
+            int64_t at = 0;
+            for (int i = 0; i < s; ++i) {
+                const int blk = i * s + (i + d) % s;
+                soff[(size_t)blk] = at;
+                at += c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk];
+            }
+        }
+        for (SgdTask& t : tasks) {
+            const int blk = (int)t.tag;
+            const int b = ((blk % s - blk / s + s) % s) & 1;
+            t.u = c->su[b] + soff[(size_t)blk];
+            t.v = c->sv[b] + soff[(size_t)blk];
+            t.r = c->sr[b] + soff[(size_t)blk];
+        }
+    }
     if ((rc = ensure(&c->tasks, &c->tasks_cap, tasks.size()))) return rc;
     HIP_TRY(hipMemcpyAsync(c->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
     int64_t launches_per_epoch = 0;
     for (const LaunchRec& l : plan[0]) (void)l, ++launches_per_epoch;
 
-    if (o->use_graph && !per_epoch_order) {
+    if (c->streamed) {
+        if ((rc = train_streamed(c, o, sh, plan[0], tasks, soff, e0, e1))) return rc;
+    } else if (o->use_graph && !per_epoch_order) {
         // the epoch loop as a graph: captured once per plan, replayed per epoch
         uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
         const int32_t shape[8] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec};

@@ -104,6 +104,20 @@ int cumf_read_ratings(cumf_ctx* ctx, const char* path, int64_t* nnz, int32_t* m,
 /* The same for the test ratings (any ids; those outside the model predict b). */
 int cumf_read_test(cumf_ctx* ctx, const char* path, int64_t* nnz);
 
+/* Out-of-core ratings for the stratum schedule (the reference's streamed
+ * u x v x x x y grid, singleGPU/sgd_kernel.cu:373-690, rebuilt for HBM-sized
+ * blocks): R (host, ids already permuted and values scaled) is split into
+ * s x s blocks exactly as cumf_partition_strata splits device ratings (even
+ * s, 2..64) and kept in pinned host memory; cumf_train with the stratum
+ * schedule then copies each diagonal into one of two device staging buffers
+ * on a second stream while the previous diagonal trains.  Device memory
+ * holds two diagonals instead of all ratings.  offsets (s*s + 1 entries, may
+ * be NULL) receives the block bounds.  Moments, remap, scaling and the other
+ * partitions are not available on streamed ratings; any later set/read/alloc
+ * of ratings replaces them. */
+int cumf_stream_ratings(cumf_ctx* ctx, const cumf_node* R, int64_t nnz, int32_t m, int32_t n, int32_t s,
+                        int64_t* offsets);
+
 /* Test ratings for cumf_test_rmse (ids outside [0,m) x [0,n) predict b,
  * test/mf.cpp:3796-3797). */
 int cumf_set_test(cumf_ctx* ctx, const cumf_node* R, int64_t nnz);
