@@ -32,6 +32,7 @@ PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3, "bf16sr
 HOGWILD, SERIAL, GRID, STRATUM = 0, 1, 2, 3
 # mf-predict -e criteria (test/mf.h:30-40)
 EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
+EVAL_ROW_MPR, EVAL_COL_MPR, EVAL_ROW_AUC, EVAL_COL_AUC = 10, 11, 12, 13
 SCHEDULES = {"hogwild": HOGWILD, "serial": SERIAL, "grid": GRID, "stratum": STRATUM}
 ELEM_BYTES = {FP32: 4, FP16: 2, BF16: 2, FP8_E4M3: 1}
 STORAGE_DTYPE = {FP32: np.float32, FP16: np.uint16, BF16: np.uint16, FP8_E4M3: np.uint8, BF16_SR: np.uint16}

@@ -449,8 +449,16 @@ int cumf_batch_test_eval(cumf_batch* b, int32_t index, int32_t criterion, float 
     Problem* p;
     if ((rc = problem_of(b, index, &p))) return rc;
     if (!eval_supported(criterion))
-        return fail(CUMF_ERR_ARG, "criterion %d not supported (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)", criterion);
+        return fail(CUMF_ERR_ARG, "criterion %d not supported (0, 1, 2, 5, 6, 10-13)", criterion);
     if (!value) return fail(CUMF_ERR_ARG, "null output");
+    if (criterion >= 10) {   // row/column MPR (10, 11) and AUC (12, 13)
+        double mpr = 0, auc = 0;
+        if ((rc = rank_eval(b->stream, p->tu, p->tv, p->tr, p->tnnz, p->P, p->ps, p->Q, p->qs, p->m, p->n, p->k,
+                            b->prec, mult, bias, criterion == 11 || criterion == 13, &mpr, &auc)))
+            return rc;
+        *value = criterion <= 11 ? mpr : auc;
+        return CUMF_OK;
+    }
     if (p->tnnz == 0) {
         *value = 0.0;
         return CUMF_OK;

@@ -857,10 +857,18 @@ int cumf_test_eval(cumf_ctx* c, int32_t criterion, float mult, float b, double* 
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!eval_supported(criterion))
-        return fail(CUMF_ERR_ARG, "criterion %d not supported (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)", criterion);
+        return fail(CUMF_ERR_ARG, "criterion %d not supported (0, 1, 2, 5, 6, 10-13)", criterion);
     if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
     if (!c->have_test) return fail(CUMF_ERR_STATE, "no test ratings");
     if (!value) return fail(CUMF_ERR_ARG, "null output");
+    if (criterion >= 10) {   // row/column MPR (10, 11) and AUC (12, 13)
+        double mpr = 0, auc = 0;
+        if ((rc = rank_eval(c->stream, c->tu, c->tv, c->tr, c->tnnz, c->P, c->ps, c->Q, c->qs, c->m, c->n, c->k,
+                            c->prec, mult, b, criterion == 11 || criterion == 13, &mpr, &auc)))
+            return rc;
+        *value = criterion <= 11 ? mpr : auc;
+        return CUMF_OK;
+    }
     if (c->tnnz == 0) {
         *value = 0.0;   // test/mf.cpp:3818-3819
         return CUMF_OK;

@@ -63,6 +63,16 @@ hipError_t launch_export(const void* F, const float* scales, int64_t rows, int k
                          const int32_t* map, float* out, hipStream_t st);
 int rmse_blocks(int64_t nnz);
 bool eval_supported(int criterion);
+// The ranking criteria end to end (host grouping + rank kernel): MPR and AUC.
+int rank_eval(hipStream_t st, const int32_t* tu, const int32_t* tv, const float* tr, int64_t tnnz, const void* P,
+              const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k, int prec, float mult,
+              float b, bool transpose, double* mpr, double* auc);
+// calc_mpr_auc per listed row: rows[i] with sorted positive columns
+// pcols[pbeg[i] .. pbeg[i+1]) over ncols columns; writes u_mpr/(ncols-pos)
+// and u_auc/(ncols-pos)/pos per row.
+hipError_t launch_rank(const int32_t* rows, const int64_t* pbeg, const int32_t* pcols, int32_t nrows, int64_t ncols,
+                       const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,
+                       int prec, float mult, float b, int transpose, double* out_mpr, double* out_auc, hipStream_t st);
 // criterion: 0 RMSE (sum of squares), 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC (sums of terms)
 hipError_t launch_eval(int criterion, const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz,
                        const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,

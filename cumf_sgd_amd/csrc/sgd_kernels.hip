@@ -338,9 +338,109 @@ int rmse_blocks(int64_t nnz)
     return (int)b;
 }
 
+// ---------------------------------------------------------------------------
+// Ranking criteria (calc_mpr_auc, test/mf.cpp:3914-4034): one block per row
+// that has test positives.  Every column of the row is scored with the same
+// sequential FP32 dot as eval_kernel (b outside the model); each (negative
+// column, positive) pair counts towards MPR when the positive scores <= the
+// negative and towards AUC otherwise.  Positives are scored into LDS in
+// chunks; a column is a positive when binary search finds it in the row's
+// sorted positive columns.  Writes u_mpr / (n - pos) and
+// u_auc / (n - pos) / pos per row.
+// ---------------------------------------------------------------------------
+constexpr int kRankChunk = 2048;
+
+template <int PREC>
+__device__ inline float rank_score(const void* P, const float* ps, const void* Q, const float* qs, int32_t m,
+                                   int32_t n, int k, float mult, float b, int64_t u, int64_t v)
+{
+    if (u >= m || v >= n) return b;
+    float z = 0.0f;
+    for (int e = 0; e < k; ++e) {
+        const float pe = __fmul_rn(widen_at<PREC>(P, ps, u, k, e), mult);
+        const float qe = __fmul_rn(widen_at<PREC>(Q, qs, v, k, e), mult);
+        z = __fadd_rn(z, __fmul_rn(pe, qe));
+    }
+    return __builtin_isnan(z) ? b : z;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void rank_kernel(const int32_t* rows, const int64_t* pbeg, const int32_t* pcols,
+                                                   int32_t nrows, int64_t ncols, const void* P, const float* ps,
+                                                   const void* Q, const float* qs, int32_t m, int32_t n, int k,
+                                                   float mult, float b, int transpose, double* out_mpr,
+                                                   double* out_auc)
+{
+    __shared__ float s_pos[kRankChunk];
+    __shared__ unsigned long long red[2][256];
+    for (int32_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+        const int64_t row = rows[ri], beg = pbeg[ri], cnt = pbeg[ri + 1] - beg;
+        unsigned long long c_mpr = 0, c_auc = 0;
+        for (int64_t c0 = 0; c0 < cnt; c0 += kRankChunk) {
+            const int cc = (int)(cnt - c0 < kRankChunk ? cnt - c0 : kRankChunk);
+            __syncthreads();
+            for (int t = threadIdx.x; t < cc; t += blockDim.x) {
+                const int64_t col = pcols[beg + c0 + t];
+                s_pos[t] = transpose ? rank_score<PREC>(P, ps, Q, qs, m, n, k, mult, b, col, row)
+                                     : rank_score<PREC>(P, ps, Q, qs, m, n, k, mult, b, row, col);
+            }
+            __syncthreads();
+            for (int64_t j = threadIdx.x; j < ncols; j += blockDim.x) {
+                int64_t lo = 0, hi = cnt;   // is j one of the row's positives?
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (pcols[beg + mid] < j) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < cnt && pcols[beg + lo] == j) continue;
+                const float sj = transpose ? rank_score<PREC>(P, ps, Q, qs, m, n, k, mult, b, j, row)
+                                           : rank_score<PREC>(P, ps, Q, qs, m, n, k, mult, b, row, j);
+                for (int t = 0; t < cc; ++t) {
+                    if (s_pos[t] <= sj) ++c_mpr;
+                    else ++c_auc;
+                }
+            }
+        }
+        red[0][threadIdx.x] = c_mpr;
+        red[1][threadIdx.x] = c_auc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + w];
+                red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double neg = (double)(ncols - cnt);
+            out_mpr[ri] = (double)red[0][0] / neg;
+            out_auc[ri] = (double)red[1][0] / neg / (double)cnt;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_rank(const int32_t* rows, const int64_t* pbeg, const int32_t* pcols, int32_t nrows, int64_t ncols,
+                       const void* P, const float* ps, const void* Q, const float* qs, int32_t m, int32_t n, int k,
+                       int prec, float mult, float b, int transpose, double* out_mpr, double* out_auc, hipStream_t st)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (prec == kBF16SR) prec = kBF16;
+    const int blocks = nrows < 8192 ? nrows : 8192;
+    switch (prec) {
+    case kF32: hipLaunchKernelGGL(rank_kernel<kF32>, dim3(blocks), dim3(256), 0, st, rows, pbeg, pcols, nrows, ncols, P, ps, Q, qs, m, n, k, mult, b, transpose, out_mpr, out_auc); break;
+    case kF16: hipLaunchKernelGGL(rank_kernel<kF16>, dim3(blocks), dim3(256), 0, st, rows, pbeg, pcols, nrows, ncols, P, ps, Q, qs, m, n, k, mult, b, transpose, out_mpr, out_auc); break;
+    case kBF16: hipLaunchKernelGGL(rank_kernel<kBF16>, dim3(blocks), dim3(256), 0, st, rows, pbeg, pcols, nrows, ncols, P, ps, Q, qs, m, n, k, mult, b, transpose, out_mpr, out_auc); break;
+    case kE4M3: hipLaunchKernelGGL(rank_kernel<kE4M3>, dim3(blocks), dim3(256), 0, st, rows, pbeg, pcols, nrows, ncols, P, ps, Q, qs, m, n, k, mult, b, transpose, out_mpr, out_auc); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 bool eval_supported(int criterion)
 {
-    return criterion == 0 || criterion == 1 || criterion == 2 || criterion == 5 || criterion == 6;
+    return criterion == 0 || criterion == 1 || criterion == 2 || criterion == 5 || criterion == 6 ||
+           (criterion >= 10 && criterion <= 13);
 }
 
 template <int PREC>

@@ -2,9 +2,9 @@
 //   cumf_predict [-e eval] [--gpu N] test_file model_file [output_file]
 // Reads the same 12-byte test triplets and model file, evaluates the
 // criterion with the device kernel and prints the same result line
-// ("RMSE = 0.1234", test/mf-predict.cpp:129-162).  Criteria 0 RMSE, 1 MAE,
-// 2 GKL, 5 LOGLOSS, 6 ACC; the ranking criteria 10-13 (MPR, AUC) are not
-// provided.  output_file is accepted and, as in the reference, not written.
+// ("RMSE = 0.1234", test/mf-predict.cpp:129-182).  Criteria 0 RMSE, 1 MAE,
+// 2 GKL, 5 LOGLOSS, 6 ACC, 10/11 row/column MPR, 12/13 row/column AUC.
+// output_file is accepted and, as in the reference, not written.
 // Exit codes: 1 usage, 2 read, 3 evaluation.
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +26,10 @@ const char* kHelp =
     "\t 2 -- generalized KL-divergence\n"
     "\t 5 -- logarithmic error\n"
     "\t 6 -- accuracy\n"
+    "\t10 -- row-wise mean percentile rank\n"
+    "\t11 -- column-wise mean percentile rank\n"
+    "\t12 -- row-wise area under the curve\n"
+    "\t13 -- column-wise area under the curve\n"
     "--gpu <id>: device (default 0)\n";
 
 struct Args {
@@ -44,8 +48,8 @@ Args parse(int argc, char** argv)
             if (i + 1 >= argc) throw std::invalid_argument("need to specify evaluation criterion after -e");
             a.eval = std::atoi(argv[++i]);
             if (a.eval != CUMF_EVAL_RMSE && a.eval != CUMF_EVAL_MAE && a.eval != CUMF_EVAL_GKL &&
-                a.eval != CUMF_EVAL_LOGLOSS && a.eval != CUMF_EVAL_ACC)
-                throw std::invalid_argument("unknown evaluation criterion (supported: 0, 1, 2, 5, 6)");
+                a.eval != CUMF_EVAL_LOGLOSS && a.eval != CUMF_EVAL_ACC && (a.eval < 10 || a.eval > 13))
+                throw std::invalid_argument("unknown evaluation criterion");
         } else if (!std::strcmp(argv[i], "--gpu")) {
             if (i + 1 >= argc) throw std::invalid_argument("need to specify a device after --gpu");
             a.gpu = std::atoi(argv[++i]);
@@ -67,6 +71,10 @@ const char* label(int eval)
     case CUMF_EVAL_GKL: return "GKL";
     case CUMF_EVAL_LOGLOSS: return "LOGLOSS";
     case CUMF_EVAL_ACC: return "ACCURACY";
+    case CUMF_EVAL_ROW_MPR: return "Row-wise MPR";
+    case CUMF_EVAL_COL_MPR: return "Column-wise MPR";
+    case CUMF_EVAL_ROW_AUC: return "Row-wise AUC";
+    case CUMF_EVAL_COL_AUC: return "Colmn-wise AUC";   // the reference's spelling (mf-predict.cpp:168)
     default: return "RMSE";
     }
 }

@@ -179,10 +179,16 @@ int cumf_train(cumf_ctx* ctx, const cumf_train_opts* opts, int32_t epoch_begin, 
 int cumf_test_rmse(cumf_ctx* ctx, float mult, float b, double* rmse);
 
 /* Evaluation criteria of the reference's mf-predict -e (test/mf.h:30-40,
- * test/mf-predict.cpp:129-162), replacing calc_rmse/calc_mae/calc_gkl/
- * calc_logloss/calc_accuracy (test/mf.cpp:3814-3912) on the same predictions
- * as cumf_test_rmse.  Row/column MPR and AUC (10-13) are not provided. */
-enum cumf_eval { CUMF_EVAL_RMSE = 0, CUMF_EVAL_MAE = 1, CUMF_EVAL_GKL = 2, CUMF_EVAL_LOGLOSS = 5, CUMF_EVAL_ACC = 6 };
+ * test/mf-predict.cpp:129-182), replacing calc_rmse/calc_mae/calc_gkl/
+ * calc_logloss/calc_accuracy (test/mf.cpp:3814-3912) and calc_mpr_auc
+ * (test/mf.cpp:3914-4044) on the same predictions as cumf_test_rmse.  The
+ * ranking criteria score every column of each row (user, or item for the
+ * column-wise ones) that has test positives (r > 0); they need distinct
+ * (row, column) positives. */
+enum cumf_eval {
+    CUMF_EVAL_RMSE = 0, CUMF_EVAL_MAE = 1, CUMF_EVAL_GKL = 2, CUMF_EVAL_LOGLOSS = 5, CUMF_EVAL_ACC = 6,
+    CUMF_EVAL_ROW_MPR = 10, CUMF_EVAL_COL_MPR = 11, CUMF_EVAL_ROW_AUC = 12, CUMF_EVAL_COL_AUC = 13
+};
 int cumf_test_eval(cumf_ctx* ctx, int32_t criterion, float mult, float b, double* value);
 
 /* Load a trained FP32 model (any k up to 4096, as mf_load_model reads it,

@@ -64,6 +64,8 @@ _sig = {
     "orc_inv_shuffle": ([P, P, I32, ctypes.c_int], None),
     "orc_rmse": ([P, P, P, I64, P, P, I32, I32, ctypes.c_int, F], ctypes.c_double),
     "orc_eval": ([ctypes.c_int, P, P, P, I64, P, P, I32, I32, ctypes.c_int, F], ctypes.c_double),
+    "orc_mpr_auc": ([P, P, P, I64, P, P, I32, I32, ctypes.c_int, F, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                     ctypes.POINTER(ctypes.c_double)], None),
     "orc_save_model": ([ctypes.c_char_p, I32, I32, I32, F, P, P], ctypes.c_int),
 }
 for _n, (_a, _r) in _sig.items():
@@ -238,11 +240,28 @@ def rmse(u, v, r, P_, Q_, b):
     return _lib.orc_rmse(_p(u), _p(v), _p(r), len(u), _p(P_), _p(Q_), P_.shape[0], Q_.shape[0], P_.shape[1], b)
 
 
-RMSE, MAE, GKL, LOGLOSS, ACC = 0, 1, 2, 5, 6
+RMSE, MAE, GKL, LOGLOSS, ACC, ROW_MPR, COL_MPR, ROW_AUC, COL_AUC = 0, 1, 2, 5, 6, 10, 11, 12, 13
+
+
+def mpr_auc(u, v, r, P_, Q_, b, transpose: bool):
+    """calc_mpr_auc (test/mf.cpp:3914-4034): returns (MPR, AUC)."""
+    P_ = np.ascontiguousarray(P_, np.float32)
+    Q_ = np.ascontiguousarray(Q_, np.float32)
+    u = np.ascontiguousarray(u, np.int32)
+    v = np.ascontiguousarray(v, np.int32)
+    r = np.ascontiguousarray(r, np.float32)
+    a, c = ctypes.c_double(), ctypes.c_double()
+    _lib.orc_mpr_auc(_p(u), _p(v), _p(r), len(u), _p(P_), _p(Q_), P_.shape[0], Q_.shape[0], P_.shape[1], b,
+                     int(transpose), ctypes.byref(a), ctypes.byref(c))
+    return a.value, c.value
 
 
 def evaluate(criterion, u, v, r, P_, Q_, b):
-    """mf-predict's -e criterion on float factors (0 RMSE, 1 MAE, 2 GKL, 5 LOGLOSS, 6 ACC)."""
+    """mf-predict's -e criterion on float factors (0 RMSE, 1 MAE, 2 GKL,
+    5 LOGLOSS, 6 ACC, 10/11 row/column MPR, 12/13 row/column AUC)."""
+    if criterion in (ROW_MPR, COL_MPR, ROW_AUC, COL_AUC):
+        mpr, auc = mpr_auc(u, v, r, P_, Q_, b, criterion in (COL_MPR, COL_AUC))
+        return mpr if criterion in (ROW_MPR, COL_MPR) else auc
     P_ = np.ascontiguousarray(P_, np.float32)
     Q_ = np.ascontiguousarray(Q_, np.float32)
     u = np.ascontiguousarray(u, np.int32)
@@ -267,7 +286,9 @@ def mf_predict_rmse(test_path: str, model_path: str) -> str:
     return mf_predict_eval(test_path, model_path, RMSE)
 
 
-_EVAL_LABEL = {RMSE: "RMSE = ", MAE: "MAE = ", GKL: "GKL = ", LOGLOSS: "LOGLOSS = ", ACC: "ACCURACY = "}
+_EVAL_LABEL = {RMSE: "RMSE = ", MAE: "MAE = ", GKL: "GKL = ", LOGLOSS: "LOGLOSS = ", ACC: "ACCURACY = ",
+               ROW_MPR: "Row-wise MPR = ", COL_MPR: "Column-wise MPR = ", ROW_AUC: "Row-wise AUC = ",
+               COL_AUC: "Colmn-wise AUC = "}
 
 
 def mf_predict_eval(test_path: str, model_path: str, criterion: int) -> str:

@@ -770,6 +770,103 @@ EXPORT double orc_eval(int criterion, const int32_t* tu, const int32_t* tv, cons
     return acc / nnz;
 }
 
+/* calc_mpr_auc (test/mf.cpp:3914-4034): per row (user, or item when
+ * transposed) with test positives (r > 0), every column of the row is scored
+ * with mf_predict; each (negative column, positive) pair counts towards MPR
+ * when the positive scores <= the negative and towards AUC otherwise.
+ * Restated as the reference computes it: rows sorted by (row, col), the
+ * row's positives swapped to the front and sorted by score, a binary search
+ * per negative.  m, n = max(test extent, model extent).  Test pairs must be
+ * distinct (the reference's swap assumes it). */
+static int cmp_pair(const void* a, const void* b)
+{
+    const int64_t* x = (const int64_t*)a;
+    const int64_t* y = (const int64_t*)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+    return 0;
+}
+static int cmp_score(const void* a, const void* b)
+{
+    float x = *(const float*)a, y = *(const float*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+EXPORT void orc_mpr_auc(const int32_t* tu, const int32_t* tv, const float* tr, int64_t nnz, const float* P,
+                        const float* Q, int32_t model_m, int32_t model_n, int k, float b, int transpose,
+                        double* mpr_out, double* auc_out)
+{
+    int64_t pm = 0, pn = 0;   /* read_problem's extents: largest id + 1 */
+    for (int64_t i = 0; i < nnz; ++i) {
+        if (tu[i] + 1 > pm) pm = tu[i] + 1;
+        if (tv[i] + 1 > pn) pn = tv[i] + 1;
+    }
+    int64_t m = transpose ? (pn > model_n ? pn : model_n) : (pm > model_m ? pm : model_m);
+    int64_t n = transpose ? (pm > model_m ? pm : model_m) : (pn > model_n ? pn : model_n);
+    /* (row, col, r-as-bits) sorted by (row, col) */
+    int64_t* rec = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)(nnz ? nnz : 1));
+    for (int64_t i = 0; i < nnz; ++i) {
+        rec[3 * i] = transpose ? tv[i] : tu[i];
+        rec[3 * i + 1] = transpose ? tu[i] : tv[i];
+        uint32_t bits;
+        memcpy(&bits, &tr[i], 4);
+        rec[3 * i + 2] = bits;
+    }
+    qsort(rec, (size_t)nnz, sizeof(int64_t) * 3, cmp_pair);
+    int64_t* cnt = (int64_t*)calloc((size_t)m + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < nnz; ++i) cnt[rec[3 * i] + 1]++;
+    for (int64_t i = 1; i <= m; ++i) cnt[i] += cnt[i - 1];
+    float* score = (float*)malloc(sizeof(float) * (size_t)(n ? n : 1));
+    int64_t* col = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+    char* is_pos = (char*)calloc((size_t)(n ? n : 1), 1);
+    float* ps = (float*)malloc(sizeof(float) * (size_t)(n ? n : 1));
+    int64_t total_m = 0, total_pos = 0;
+    double all_mpr = 0, all_auc = 0;
+    for (int64_t i = 0; i < m; ++i) {
+        if (cnt[i + 1] - cnt[i] < 1) continue;
+        for (int64_t j = 0; j < n; ++j) {
+            int32_t uu = (int32_t)(transpose ? j : i), vv = (int32_t)(transpose ? i : j);
+            score[j] = predict_one(P, Q, model_m, model_n, k, b, uu, vv);
+            col[j] = j;
+            is_pos[j] = 0;
+        }
+        int64_t pos = 0;
+        for (int64_t t = cnt[i]; t < cnt[i + 1]; ++t) {
+            uint32_t bits = (uint32_t)rec[3 * t + 2];
+            float r;
+            memcpy(&r, &bits, 4);
+            if (r <= 0) continue;
+            is_pos[rec[3 * t + 1]] = 1;
+            ps[pos++] = score[rec[3 * t + 1]];
+        }
+        if (n - pos < 1 || pos < 1) continue;
+        total_m++;
+        total_pos += pos;
+        qsort(ps, (size_t)pos, sizeof(float), cmp_score);
+        double u_mpr = 0, u_auc = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (is_pos[j]) continue;
+            float s = score[j];
+            if (ps[pos - 1] <= s) {
+                u_mpr += (double)pos;
+                continue;
+            }
+            int64_t left = 0, right = pos - 1;
+            while (left < right) {
+                int64_t mid = (left + right) / 2;
+                if (ps[mid] > s) right = mid;
+                else left = mid + 1;
+            }
+            u_mpr += (double)left;
+            u_auc += (double)(pos - left);
+        }
+        all_mpr += u_mpr / (double)(n - pos);
+        all_auc += u_auc / (double)(n - pos) / (double)pos;
+    }
+    *mpr_out = total_pos ? all_mpr / (double)total_pos : NAN;
+    *auc_out = total_m ? all_auc / (double)total_m : NAN;
+    free(rec); free(cnt); free(score); free(col); free(is_pos); free(ps);
+}
+
 /* save_model, singleGPU/main.cpp:231-273: header {int 0, int m, int n, int k,
  * float b}, then m rows of P and n rows of Q as float32. */
 EXPORT int orc_save_model(const char* path, int32_t m, int32_t n, int32_t k, float b,

@@ -3,7 +3,8 @@ reference's own evaluator (test/mf-predict, compiled from the reference's
 sources into oracle/_ref/ by oracle/Makefile.ref).
 
 For each case it writes a small model file and test file, runs
-`mf-predict -e c` for c in 0 (RMSE), 1 (MAE), 2 (GKL), 5 (LOGLOSS), 6 (ACC),
+`mf-predict -e c` for c in 0 (RMSE), 1 (MAE), 2 (GKL), 5 (LOGLOSS), 6 (ACC)
+(and 10-13, row/column MPR and AUC, on the case with distinct pairs),
 and stores the inputs (npz) and the printed 4-decimal values
 (mf_predict_vectors.json) beside this script.
 
@@ -23,6 +24,7 @@ import oracle  # noqa: E402
 from cumf_sgd_amd import datagen  # noqa: E402
 
 CRITERIA = {"rmse": 0, "mae": 1, "gkl": 2, "logloss": 5, "acc": 6}
+RANKING = {"row_mpr": 10, "col_mpr": 11, "row_auc": 12, "col_auc": 13}
 
 
 def cases():
@@ -53,6 +55,16 @@ def cases():
     tv = rng.integers(0, n, 2000).astype(np.int32)
     tr = (rng.random(2000) * 20 + 0.5).astype(np.float32)
     out.append(("wide_k8", tu, tv, tr, P, Q, np.float32(7.0)))
+    # distinct pairs with +-1 labels and columns past the model: the ranking
+    # criteria (10-13) are defined on distinct positives
+    m, n, k = 50, 35, 16
+    P = rng.normal(0, 0.4, (m, k)).astype(np.float32)
+    Q = rng.normal(0, 0.4, (n, k)).astype(np.float32)
+    flat = rng.choice((m + 2) * (n + 3), 900, replace=False)
+    tu = (flat // (n + 3)).astype(np.int32)
+    tv = (flat % (n + 3)).astype(np.int32)
+    tr = np.where(rng.random(900) < 0.6, 1.0, -1.0).astype(np.float32)
+    out.append(("ranking", tu, tv, tr, P, Q, np.float32(0.05)))
     return out
 
 
@@ -68,6 +80,9 @@ def main():
             entry = {"file": f"mf_predict_{name}.npz"}
             for key, c in CRITERIA.items():
                 entry[key] = oracle.mf_predict_eval(test, model, c)
+            if name == "ranking":
+                for key, c in RANKING.items():
+                    entry[key] = oracle.mf_predict_eval(test, model, c)
             np.savez_compressed(os.path.join(HERE, entry["file"]), tu=tu, tv=tv, tr=tr, P=P, Q=Q, b=b)
             res["cases"].append(entry)
             print(entry)

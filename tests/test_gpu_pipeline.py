@@ -282,3 +282,65 @@ def test_cli_file_pipeline_equals_host_pipeline_in_serial_mode(gpu, tmp_path):
     cs.save_model(host_model, P, Q, b)
     assert open(model, "rb").read() == open(host_model, "rb").read()
     assert rm == [f"{x:.4f}" for x in rmse]
+
+
+def test_ranking_criteria_match_oracle_and_mf_predict(gpu, tmp_path):
+    """Row/column MPR and AUC (mf-predict -e 10..13) on the GPU: equal to the
+    oracle's restatement of calc_mpr_auc up to the FP64 summation order (the
+    scores are the same sequential FP32 dots, so every comparison agrees), in
+    every storage format's widened values, and printed by cumf_predict as the
+    reference's compiled mf-predict prints them; duplicate positives are
+    rejected."""
+    import cumf_sgd_amd as cs
+    import oracle
+    from cumf_sgd_amd import datagen
+
+    rng = np.random.default_rng(13)
+    m, n, k = 3000, 900, 64
+    P = rng.normal(0, 0.3, (m, k)).astype(np.float32)
+    Q = rng.normal(0, 0.3, (n, k)).astype(np.float32)
+    flat = rng.choice((m + 5) * (n + 4), 60_000, replace=False)
+    tu = (flat // (n + 4)).astype(np.int32)
+    tv = (flat % (n + 4)).astype(np.int32)
+    tr = np.where(rng.random(len(tu)) < 0.5, 1.0, -1.0).astype(np.float32)
+    b = np.float32(0.1)
+    with cs.Context(0) as c:
+        c.set_model(P, Q)
+        c.set_test(tu, tv, tr)
+        for crit in (10, 11, 12, 13):
+            got = c.test_eval(crit, 1.0, b)
+            want = oracle.evaluate(crit, tu, tv, tr, P, Q, b)
+            assert abs(got - want) <= 1e-12 * abs(want), (crit, got, want)
+        dup_u, dup_v = np.concatenate([tu, tu[:1]]), np.concatenate([tv, tv[:1]])
+        c.set_test(dup_u, dup_v, np.concatenate([tr, np.ones(1, np.float32)]))
+        if tr[0] > 0:
+            with pytest.raises(cs.CumfError):
+                c.test_eval(10, 1.0, b)
+    # a trained FP16 model through the export scale (mult)
+    u, v, r = tu[tr > 0] % m, tv[tr > 0] % n, np.ones(int((tr > 0).sum()), np.float32)
+    with cs.Context(0) as c:
+        c.set_ratings(u, v, r, m, n)
+        c.init_factors(64, cs.FP16, seed=5)
+        c.train(cs.make_opts(k=64, schedule=cs.HOGWILD, seed=5), 0, 2)
+        c.set_test(tu, tv, tr)
+        raw_p, _ = c.get_factors(0)
+        raw_q, _ = c.get_factors(1)
+        mult = np.float32(1.3)
+        Pw = (oracle.widen(raw_p, None, 1) * mult).astype(np.float32)
+        Qw = (oracle.widen(raw_q, None, 1) * mult).astype(np.float32)
+        for crit in (10, 13):
+            got = c.test_eval(crit, mult, b)
+            want = oracle.evaluate(crit, tu, tv, tr, Pw, Qw, b)
+            assert abs(got - want) <= 1e-12 * abs(want), (crit, got, want)
+    model, test = str(tmp_path / "model"), str(tmp_path / "test.bin")
+    cs.save_model(model, P, Q, b)
+    datagen.write_triplets(test, tu, tv, tr)
+    label = {10: "Row-wise MPR", 11: "Column-wise MPR", 12: "Row-wise AUC", 13: "Colmn-wise AUC"}
+    for crit in (10, 11, 12, 13):
+        out = subprocess.run([cs.PREDICT_PATH, "-e", str(crit), test, model], capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        line = out.stdout.strip().splitlines()[-1]
+        assert line.startswith(label[crit] + " = "), line
+        if oracle.have_mf_predict():
+            assert line.split("=", 1)[1].strip() == oracle.mf_predict_eval(test, model, crit)

@@ -188,7 +188,7 @@ def _predict(*args):
 def test_predict_cli_usage_and_read_errors(tmp_path):
     out = _predict()
     assert out.returncode == 1 and "usage: cumf_predict" in out.stdout
-    out = _predict("-e", "10", "a", "b")
+    out = _predict("-e", "7", "a", "b")
     assert out.returncode == 1 and "unknown evaluation criterion" in out.stdout
     out = _predict("-e")
     assert out.returncode == 1 and "after -e" in out.stdout

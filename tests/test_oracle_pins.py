@@ -367,12 +367,16 @@ def test_evaluation_criteria_match_committed_mf_predict_vectors():
     """RMSE, MAE, GKL, LOGLOSS and ACC of the oracle against the 4-decimal
     values the reference's compiled mf-predict printed for the committed
     cases (tests/golden/make_mf_predict_vectors.py): ratings 1-5 with
-    out-of-range ids, +-1 labels with a NaN factor row, a wide value range."""
+    out-of-range ids, +-1 labels with a NaN factor row, a wide value range,
+    and row/column MPR and AUC on distinct +-1 pairs past the model's extent."""
     g = json.load(open(os.path.join(GOLDEN, "mf_predict_vectors.json")))
-    assert len(g["cases"]) == 3
+    assert len(g["cases"]) == 4 and "row_auc" in g["cases"][3]
     for case in g["cases"]:
         d = np.load(os.path.join(GOLDEN, case["file"]))
-        for key, crit in (("rmse", 0), ("mae", 1), ("gkl", 2), ("logloss", 5), ("acc", 6)):
+        for key, crit in (("rmse", 0), ("mae", 1), ("gkl", 2), ("logloss", 5), ("acc", 6), ("row_mpr", 10),
+                          ("col_mpr", 11), ("row_auc", 12), ("col_auc", 13)):
+            if key not in case:
+                continue
             mine = oracle.evaluate(crit, d["tu"], d["tv"], d["tr"], d["P"], d["Q"], np.float32(d["b"]))
             want = case[key].replace("-nan", "nan")
             assert _fmt4(mine) == want, (case["file"], key, mine, want)
