@@ -629,8 +629,8 @@ int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
     HIP_TRY(hipMalloc(&dv.p, sizeof(int32_t) * (size_t)c->n));
     HIP_TRY(hipMemsetAsync(du.p, 0, sizeof(int32_t) * (size_t)c->m, c->stream));
     HIP_TRY(hipMemsetAsync(dv.p, 0, sizeof(int32_t) * (size_t)c->n, c->stream));
-    HIP_TRY(launch_degree(c->u, c->nnz, (int32_t*)du.p, c->stream));
-    HIP_TRY(launch_degree(c->v, c->nnz, (int32_t*)dv.p, c->stream));
+    HIP_TRY(launch_degree(c->u, c->nnz, (int32_t*)du.p, c->m, c->stream));
+    HIP_TRY(launch_degree(c->v, c->nnz, (int32_t*)dv.p, c->n, c->stream));
     std::vector<int32_t> hu(c->m), hv(c->n);
     HIP_TRY(hipMemcpyAsync(hu.data(), du.p, sizeof(int32_t) * c->m, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(hv.data(), dv.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));

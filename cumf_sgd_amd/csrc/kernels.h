@@ -90,6 +90,6 @@ hipError_t launch_partition(const int32_t* u, const int32_t* v, const float* r, 
                             const int32_t* vseg, int vy, int nb, int32_t* hist, int64_t* tot, int64_t* base,
                             int64_t* off, int32_t* uo, int32_t* vo, float* ro, hipStream_t st, int64_t* host_tot);
 hipError_t launch_seg_table(int32_t* seg, int64_t rows, const int64_t* bnd, int ns, hipStream_t st);
-hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, hipStream_t st);
+hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, int32_t id_range, hipStream_t st);
 
 }  // namespace cumf

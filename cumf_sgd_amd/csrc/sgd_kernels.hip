@@ -760,9 +760,36 @@ __global__ void degree_kernel(const int32_t* ids, int64_t nnz, int32_t* deg)
         atomicAdd(&deg[ids[i]], 1);
 }
 
-hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, hipStream_t st)
+// Degrees of the ids in [lo, lo + kDegreeBins) counted in LDS first: a few
+// hot ids (Zipf items) would serialise on global atomics.
+constexpr int kDegreeBins = 16384;
+
+__global__ __launch_bounds__(256) void degree_lds_kernel(const int32_t* ids, int64_t nnz, int32_t lo, int32_t* deg)
+{
+    __shared__ int32_t h[kDegreeBins];
+    for (int b = threadIdx.x; b < kDegreeBins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = (uint32_t)(ids[i] - lo);
+        if (d < (uint32_t)kDegreeBins) atomicAdd(&h[d], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kDegreeBins; b += blockDim.x)
+        if (h[b]) atomicAdd(&deg[lo + b], h[b]);
+}
+
+hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, int32_t id_range, hipStream_t st)
 {
     if (nnz <= 0) return hipSuccess;
+    if (id_range <= 8 * kDegreeBins) {
+        const int blocks = grid_for(nnz, 256) < 2048 ? grid_for(nnz, 256) : 2048;
+        for (int32_t lo = 0; lo < id_range; lo += kDegreeBins) {
+            hipLaunchKernelGGL(degree_lds_kernel, dim3(blocks), dim3(256), 0, st, ids, nnz, lo, deg);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     hipLaunchKernelGGL(degree_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, ids, nnz, deg);
     return hipGetLastError();
 }
