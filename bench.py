@@ -142,7 +142,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design (see DESIGN.md), "
+        print(json.dumps({"error": "one GPU only: multi-GPU sharding is designed, not built; "
                                    "run with --gpus 1"}))
         return 2
 

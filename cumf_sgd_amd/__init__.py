@@ -91,7 +91,7 @@ class Node(ctypes.Structure):
 
 def _load():
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        raise ImportError(f"{LIB_PATH} is not built; run `python cumf_sgd_amd/build.py`")
     lib = ctypes.CDLL(LIB_PATH)
     P, I32, I64, U64, F32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
     D = ctypes.POINTER(ctypes.c_double)

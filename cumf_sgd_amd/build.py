@@ -1,10 +1,10 @@
-"""Build recipe: the gfx950 library, the cumf_sgd executable and the checker.
+"""Build recipe: the gfx950 library, the cumf_sgd executable and the CPU oracle.
 
 Produces, in-tree:
   cumf_sgd_amd/libcumf_sgd_amd.so   HIP kernels + C ABI + host pipeline
   cumf_sgd_amd/cumf_sgd             command-line trainer (singleGPU/main.cpp's CLI)
   cumf_sgd_amd/cumf_predict         GPU evaluator (test/mf-predict's CLI)
-  oracle/liboracle.so, oracle/_ref/ checker only (tests and benchmark baseline)
+  oracle/liboracle.so, oracle/_ref/ oracle only (tests and benchmark baseline)
 """
 from __future__ import annotations
 

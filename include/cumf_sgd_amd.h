@@ -130,7 +130,7 @@ int cumf_set_test_soa(cumf_ctx* ctx, const int32_t* u, const int32_t* v, const f
 int cumf_alloc_ratings(cumf_ctx* ctx, int64_t nnz, int32_t m, int32_t n, int32_t** u, int32_t** v, float** r);
 int cumf_commit_ratings(cumf_ctx* ctx);
 
-/* Device copies of the ratings (3 arrays of nnz), e.g. for a checker. */
+/* Copies of the ratings as trained (3 arrays of nnz), e.g. for a test. */
 int cumf_get_ratings(cumf_ctx* ctx, int32_t* u, int32_t* v, float* r);
 
 /* ---- on-device preprocessing (singleGPU/sgd.cu:78-134) ------------------ */

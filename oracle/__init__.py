@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle.c) and the reference's CPU tools.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, always as the checker, never as the product.
+CPU-baseline leg of bench.py, always as the reference to check against, never as
+the product.
 """
 from __future__ import annotations
 

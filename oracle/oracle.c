@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
  * C++ host, the Python package) links, loads or calls this file.  Only the
  * tests, the smoke check and the CPU-baseline leg of the benchmark use it, and
- * only as the checker.
+ * only as the reference to check against.
  *
  * Each function restates one piece of the reference's algorithm (cited as
  * file:line under the reference tree) or one piece of this project's own
