@@ -62,13 +62,14 @@ def prepare(ctx, R, seed: int):
     return scale, np.float32(ave / std), np.float32(np.float32(0.05) / scale)
 
 
-def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, depth=0, rmse_every=True):
+def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, depth=0, batch_len=0,
+               rmse_every=True):
     """Fresh factors, `epochs` epochs; returns (per-epoch train seconds, rmse list)."""
     import cumf_sgd_amd as cs
 
     ctx.init_factors(K, prec, seed)
     opts = cs.make_opts(k=K, schedule=schedule, num_workers=workers, lambda_p=float(lam), seed=seed,
-                        lanes_per_sample=lanes, pipeline_depth=depth)
+                        lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len)
     fs = np.float32(np.sqrt(scale))
     times, rmses = [], []
     for ep in range(epochs):
@@ -187,8 +188,8 @@ def main() -> int:
         t1 = time.perf_counter()
         opts, tuned_ms = ctx.autotune(opts)
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
-            f"workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
-    tuned = (opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth)
+            f"batch_len={opts.batch_len} workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
+    tuned = (opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth, opts.batch_len)
     for ep in range(args.warmup):
         ctx.train(opts, ep, ep + 1)
     ctx.sync()
@@ -218,8 +219,8 @@ def main() -> int:
         for p in (0, 1, 2, 4, 3):
             # the headline format trains with the measured shape, the others
             # with their defaults (or what the command line fixed)
-            w, l, d = tuned if p == prec else (args.workers, args.lanes, args.depth)
-            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d)
+            w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, 0)
+            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl)
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005

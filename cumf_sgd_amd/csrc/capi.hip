@@ -935,7 +935,7 @@ struct Shape {
 // fewest workers that keep the same windows per worker, so an epoch
 // over-samples the ratings as little as possible
 constexpr int kDefaultWavesPerCu = 128;
-constexpr int kDefaultWavesPerCuFp8 = 512;   // FP8 is ALU-heavier: more, shorter workers hide it best
+constexpr int kDefaultWavesPerCuFp8 = 512;   // FP8 and stratum launches: more, shorter workers
 
 int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
 {
@@ -969,7 +969,10 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     } else if (o->num_workers > 0) {
         W = o->num_workers;
     } else {
-        W = default_workers(c, sh->S, sh->L, c->prec == CUMF_FP8_E4M3 ? kDefaultWavesPerCuFp8 : kDefaultWavesPerCu);
+        // stratum launches are short: one window per worker (the per-block
+        // cap) measured fastest; FP8 likewise
+        const bool one_window = o->schedule == CUMF_SCHED_STRATUM || c->prec == CUMF_FP8_E4M3;
+        W = default_workers(c, sh->S, sh->L, one_window ? kDefaultWavesPerCuFp8 : kDefaultWavesPerCu);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;
@@ -1375,38 +1378,46 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
     const int64_t keep_l = c->launches, keep_u = c->updates;
     const cumf_launch_info keep_last = c->last;
 
-    const int groups[4] = {64, 32, 16, 8};
-    const int depths[3] = {2, 4, 8};
-    const int waves_per_cu[5] = {32, 64, 128, 256, 512};
-    const int L = base->batch_len ? base->batch_len : 128;
+    // a field the caller set (non-zero) stays fixed; the zero ones are searched
+    std::vector<int> groups{64, 32, 16, 8}, depths{2, 4, 8}, waves_per_cu{32, 64, 128, 256, 512}, lens{128, 64, 32};
+    if (base->lanes_per_sample) groups = {base->lanes_per_sample};
+    if (base->pipeline_depth) depths = {base->pipeline_depth};
+    if (base->batch_len) lens = {base->batch_len};
     double best_t = 1e300;
     cumf_train_opts bo = *base;
-    bool warm = false;
-    for (int G : groups) {
-        if (!sgd_shape_supported(c->k, c->prec, G)) continue;
-        if (base->coherent && (c->k / G) * (int)eb < 4) continue;
-        for (int D : depths)
-            for (int wpc : waves_per_cu) {
-                cumf_train_opts o = *base;
-                o.lanes_per_sample = G;
-                o.pipeline_depth = D;
-                o.num_workers = default_workers(c, 64 / G, L, wpc);
-                o.use_graph = 0;
-                if (!warm) {   // first launch of the code object is not timed
+    bool warm = false, any = false;
+    for (int L : lens)
+        for (int G : groups) {
+            if (!sgd_shape_supported(c->k, c->prec, G)) continue;
+            if (base->coherent && (c->k / G) * (int)eb < 4) continue;
+            for (int D : depths) {
+                if (L % D) continue;
+                for (size_t wi = 0; wi < (base->num_workers ? 1 : waves_per_cu.size()); ++wi) {
+                    cumf_train_opts o = *base;
+                    o.lanes_per_sample = G;
+                    o.pipeline_depth = D;
+                    o.batch_len = L;
+                    o.num_workers = base->num_workers ? base->num_workers
+                                                      : default_workers(c, 64 / G, L, waves_per_cu[wi]);
+                    o.use_graph = 0;
+                    if (!warm) {   // first launch of the code object is not timed
+                        if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
+                        warm = true;
+                    }
+                    c->kernel_ms = 0;
                     if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
-                    warm = true;
-                }
-                c->kernel_ms = 0;
-                if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;
-                // the epoch's time, over-sampling included: that is what an
-                // epoch costs the caller
-                const double t = c->kernel_ms;
-                if (t < best_t) {
-                    best_t = t;
-                    bo = o;
+                    // the epoch's time, over-sampling included: that is what an
+                    // epoch costs the caller
+                    const double t = c->kernel_ms;
+                    any = true;
+                    if (t < best_t) {
+                        best_t = t;
+                        bo = o;
+                    }
                 }
             }
-    }
+        }
+    if (!any) return fail(CUMF_ERR_ARG, "no supported launch shape for k=%d precision=%d with the fixed fields", c->k, c->prec);
     HIP_TRY(hipMemcpyAsync(c->P, sp.p, pb, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->Q, sq.p, qb, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == CUMF_FP8_E4M3) {

@@ -63,7 +63,7 @@ typedef struct cumf_node {
 /* Training options.  Field meanings follow Parameter (singleGPU/sgd.h:26-46). */
 typedef struct cumf_train_opts {
     int32_t k;                 /* factor length: 32, 64 or 128 */
-    int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU (FP8: 512), rounded for least over-sampling */
+    int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU (FP8 or stratum: 512), rounded for least over-sampling */
     int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128 */
     int32_t schedule;          /* enum cumf_schedule */
     int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane */
@@ -197,12 +197,14 @@ int cumf_test_eval(cumf_ctx* ctx, int32_t criterion, float mult, float b, double
  * {32, 64, 128}. */
 int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* P, const float* Q);
 
-/* Pick the launch shape: time one epoch of every supported (lanes_per_sample,
- * pipeline_depth, workers for {32, 64, 128, 256, 512} waves per CU) for base's
- * schedule on this context's problem, write the one with the shortest epoch
- * into *best (other fields copied from base) and its epoch's kernel time in
- * ms into *best_ms, then restore the factors the trials trained.  Kernel-time
- * totals and last_launch are not changed.  A serial schedule returns base. */
+/* Pick the launch shape: time one epoch of every supported combination of
+ * lanes_per_sample {64, 32, 16, 8}, pipeline_depth {2, 4, 8}, batch_len
+ * {128, 64, 32} and workers for {32, 64, 128, 256, 512} waves per CU (a field
+ * that is non-zero in base stays fixed) for base's schedule on this
+ * context's problem, write the one with the shortest epoch into *best (other
+ * fields copied from base) and its epoch's kernel time in ms into *best_ms,
+ * then restore the factors the trials trained.  Kernel-time totals and
+ * last_launch are not changed.  A serial schedule returns base. */
 int cumf_autotune(cumf_ctx* ctx, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms);
 
 /* Time spent in training kernels since the last reset (HIP events on the

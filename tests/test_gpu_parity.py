@@ -361,6 +361,9 @@ def test_autotune_restores_factors_and_picks_a_valid_shape(gpu, prec, sched):
         assert (last1.num_workers, last1.lanes_per_sample, last1.pipeline_depth) == \
             (last0.num_workers, last0.lanes_per_sample, last0.pipeline_depth)
         assert best.lanes_per_sample in (8, 16, 32, 64) and best.pipeline_depth in (2, 4, 8)
+        assert best.batch_len in (32, 64, 128) and best.batch_len % best.pipeline_depth == 0
+        fixed, _ = c.autotune(cs.make_opts(k=64, schedule=schedule, seed=5, lanes_per_sample=16, batch_len=64))
+        assert fixed.lanes_per_sample == 16 and fixed.batch_len == 64
         assert best.num_workers > 0 and best.schedule == schedule
         c.train(best, 1, 3)
         info = c.last_launch()
