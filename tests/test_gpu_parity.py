@@ -443,3 +443,63 @@ def test_streamed_ratings_train_the_medium_workload(gpu):
     print(f"resident {res[False][1] / 1e9:.2f}e9 updates/s rmse {res[False][0]:.5f}; streamed "
           f"{res[True][1] / 1e9:.2f}e9 updates/s rmse {res[True][0]:.5f}")
     assert abs(res[True][0] - res[False][0]) < 0.005
+
+
+@pytest.mark.parametrize("nnz", [1, 2, 127, 129, 1000])
+@pytest.mark.parametrize("prec", [1, 3])
+def test_ragged_and_tiny_inputs(gpu, nnz, prec):
+    """Windows longer than the data (nnz < 128), one rating past a window,
+    a single user and item: serialized epochs stay bit-exact with the
+    oracle; Hogwild and stratum epochs run and stay finite."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    rng = np.random.default_rng(nnz)
+    m, n = (1, 1) if nnz <= 2 else (37, 23)
+    u = rng.integers(0, m, nnz).astype(np.int32)
+    v = rng.integers(0, n, nnz).astype(np.int32)
+    r = rng.normal(0, 1, nnz).astype(np.float32)
+    k = 32
+    G = 8
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(k, prec, seed=2)
+        P0, ps0 = c.get_factors(0)
+        Q0, qs0 = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.SERIAL, seed=2, lanes_per_sample=G), 0, 2)
+        P1, _ = c.get_factors(0)
+        Q1, _ = c.get_factors(1)
+        lr = oracle.lr_table(0.08, 0.3, 0.0, 6)
+        Pw, Qw, psw, qsw = P0, Q0, ps0, qs0
+        for ep in range(2):
+            Pw, Qw, psw, qsw = oracle.sgd_serial(u, v, r, Pw, Qw, lr, ep, ep + 1, k=k, G=G, prec=prec, lambda_p=0.05,
+                                                 lambda_q=0.05, seed=2, ps=psw, qs=qsw)
+            if prec == 3:
+                Pw, psw = oracle.fp8_rescale(Pw, psw)
+                Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+        assert np.array_equal(P1.view(np.uint8), Pw.view(np.uint8))
+        assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))
+        c.train(cs.make_opts(k=k, schedule=cs.HOGWILD, seed=2), 2, 4)
+        off = c.partition_strata(2)
+        assert off[-1] == nnz
+        c.train(cs.make_opts(k=k, schedule=cs.STRATUM, seed=2), 4, 5)
+        P2, ps2 = c.get_factors(0)
+        assert np.all(np.isfinite(oracle.widen(P2, ps2, prec)))
+        c.set_test(u, v, r)
+        assert np.isfinite(c.test_rmse())
+
+
+def test_empty_ratings_and_test_set(gpu):
+    """No ratings: training is a no-op that succeeds; an empty test set has
+    RMSE 0 (test/mf.cpp:3818-3819)."""
+    import cumf_sgd_amd as cs
+
+    e = np.zeros(0, np.int32)
+    with cs.Context(0) as c:
+        c.set_ratings(e, e, np.zeros(0, np.float32), 5, 4)
+        c.init_factors(64, cs.FP16, seed=1)
+        P0, _ = c.get_factors(0)
+        c.train(cs.make_opts(k=64, schedule=cs.HOGWILD), 0, 3)
+        assert np.array_equal(c.get_factors(0)[0], P0)
+        assert c.kernel_time().launches == 0
+        c.set_test(e, e, np.zeros(0, np.float32))
+        assert c.test_rmse() == 0.0
