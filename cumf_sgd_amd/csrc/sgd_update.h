@@ -149,6 +149,10 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         const int ep = ep0 + w / T.update_count;
         const int b = w % T.update_count;
         const float lr = LR[ep];
+        // p + lr (e q - lp p) = p (1 - lr lp) + (lr e) q: one multiply and one
+        // fma per element
+        const float cp = __builtin_fmaf(-lr, T.lambda_p, 1.0f);
+        const float cq = __builtin_fmaf(-lr, T.lambda_q, 1.0f);
         U4 srk = U4{0u, 0u, 0u, 0u};
         if constexpr (SR) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
         stage_window();
@@ -198,6 +202,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
                 const float dot = slot_sum<G>(part);
                 const float err = rc - dot;
+                const float lerr = lr * err;
 
                 Words<NW> po, qo;
                 uint32_t rbits[4] = {0u, 0u, 0u, 0u};
@@ -205,8 +210,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 const float qinv = FP8 ? 1.0f / qsc : 1.0f;
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const float pn = __builtin_fmaf(lr, __builtin_fmaf(err, q[e], -(T.lambda_p * p[e])), p[e]);
-                    const float qn = __builtin_fmaf(lr, __builtin_fmaf(err, p[e], -(T.lambda_q * q[e])), q[e]);
+                    const float pn = __builtin_fmaf(lerr, q[e], p[e] * cp);
+                    const float qn = __builtin_fmaf(lerr, p[e], q[e] * cq);
                     uint32_t pb = 0u, qb = 0u;
                     if constexpr (PREC == kF32) {
                         pb = __float_as_uint(pn);

@@ -489,7 +489,9 @@ EXPORT void orc_scale_problem(orc_node* R, int64_t nnz, float scale, int ux, int
 /*  - the G partials are summed by a pairwise butterfly (lane l adds lane    */
 /*    l^1, then l^2, l^4, ...), i.e. a balanced tree in lane order;          */
 /*  - e = r - dot;                                                           */
-/*  - p' = fma(lr, fma(e, q, -(lp*p)), p), q' = fma(lr, fma(e, p, -(lq*q)), q).*/
+/*  - cp = fma(-lr, lp, 1), cq = fma(-lr, lq, 1), le = lr*e;                */
+/*    p' = fma(le, q, p*cp), q' = fma(le, p, q*cq) (the same step: p (1 -    */
+/*    lr lp) + lr e q).                                                      */
 /* ------------------------------------------------------------------------ */
 static float tree_dot(const float* p, const float* q, int k, int G, float* lanes)
 {
@@ -562,9 +564,12 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
     }
     float dot = tree_dot(p, q, k, c->G, lanes);
     float e = r - dot;
+    /* p + lr (e q - lp p), evaluated as p (1 - lr lp) + (lr e) q */
+    float cp = fmaf(-lr, c->lambda_p, 1.0f), cq = fmaf(-lr, c->lambda_q, 1.0f);
+    float le = lr * e;
     for (int j = 0; j < k; ++j) {
-        float pn = fmaf(lr, fmaf(e, q[j], -(c->lambda_p * p[j])), p[j]);
-        float qn = fmaf(lr, fmaf(e, p[j], -(c->lambda_q * q[j])), q[j]);
+        float pn = fmaf(le, q[j], p[j] * cp);
+        float qn = fmaf(le, p[j], q[j] * cq);
         uint32_t rp = 0, rq = 0;
         if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
             rp = sr_bits(K, sj, 0, j);
