@@ -6,9 +6,10 @@ k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated (seeded) on the host
 and copied to HBM.  One
 step = one epoch of batch-Hogwild SGD over all training ratings (the
 reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
-by default under the stratum schedule (8 x 8 nnz-balanced blocks, the 8
-conflict-free blocks of a diagonal on the 8 XCDs, one launch per diagonal;
---schedule hogwild runs the reference's 1x1 mode).
+by default under the stratum schedule (16 x 16 nnz-balanced blocks, the 16
+conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
+the least-loaded one, one launch per diagonal; --schedule hogwild runs the
+reference's 1x1 mode).
 Inputs are resident in HBM before the timed region starts.
 
 Printed: one JSON line with value (rating updates/s of the timed epochs),
@@ -128,9 +129,9 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum"],
-                    help="stratum: s x s conflict-free blocks, block i on XCD i mod 8 (default); hogwild: the "
-                         "reference's 1x1 mode")
-    ap.add_argument("--strata", type=int, default=8, help="s for the stratum schedule (s x s blocks)")
+                    help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
+                         "(default); hogwild: the reference's 1x1 mode")
+    ap.add_argument("--strata", type=int, default=16, help="s for the stratum schedule (s x s blocks)")
     ap.add_argument("--rmse-epochs", type=int, default=20)
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)

@@ -1054,19 +1054,27 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
         }
     } else {
         // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
-        // which share no P and no Q rows; block i runs on XCD i mod num_xcd.
+        // which share no P and no Q rows.  Each block runs on one XCD; the
+        // blocks of a diagonal go to XCDs largest first onto the least-loaded
+        // one (a launch lasts as long as its most loaded XCD).
         const int s = c->ux;
         const int per_xcd = sh.W / num_xcd;
         for (int d = 0; d < s; ++d) {
             std::vector<SgdTask> t;
+            std::vector<int> order((size_t)s);
+            for (int i = 0; i < s; ++i) order[(size_t)i] = i * s + (i + d) % s;
+            auto size_of = [&](int blk) { return c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk]; };
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return size_of(a) > size_of(b); });
+            std::vector<std::vector<int>> assigned((size_t)num_xcd);
+            std::vector<int64_t> load((size_t)num_xcd, 0);
+            for (int blk : order) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                assigned[(size_t)x].push_back(blk);
+                load[(size_t)x] += size_of(blk);
+            }
             for (int x = 0; x < num_xcd; ++x) {
-                std::vector<int> mine;
-                int64_t tot = 0;
-                for (int i = x; i < s; i += num_xcd) {
-                    const int blk = i * s + (i + d) % s;
-                    mine.push_back(blk);
-                    tot += c->offsets[blk + 1] - c->offsets[blk];
-                }
+                const std::vector<int>& mine = assigned[(size_t)x];
+                const int64_t tot = load[(size_t)x];
                 int wb = x * per_xcd, left = per_xcd;
                 for (size_t j = 0; j < mine.size(); ++j) {
                     const int blk = mine[j];
