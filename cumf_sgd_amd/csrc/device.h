@@ -92,11 +92,56 @@ __device__ inline uint32_t f32_to_f16(float f)
     _Float16 x = (_Float16)f;   // v_cvt_f16_f32, round to nearest even
     return __builtin_bit_cast(uint16_t, x);
 }
+// a * b rounded to FP32, a the FP16 in half HA of word aw, b an FP32 value.
+// One v_fma_mix_f32 with a -0.0 addend: x + (-0) == x for every x, signed
+// zeros included, so this is the FP32 product of the widened operands
+// without a separate conversion.
+template <bool HA>
+__device__ inline float mul_f16_f32(uint32_t aw, float b)
+{
+    float d;
+    const float nz = -0.0f;
+    if constexpr (HA)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(aw), "v"(b), "s"(nz));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(aw), "v"(b), "s"(nz));
+    return d;
+}
+// The same for two FP16 operands, both in half H of their words.
+template <bool H>
+__device__ inline float mul_f16_f16(uint32_t aw, uint32_t bw)
+{
+    float d;
+    const float nz = -0.0f;
+    if constexpr (H)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,1,0] op_sel_hi:[1,1,0]" : "=v"(d) : "v"(aw), "v"(bw), "s"(nz));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,1,0]" : "=v"(d) : "v"(aw), "v"(bw), "s"(nz));
+    return d;
+}
+// Two FP32 values rounded to FP16 (nearest even) in one word, lo = a,
+// hi = b (v_cvt_pk_f16_f32).  The values are first rounded to FP32 by their
+// producers: the empty asm keeps the compiler from fusing a producing fma
+// into one fma-to-f16 rounding.
+__device__ inline uint32_t f32x2_to_f16x2(float a, float b)
+{
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    asm("" : "+v"(a), "+v"(b));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
+}
 __device__ inline float bf16_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
 // Round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ inline uint32_t f32_to_bf16(float f)
 {
     return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+// Two values in one word, lo = a, hi = b: one v_cvt_pk_bf16_f32.
+__device__ inline uint32_t f32x2_to_bf16x2(float a, float b)
+{
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, b2));
 }
 // Stochastic rounding with 16 random bits r16 (low half): adding them to the
 // 16 bits that truncation drops carries into the kept part with probability

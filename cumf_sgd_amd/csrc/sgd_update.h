@@ -23,9 +23,9 @@ namespace cumf {
 //
 // Arithmetic (shared with the CPU oracle): lane partial = p0*q0 then fma over
 // the lane's other elements; butterfly sum over the slot (lane l + lane l^d,
-// d = 1, 2, 4, ...); e = r - dot; p' = fma(lr, fma(e, q, -(lp*p)), p) and
-// q' = fma(lr, fma(e, p, -(lq*q)), q) with the old p; store rounds p', q' to
-// the storage format.
+// d = 1, 2, 4, ...); e = r - dot; with cp = fma(-lr, lp, 1), cq likewise and
+// le = lr*e: p' = fma(le, q, p*cp) and q' = fma(le, p, q*cq) with the old p;
+// store rounds p', q' to the storage format.
 // ---------------------------------------------------------------------------
 constexpr int kWavesPerBlock = kSgdWavesPerBlock;
 constexpr int kMaxBatch = kSgdMaxBatch;
@@ -78,6 +78,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     constexpr int NW = (TB + 3) / 4;
     constexpr bool FP8 = PREC == kE4M3;
     constexpr bool SR = FP8 || PREC == kBF16SR;   // stochastic rounding on the write-back
+    constexpr bool PAIR = (PREC == kF16 || PREC == kBF16) && E % 2 == 0;   // packed two-element write-back
+    constexpr bool MIX = PAIR && PREC == kF16;   // FP16 operands read in place (v_fma_mix_f32)
     constexpr int RPL = (kMaxBatch + G - 1) / G;   // staged records per lane
     static_assert(K % G == 0 && 64 % G == 0, "lane group must divide k and 64");
     static_assert(!FP8 || E % 4 == 0, "FP8 rows need 4 elements per lane");
@@ -176,19 +178,28 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int j = base + d;
-                Words<NW> pw = pend[d].p, qw = pend[d].q;
                 const int uc = pend[d].u, vc = pend[d].v;
                 const float rc = pend[d].r;
                 float psc = 1.0f, qsc = 1.0f;
                 if constexpr (FP8) { psc = pend[d].ps; qsc = pend[d].qs; }
                 // forward rows written after this sample's rows were requested
-                // (samples j-D+1 .. j-1), oldest first so the newest wins
+                // (samples j-D+1 .. j-1), oldest first so the newest wins; the
+                // wave skips this when none of its slots has such a row
+                bool hit = false;
 #pragma unroll
                 for (int back = D - 1; back >= 1; --back) {
                     const int s = (d - back + D) % D;
-                    if (wu[s] == uc) pw = wp[s];
-                    if (wv[s] == vc) qw = wq[s];
+                    hit |= (wu[s] == uc) | (wv[s] == vc);
                 }
+                if (__builtin_amdgcn_ballot_w64(hit)) {
+#pragma unroll
+                    for (int back = D - 1; back >= 1; --back) {
+                        const int s = (d - back + D) % D;
+                        if (wu[s] == uc) pend[d].p = wp[s];
+                        if (wv[s] == vc) pend[d].q = wq[s];
+                    }
+                }
+                const Words<NW> pw = pend[d].p, qw = pend[d].q;
 
                 float p[E], q[E];
 #pragma unroll
@@ -197,7 +208,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                     q[e] = unpack<PREC>(qw, e);
                     if constexpr (FP8) { p[e] *= psc; q[e] *= qsc; }
                 }
-                float part = p[0] * q[0];
+                float part;
+                if constexpr (MIX) part = mul_f16_f16<false>(pw.w[0], qw.w[0]);
+                else part = p[0] * q[0];
 #pragma unroll
                 for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
                 const float dot = slot_sum<G>(part);
@@ -205,6 +218,36 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 const float lerr = lr * err;
 
                 Words<NW> po, qo;
+                if constexpr (PAIR) {
+                    // two elements per word: FP16 rows are read as halves by
+                    // v_fma_mix_f32; one packed conversion writes both back
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) {
+                        float p0, p1, q0, q1;
+                        if constexpr (MIX) {
+                            p0 = mul_f16_f32<false>(pw.w[i], cp);
+                            p1 = mul_f16_f32<true>(pw.w[i], cp);
+                            q0 = mul_f16_f32<false>(qw.w[i], cq);
+                            q1 = mul_f16_f32<true>(qw.w[i], cq);
+                        } else {
+                            p0 = p[2 * i] * cp;
+                            p1 = p[2 * i + 1] * cp;
+                            q0 = q[2 * i] * cq;
+                            q1 = q[2 * i + 1] * cq;
+                        }
+                        p0 = __builtin_fmaf(lerr, q[2 * i], p0);
+                        p1 = __builtin_fmaf(lerr, q[2 * i + 1], p1);
+                        q0 = __builtin_fmaf(lerr, p[2 * i], q0);
+                        q1 = __builtin_fmaf(lerr, p[2 * i + 1], q1);
+                        if constexpr (MIX) {
+                            po.w[i] = f32x2_to_f16x2(p0, p1);
+                            qo.w[i] = f32x2_to_f16x2(q0, q1);
+                        } else {
+                            po.w[i] = f32x2_to_bf16x2(p0, p1);
+                            qo.w[i] = f32x2_to_bf16x2(q0, q1);
+                        }
+                    }
+                } else {
                 uint32_t rbits[4] = {0u, 0u, 0u, 0u};
                 const float pinv = FP8 ? 1.0f / psc : 1.0f;
                 const float qinv = FP8 ? 1.0f / qsc : 1.0f;
@@ -266,6 +309,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         pack<PREC>(po, e, pb);
                         pack<PREC>(qo, e, qb);
                     }
+                }
                 }
                 store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
                 store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
