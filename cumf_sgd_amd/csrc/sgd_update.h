@@ -2,6 +2,8 @@
 // sgd_update_<format>.hip (one translation unit each, compiled in parallel).
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace cumf {
@@ -174,7 +176,12 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
             __builtin_amdgcn_sched_barrier(0);
         }
 
-        for (int base = 0; base < L; base += D) {
+        // D samples per round.  The first and the last round are peeled: the
+        // loop then runs in one steady state (every sample stores its rows and
+        // requests those of sample j + D), so that the compiler's wait for a
+        // sample's rows stays partial, with the D - 1 younger requests and
+        // their stores still in flight.
+        auto round = [&](int base, auto more) __attribute__((always_inline)) {
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int j = base + d;
@@ -317,8 +324,20 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 wq[d] = qo;
                 wu[d] = uc;
                 wv[d] = vc;
-                if (j + D < L) request(pend[d], j + D);
+                if constexpr (decltype(more)::value) {
+                    if (FP8 ? j + D < L : true) request(pend[d], j + D);
+                }
             }
+        };
+        if constexpr (FP8) {
+            // the FP8 step is long and its registers are tight: one loop
+            for (int base = 0; base < L; base += D) round(base, std::true_type{});
+        } else {
+            if (L > D) {
+                round(0, std::true_type{});
+                for (int base = D; base < L - D; base += D) round(base, std::true_type{});
+            }
+            round(L - D, std::false_type{});
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
