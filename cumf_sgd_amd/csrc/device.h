@@ -204,6 +204,19 @@ __device__ inline uint32_t e4m3_sr_insert(float f, uint32_t rword, uint32_t old)
     return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, SEL);
 }
 
+// The same rounding into byte sel (0..3, a constant after unrolling) of a
+// value known not to be NaN, the clamp to +-448 done by one v_med3_f32.
+__device__ inline uint32_t e4m3_sr_clamped(float f, uint32_t rword, uint32_t old, int sel)
+{
+    const float v = __builtin_amdgcn_fmed3f(f, -448.0f, 448.0f);
+    switch (sel) {
+    case 0: return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, 0);
+    case 1: return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, 1);
+    case 2: return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, 2);
+    default: return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, 3);
+    }
+}
+
 // Smallest power of two s with amax <= target * s.
 __device__ inline float pow2_scale(float amax, float target)
 {
@@ -220,6 +233,7 @@ __device__ inline float pow2_scale(float amax, float target)
 // across XCDs (bypasses the per-XCD L2 copies).
 // ---------------------------------------------------------------------------
 template <int NW> struct Words { uint32_t w[NW]; };
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Global-address-space pointers: the kernels take their buffers from a task

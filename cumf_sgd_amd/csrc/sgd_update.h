@@ -208,21 +208,43 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 }
                 const Words<NW> pw = pend[d].p, qw = pend[d].q;
 
+                // FP8 rows stay in their scaled units (row value = byte value
+                // times the row's power-of-two scale): scaling by a power of
+                // two commutes with every rounding below, so the dot product
+                // takes the scales once and each update once, per sample
                 float p[E], q[E];
+                if constexpr (FP8) {
 #pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    p[e] = unpack<PREC>(pw, e);
-                    q[e] = unpack<PREC>(qw, e);
-                    if constexpr (FP8) { p[e] *= psc; q[e] *= qsc; }
+                    for (int i = 0; i < NW; ++i) {
+                        const f32x2 pl = __builtin_amdgcn_cvt_pk_f32_fp8((int)pw.w[i], false);
+                        const f32x2 ph = __builtin_amdgcn_cvt_pk_f32_fp8((int)pw.w[i], true);
+                        const f32x2 ql = __builtin_amdgcn_cvt_pk_f32_fp8((int)qw.w[i], false);
+                        const f32x2 qh = __builtin_amdgcn_cvt_pk_f32_fp8((int)qw.w[i], true);
+                        p[4 * i] = pl.x; p[4 * i + 1] = pl.y; p[4 * i + 2] = ph.x; p[4 * i + 3] = ph.y;
+                        q[4 * i] = ql.x; q[4 * i + 1] = ql.y; q[4 * i + 2] = qh.x; q[4 * i + 3] = qh.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        p[e] = unpack<PREC>(pw, e);
+                        q[e] = unpack<PREC>(qw, e);
+                    }
                 }
                 float part;
                 if constexpr (MIX) part = mul_f16_f16<false>(pw.w[0], qw.w[0]);
                 else part = p[0] * q[0];
 #pragma unroll
                 for (int e = 1; e < E; ++e) part = __builtin_fmaf(p[e], q[e], part);
-                const float dot = slot_sum<G>(part);
+                float dot = slot_sum<G>(part);
+                if constexpr (FP8) dot *= psc * qsc;
                 const float err = rc - dot;
                 const float lerr = lr * err;
+                // multiplier of q in p' and of p in q' (FP8: in the rows' units)
+                float lq_p = lerr, lp_q = lerr;
+                if constexpr (FP8) {
+                    lq_p = lerr * (qsc * (1.0f / psc));
+                    lp_q = lerr * (psc * (1.0f / qsc));
+                }
 
                 Words<NW> po, qo;
                 if constexpr (PAIR) {
@@ -254,14 +276,47 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             qo.w[i] = f32x2_to_bf16x2(q0, q1);
                         }
                     }
+                } else if constexpr (FP8) {
+                    // Element jj of the row rounds with the 16-bit half (jj & 1)
+                    // of word (jj >> 1) & 1 of lowbias32(K ^ (j*0x9E3779B9 +
+                    // (jj>>2)*0x85EBCA77)) (words 0-1 for P and 2-3 for Q) as the
+                    // top half of the instruction's word; values clamp to +-448
+                    // (one v_med3_f32).
+#pragma unroll
+                    for (int c = 0; c < NW; ++c) {
+                        const uint32_t t = (uint32_t)j * 0x9E3779B9u + (uint32_t)(ls * NW + c) * 0x85EBCA77u;
+                        const uint32_t h[4] = {lowbias32(srk.x ^ t), lowbias32(srk.y ^ t), lowbias32(srk.z ^ t),
+                                               lowbias32(srk.w ^ t)};
+                        uint32_t op = 0u, oq = 0u;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const int e = 4 * c + b;
+                            const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
+                            const float qn = __builtin_fmaf(lp_q, p[e], q[e] * cq);
+                            const uint32_t wp = h[b >> 1], wq = h[2 + (b >> 1)];
+                            const uint32_t rp = (b & 1) ? (wp & 0xffff0000u) : (wp << 16);
+                            const uint32_t rq = (b & 1) ? (wq & 0xffff0000u) : (wq << 16);
+                            op = e4m3_sr_clamped(pn, rp, op, b);
+                            oq = e4m3_sr_clamped(qn, rq, oq, b);
+                        }
+                        po.w[c] = op;
+                        qo.w[c] = oq;
+                    }
+                    // A NaN can only come from a non-finite multiplier (the rows
+                    // and the dot product are then finite); such a sample writes
+                    // NaN (0xff) to every element of both rows.
+                    if (__builtin_amdgcn_ballot_w64(!(__builtin_isfinite(lq_p) && __builtin_isfinite(lp_q)))) {
+                        if (!(__builtin_isfinite(lq_p) && __builtin_isfinite(lp_q))) {
+#pragma unroll
+                            for (int c = 0; c < NW; ++c) po.w[c] = qo.w[c] = 0xffffffffu;
+                        }
+                    }
                 } else {
                 uint32_t rbits[4] = {0u, 0u, 0u, 0u};
-                const float pinv = FP8 ? 1.0f / psc : 1.0f;
-                const float qinv = FP8 ? 1.0f / qsc : 1.0f;
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const float pn = __builtin_fmaf(lerr, q[e], p[e] * cp);
-                    const float qn = __builtin_fmaf(lerr, p[e], q[e] * cq);
+                    const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
+                    const float qn = __builtin_fmaf(lp_q, p[e], q[e] * cq);
                     uint32_t pb = 0u, qb = 0u;
                     if constexpr (PREC == kF32) {
                         pb = __float_as_uint(pn);
@@ -305,10 +360,10 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             const uint32_t rq = (jj & 1) ? (bq & 0xffff0000u) : (bq << 16);
                             const uint32_t op = (e & 3) ? po.w[e >> 2] : 0u, oq = (e & 3) ? qo.w[e >> 2] : 0u;
                             switch (e & 3) {
-                            case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn * qinv, rq, oq); break;
-                            case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn * qinv, rq, oq); break;
-                            case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn * qinv, rq, oq); break;
-                            default: po.w[e >> 2] = e4m3_sr_insert<3>(pn * pinv, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn * qinv, rq, oq); break;
+                            case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn, rq, oq); break;
+                            case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn, rq, oq); break;
+                            case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn, rq, oq); break;
+                            default: po.w[e >> 2] = e4m3_sr_insert<3>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn, rq, oq); break;
                             }
                         }
                     }

@@ -525,12 +525,13 @@ typedef struct {
     float lambda_p, lambda_q;
 } orc_sgd_cfg;
 
-/* Random bits for stochastic rounding of the FP8 (and BF16-SR) write-back of sample j
- * (0..L-1) of window `batch`: one Philox block per window,
+/* Random bits for stochastic rounding of the FP8 and BF16-SR write-back of
+ * sample j (0..L-1) of window `batch`: one Philox block per window,
  * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}), then for element
  * group c = e >> 2 the words W_i = lowbias32(K_i ^ (j*0x9E3779B9 +
  * c*0x85EBCA77)), i = 0..3; W_0 holds P[4c], P[4c+1] (low, high 16 bits),
- * W_1 P[4c+2], P[4c+3], W_2 and W_3 the same elements of Q. */
+ * W_1 P[4c+2], P[4c+3], W_2 and W_3 the same elements of Q.  Sixteen bits
+ * per element keep late-epoch updates far below one FP8 step unbiased. */
 static inline uint32_t lowbias32(uint32_t x)
 {
     x ^= x >> 16;
@@ -549,6 +550,41 @@ static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e
     return (e & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
+/* FP8 rows with power-of-two row scales sp, sq: the bytes' values are used
+ * in their own units (row value = byte value * scale) and the scales enter
+ * once per sample.  Scaling by a power of two commutes with every rounding
+ * (away from FP32 underflow and overflow), so this is the FP32 step on the
+ * widened rows: dot = (sp*sq) * tree(p8 . q8); with le = lr*e the stored
+ * values are p8' = fma(le*(sq*(1/sp)), q8, p8*cp) and
+ * q8' = fma(le*(sp*(1/sq)), p8, q8*cq), clamped to +-448 and rounded
+ * stochastically with sr_bits (16 random bits per element).
+ * A non-finite multiplier (the only source of a NaN) writes 0xff to every
+ * element of both rows. */
+static void sgd_update_one_e4m3(const orc_sgd_cfg* c, void* P, void* Q, float sp, float sq, int64_t bp,
+                                int64_t bq, float r, float lr, const uint32_t K[4], uint32_t sj, float* p,
+                                float* q, float* lanes)
+{
+    int k = c->k;
+    uint8_t* P8 = (uint8_t*)P;
+    uint8_t* Q8 = (uint8_t*)Q;
+    for (int j = 0; j < k; ++j) {
+        p[j] = orc_e4m3_to_f32(P8[bp + j]);
+        q[j] = orc_e4m3_to_f32(Q8[bq + j]);
+    }
+    float dot = tree_dot(p, q, k, c->G, lanes) * (sp * sq);
+    float e = r - dot;
+    float cp = fmaf(-lr, c->lambda_p, 1.0f), cq = fmaf(-lr, c->lambda_q, 1.0f);
+    float le = lr * e;
+    float lqp = le * (sq * (1.0f / sp)), lpq = le * (sp * (1.0f / sq));
+    int nan_rows = !(isfinite(lqp) && isfinite(lpq));
+    for (int j = 0; j < k; ++j) {
+        float pn = fmaf(lqp, q[j], p[j] * cp);
+        float qn = fmaf(lpq, p[j], q[j] * cq);
+        P8[bp + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(pn, sr_bits(K, sj, 0, j));
+        Q8[bq + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(qn, sr_bits(K, sj, 1, j));
+    }
+}
+
 /* One SGD update of (u, v, r) on FP32 math with storage rounding. */
 static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, float* qs,
                            int32_t u, int32_t v, float r, float lr, const uint32_t K[4], uint32_t sj,
@@ -556,8 +592,11 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
 {
     int k = c->k;
     int64_t bp = (int64_t)u * k, bq = (int64_t)v * k;
+    if (c->prec == PREC_E4M3) {
+        sgd_update_one_e4m3(c, P, Q, ps[u], qs[v], bp, bq, r, lr, K, sj, p, q, lanes);
+        return;
+    }
     float sp = 1.0f, sq = 1.0f, ip = 1.0f, iq = 1.0f;
-    if (c->prec == PREC_E4M3) { sp = ps[u]; sq = qs[v]; ip = 1.0f / sp; iq = 1.0f / sq; }
     for (int j = 0; j < k; ++j) {
         p[j] = load_elem(P, c->prec, bp + j) * sp;
         q[j] = load_elem(Q, c->prec, bq + j) * sq;

@@ -100,6 +100,47 @@ def test_serial_epoch_matches_oracle(gpu, k, G, prec, D):
     assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))
 
 
+@pytest.mark.parametrize("case", ["clamp", "nonfinite"])
+def test_fp8_clamp_and_nonfinite_rows_match_oracle(gpu, case):
+    """FP8 write-back at its edges, serialized, bytes against the oracle:
+    ratings scaled by 3 make most elements outgrow +-448 within the epoch
+    (clamped); every 300th rating infinite makes the update
+    multiplier non-finite, and those samples write NaN (0xff) to both rows,
+    which later samples then spread (about 65% of the elements)."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    if case == "clamp":
+        u, v, r, tu, tv, tr, m, n = _problem(nnz=20_000, n_test=400)
+        r = r * np.float32(3.0)
+    else:
+        u, v, r, tu, tv, tr, m, n = _problem(nnz=3_000, n_test=400)
+        r = r.copy()
+        r[::300] = np.inf
+    k, G, D, seed = 128, 8, 4, 99
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(k, cs.FP8_E4M3, seed=seed)
+        P0, ps0 = c.get_factors(0)
+        Q0, qs0 = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.SERIAL, seed=seed, lanes_per_sample=G, pipeline_depth=D), 0, 1)
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 4)
+    Pw, Qw, psw, qsw = oracle.sgd_serial(u, v, r, P0, Q0, lr, 0, 1, k=k, G=G, prec=3, lambda_p=0.05,
+                                         lambda_q=0.05, seed=seed, ps=ps0, qs=qs0)
+    # the epoch's own write-back, before the between-epoch rescale
+    b = np.concatenate([Pw.view(np.uint8).ravel(), Qw.view(np.uint8).ravel()])
+    Pw, psw = oracle.fp8_rescale(Pw, psw)
+    Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+    assert np.array_equal(P1.view(np.uint8), Pw.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))
+    assert np.array_equal(ps1, psw) and np.array_equal(qs1, qsw)
+    if case == "clamp":
+        assert 0.2 * b.size < np.count_nonzero((b & 0x7f) == 0x7e) < b.size   # +-448 reached
+    else:
+        assert 0.3 * b.size < np.count_nonzero((b & 0x7f) == 0x7f) < b.size         # NaN rows
+
+
 def _close(got, want, rel):
     if np.isnan(want):
         return bool(np.isnan(got))
