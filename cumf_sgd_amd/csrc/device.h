@@ -243,6 +243,13 @@ typedef __attribute__((address_space(1))) char gchar;
 typedef __attribute__((address_space(1))) const int32_t gcint;
 typedef __attribute__((address_space(1))) const float gcfloat;
 template <typename T>
+__device__ inline __attribute__((address_space(3))) void* to_lds(T* p)
+{
+    return (__attribute__((address_space(3))) void*)p;
+}
+// s_waitcnt immediate (gfx9 encoding) that waits for LDS reads only
+constexpr int kWaitLgkm0 = 0xC07F;
+template <typename T>
 __device__ inline __attribute__((address_space(1))) T* to_global(T* p)
 {
     return (__attribute__((address_space(1))) T*)p;

@@ -71,8 +71,8 @@ struct Pending {
 
 // One worker (a slot of G lanes) running its windows of task T.
 template <int K, int G, int PREC, bool COH, int D>
-__device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int ls, int32_t* su,
-                                           int32_t* sv, float* sr)
+__device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int slot, int ls,
+                                           int32_t* su, int32_t* sv, float* sr)
 {
     constexpr int E = K / G;
     constexpr int EB = PrecBytes<PREC>::value;
@@ -101,41 +101,39 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     constexpr int64_t kRow = (int64_t)K * EB;
     const int64_t nnz = T.nnz;
 
-    // records of one window, j = ls + G*c, held in registers until staged
-    int32_t ru[RPL], rv[RPL];
-    float rr[RPL];
+    // The records of one window go from memory straight into LDS (global_load_lds,
+    // no registers): su/sv/sr are the wave's arrays, and load c of lane
+    // (slot, ls) fetches record j = ls + G*c of its slot into word 64*c + lane,
+    // so record j of a slot sits at (j / G) * 64 + slot * G + j % G.
     auto fetch_window = [&](int ep, int b) {
         const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)b, nnz);
+        // reads of the previous window's records have returned before they
+        // are overwritten
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
 #pragma unroll
         for (int c = 0; c < RPL; ++c) {
             const int j = ls + G * c;
             if (j < L) {
                 int64_t idx = start + j;
                 while (idx >= nnz) idx -= nnz;
-                ru[c] = Ru[idx];
-                rv[c] = Rv[idx];
-                rr[c] = Rr[idx];
+                __builtin_amdgcn_global_load_lds(Ru + idx, to_lds(su + 64 * c), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(Rv + idx, to_lds(sv + 64 * c), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(Rr + idx, to_lds(sr + 64 * c), 4, 0, 0);
             }
         }
     };
     auto stage_window = [&]() {
-#pragma unroll
-        for (int c = 0; c < RPL; ++c) {
-            const int j = ls + G * c;
-            if (j < L) {
-                su[j] = ru[c];
-                sv[j] = rv[c];
-                sr[j] = rr[c];
-            }
-        }
+        __builtin_amdgcn_s_waitcnt(0);   // the window's records are in LDS
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    const int rbase = slot * G;
     auto request = [&](Pending<NW>& pd, int j) {
-        pd.u = su[j];
-        pd.v = sv[j];
-        pd.r = sr[j];
+        const int at = (j / G) * 64 + rbase + j % G;
+        pd.u = su[at];
+        pd.v = sv[at];
+        pd.r = sr[at];
         pd.p = load_raw<TB, COH>(Pb + (int64_t)pd.u * kRow);
         pd.q = load_raw<TB, COH>(Qb + (int64_t)pd.v * kRow);
         if constexpr (FP8) {
@@ -147,6 +145,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     const int ep0 = a.epoch_dev ? *to_global(a.epoch_dev) : a.epoch_begin;
     const int nwin = (a.epoch_end - a.epoch_begin) * T.update_count;
     if (nwin <= 0) return;
+    // the task's pointers are in registers before the first window's loads
+    // (else each guarded load below waits for them again)
+    __builtin_amdgcn_s_waitcnt(0);
     fetch_window(ep0, 0);
 
     for (int w = 0; w < nwin; ++w) {
@@ -160,7 +161,10 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         U4 srk = U4{0u, 0u, 0u, 0u};
         if constexpr (SR) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
         stage_window();
-        if (w + 1 < nwin) fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
+        const bool more_windows = w + 1 < nwin;
+        auto fetch_next = [&]() {
+            if (more_windows) fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
+        };
 
         Pending<NW> pend[D];
         // ring of the last D written rows (slot d holds the sample j with j % D == d)
@@ -387,11 +391,15 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         if constexpr (FP8) {
             // the FP8 step is long and its registers are tight: one loop
             for (int base = 0; base < L; base += D) round(base, std::true_type{});
+            fetch_next();
         } else {
             if (L > D) {
                 round(0, std::true_type{});
                 for (int base = D; base < L - D; base += D) round(base, std::true_type{});
             }
+            // every record of the window has been read: the next window's
+            // records load during the last round
+            fetch_next();
             round(L - D, std::false_type{});
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -404,9 +412,9 @@ template <int K, int G, int PREC, bool COH, int D>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
 {
     constexpr int S = 64 / G;
-    __shared__ int32_t s_u[kWavesPerBlock][S][kMaxBatch];
-    __shared__ int32_t s_v[kWavesPerBlock][S][kMaxBatch];
-    __shared__ float s_r[kWavesPerBlock][S][kMaxBatch];
+    __shared__ int32_t s_u[kWavesPerBlock][S * kMaxBatch];
+    __shared__ int32_t s_v[kWavesPerBlock][S * kMaxBatch];
+    __shared__ float s_r[kWavesPerBlock][S * kMaxBatch];
 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -421,8 +429,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
     while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
     if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
     const SgdTask T = a.tasks[t];
-    sgd_worker<K, G, PREC, COH, D>(a, T, (uint32_t)(worker - T.worker_begin), ls, s_u[wave][slot], s_v[wave][slot],
-                                   s_r[wave][slot]);
+    sgd_worker<K, G, PREC, COH, D>(a, T, (uint32_t)(worker - T.worker_begin), slot, ls, s_u[wave], s_v[wave],
+                                   s_r[wave]);
 }
 
 // Lane group of a factor length in batched launches (one per k).
@@ -463,8 +471,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLau
         const int slot = lane / G, ls = lane % G;                                                        \
         const uint32_t lw = (uint32_t)(wl * (64 / G) + slot);                                            \
         if (lw >= (uint32_t)T.worker_count) return;                                                      \
-        sgd_worker<KK, G, PREC, COH, D>(a, T, lw, ls, &s_u[wave][slot * kMaxBatch], &s_v[wave][slot * kMaxBatch], \
-                                        &s_r[wave][slot * kMaxBatch]);                                   \
+        sgd_worker<KK, G, PREC, COH, D>(a, T, lw, slot, ls, s_u[wave], s_v[wave], s_r[wave]);           \
         return;                                                                                          \
     }
     switch (T.k) {
