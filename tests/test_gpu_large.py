@@ -53,7 +53,7 @@ def test_large_sparse_formats(gpu):
         del R
         _free()
         scale, b = _prep(c, 3, 1_000_990, 624_961, c.nnz)
-        off = c.partition_strata(8)
+        off = c.partition_strata(16)
         assert off[-1] == c.nnz and np.all(np.diff(off) >= 0)
         lam = float(np.float32(0.05) / scale)
         for prec in (0, 2, 3):
@@ -105,7 +105,7 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
         _free()
         print(f"generated {nnz} ratings in {time.time() - t0:.1f}s")
         scale, b = _prep(c, 5, m, n, nnz)
-        off = c.partition_strata(8)
+        off = c.partition_strata(16)
         assert off[-1] == nnz and off[0] == 0 and np.all(np.diff(off) >= 0)
         c.init_factors(128, cs.FP16, 5)
         o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=float(np.float32(0.05) / scale), seed=5, alpha=0.002,
