@@ -188,24 +188,14 @@ __device__ inline uint32_t f32_to_e4m3_rne(float f)
     r = r > 0x43e00000u ? 0x43e00000u : r;
     return sign | e4m3_pack_normal(r);
 }
-// Stochastic rounding into byte SEL of `old`, with v_cvt_sr_fp8_f32.  The
-// instruction adds the top 20 bits of its random word into the 20-bit
-// fraction of the dropped part (in units of the E4M3 quantum, 2^(e-3) for
-// normal values, 2^-9 for subnormal ones) and truncates; rword carries 16
-// random bits in its top half, so |f| rounds up when
-// floor(frac * 2^20) + (rword >> 12) >= 2^20.  The instruction turns values
-// above 448 into NaN, so |f| is clamped to 448 first (NaN stays NaN, and
-// becomes 0xff).
-template <int SEL>
-__device__ inline uint32_t e4m3_sr_insert(float f, uint32_t rword, uint32_t old)
-{
-    const float c = __builtin_fminf(__builtin_fmaxf(f, -448.0f), 448.0f);
-    const float v = __builtin_isnan(f) ? f : c;
-    return (uint32_t)__builtin_amdgcn_cvt_sr_fp8_f32(v, (int)rword, (int)old, SEL);
-}
-
-// The same rounding into byte sel (0..3, a constant after unrolling) of a
-// value known not to be NaN, the clamp to +-448 done by one v_med3_f32.
+// Stochastic rounding into byte sel (0..3, a constant after unrolling) of
+// `old`, with v_cvt_sr_fp8_f32.  The instruction adds the top 20 bits of its
+// random word into the 20-bit fraction of the dropped part (in units of the
+// E4M3 quantum, 2^(e-3) for normal values, 2^-9 for subnormal ones) and
+// truncates; rword carries 16 random bits in its top half, so |f| rounds up
+// when floor(frac * 2^20) + (rword >> 12) >= 2^20.  The instruction turns
+// values above 448 into NaN, so f is clamped to +-448 first, by one
+// v_med3_f32 (the caller keeps NaN away: a median would make it a bound).
 __device__ inline uint32_t e4m3_sr_clamped(float f, uint32_t rword, uint32_t old, int sel)
 {
     const float v = __builtin_amdgcn_fmed3f(f, -448.0f, 448.0f);
@@ -343,9 +333,7 @@ __device__ inline float unpack(const Words<NW>& r, int i)
     }
 }
 
-// Stochastic-rounding bits: one Philox block per window (key material
-// K0..K3), then word i of element group c of sample j in the window is
-// lowbias32(K_i ^ (j * 0x9E3779B9 + c * 0x85EBCA77)), a bijective 32-bit mixer.
+// lowbias32 (C. Wellons' hash prospector), a bijective 32-bit mixer.
 __host__ __device__ inline uint32_t lowbias32(uint32_t x)
 {
     x ^= x >> 16;
@@ -354,6 +342,25 @@ __host__ __device__ inline uint32_t lowbias32(uint32_t x)
     x *= 0x846ca68bu;
     x ^= x >> 16;
     return x;
+}
+
+// Random words for stochastic rounding: lane l of the worker's lane group
+// draws, for sample j of the window, the seed lowbias32(K ^ (j*0x9E3779B9 +
+// l*0x85EBCA77)) (K: Philox word 0 of the window for P, word 2 for Q) and
+// steps it with sr_next for its further words: one multiply-based hash per
+// row and lane, then three shift-xors and an add per word.
+__host__ __device__ inline uint32_t sr_lane_seed(uint32_t key, uint32_t j, uint32_t lane)
+{
+    return lowbias32(key ^ (j * 0x9E3779B9u + lane * 0x85EBCA77u));
+}
+// xorshift32 (Marsaglia, 2003) plus a Weyl increment, which also keeps the
+// sequence off the all-zero fixed point.
+__host__ __device__ inline uint32_t sr_next(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x + 0x6D2B79F5u;
 }
 
 template <int PREC, int NW>

@@ -281,23 +281,24 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         }
                     }
                 } else if constexpr (FP8) {
-                    // Element jj of the row rounds with the 16-bit half (jj & 1)
-                    // of word (jj >> 1) & 1 of lowbias32(K ^ (j*0x9E3779B9 +
-                    // (jj>>2)*0x85EBCA77)) (words 0-1 for P and 2-3 for Q) as the
-                    // top half of the instruction's word; values clamp to +-448
-                    // (one v_med3_f32).
+                    // Element e of the lane rounds with the 16-bit half (e & 1)
+                    // of word e >> 1 of the lane's random words (sr_lane_seed,
+                    // sr_next) as the top half of the instruction's word; values
+                    // clamp to +-448 (one v_med3_f32).
+                    uint32_t wp = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
+                    uint32_t wq = sr_lane_seed(srk.z, (uint32_t)j, (uint32_t)ls);
 #pragma unroll
                     for (int c = 0; c < NW; ++c) {
-                        const uint32_t t = (uint32_t)j * 0x9E3779B9u + (uint32_t)(ls * NW + c) * 0x85EBCA77u;
-                        const uint32_t h[4] = {lowbias32(srk.x ^ t), lowbias32(srk.y ^ t), lowbias32(srk.z ^ t),
-                                               lowbias32(srk.w ^ t)};
                         uint32_t op = 0u, oq = 0u;
 #pragma unroll
                         for (int b = 0; b < 4; ++b) {
                             const int e = 4 * c + b;
+                            if (e >= 2 && (e & 1) == 0) {
+                                wp = sr_next(wp);
+                                wq = sr_next(wq);
+                            }
                             const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
                             const float qn = __builtin_fmaf(lp_q, p[e], q[e] * cq);
-                            const uint32_t wp = h[b >> 1], wq = h[2 + (b >> 1)];
                             const uint32_t rp = (b & 1) ? (wp & 0xffff0000u) : (wp << 16);
                             const uint32_t rq = (b & 1) ? (wq & 0xffff0000u) : (wq << 16);
                             op = e4m3_sr_clamped(pn, rp, op, b);
@@ -316,7 +317,11 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         }
                     }
                 } else {
-                uint32_t rbits[4] = {0u, 0u, 0u, 0u};
+                uint32_t wp = 0u, wq = 0u;   // BF16-SR: the lane's random words, as for FP8
+                if constexpr (SR) {
+                    wp = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
+                    wq = sr_lane_seed(srk.z, (uint32_t)j, (uint32_t)ls);
+                }
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
@@ -332,20 +337,15 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         pb = f32_to_bf16(pn);
                         qb = f32_to_bf16(qn);
                     } else {
-                        // element jj (of the row) takes the 16-bit half (jj & 1) of word
-                        // (jj >> 1) & 1 of lowbias32(K ^ (j*0x9E3779B9 + (jj>>2)*0x85EBCA77)),
-                        // words 0-1 for P and 2-3 for Q
-                        const int jj = ls * E + e;
-                        if ((e & 3) == 0) {
-                            const uint32_t t = (uint32_t)j * 0x9E3779B9u + (uint32_t)(jj >> 2) * 0x85EBCA77u;
-                            rbits[0] = lowbias32(srk.x ^ t);
-                            rbits[1] = lowbias32(srk.y ^ t);
-                            rbits[2] = lowbias32(srk.z ^ t);
-                            rbits[3] = lowbias32(srk.w ^ t);
+                        // element e of the lane takes the 16-bit half (e & 1) of
+                        // the lane's random word e >> 1
+                        if (e >= 2 && (e & 1) == 0) {
+                            wp = sr_next(wp);
+                            wq = sr_next(wq);
                         }
-                        const uint32_t bp = rbits[(jj >> 1) & 1], bq = rbits[2 | ((jj >> 1) & 1)];
+                        const uint32_t bp = wp, bq = wq;
                         if constexpr (PREC == kBF16SR && E % 2 == 0) {
-                            // (jj & 1) == (e & 1): the instruction writes that half directly
+                            // the instruction writes half (e & 1) of the word directly
                             const uint32_t rp = (e & 1) ? (bp & 0xffff0000u) : (bp << 16);
                             const uint32_t rq = (e & 1) ? (bq & 0xffff0000u) : (bq << 16);
                             const uint32_t op = (e & 1) ? po.w[e >> 1] : 0u, oq = (e & 1) ? qo.w[e >> 1] : 0u;
@@ -356,22 +356,13 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                                 po.w[e >> 1] = bf16_sr_insert<false>(pn, rp, op);
                                 qo.w[e >> 1] = bf16_sr_insert<false>(qn, rq, oq);
                             }
-                        } else if constexpr (PREC == kBF16SR) {
-                            pb = f32_to_bf16_sr(pn, (jj & 1) ? (bp >> 16) : bp);
-                            qb = f32_to_bf16_sr(qn, (jj & 1) ? (bq >> 16) : bq);
                         } else {
-                            const uint32_t rp = (jj & 1) ? (bp & 0xffff0000u) : (bp << 16);
-                            const uint32_t rq = (jj & 1) ? (bq & 0xffff0000u) : (bq << 16);
-                            const uint32_t op = (e & 3) ? po.w[e >> 2] : 0u, oq = (e & 3) ? qo.w[e >> 2] : 0u;
-                            switch (e & 3) {
-                            case 0: po.w[e >> 2] = e4m3_sr_insert<0>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<0>(qn, rq, oq); break;
-                            case 1: po.w[e >> 2] = e4m3_sr_insert<1>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<1>(qn, rq, oq); break;
-                            case 2: po.w[e >> 2] = e4m3_sr_insert<2>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<2>(qn, rq, oq); break;
-                            default: po.w[e >> 2] = e4m3_sr_insert<3>(pn, rp, op); qo.w[e >> 2] = e4m3_sr_insert<3>(qn, rq, oq); break;
-                            }
+                            static_assert(PREC == kBF16SR, "FP8 has its own write-back above");
+                            pb = f32_to_bf16_sr(pn, (e & 1) ? (bp >> 16) : bp);
+                            qb = f32_to_bf16_sr(qn, (e & 1) ? (bq >> 16) : bq);
                         }
                     }
-                    if constexpr (!FP8 && !(PREC == kBF16SR && E % 2 == 0)) {
+                    if constexpr (!(PREC == kBF16SR && E % 2 == 0)) {
                         pack<PREC>(po, e, pb);
                         pack<PREC>(qo, e, qb);
                     }

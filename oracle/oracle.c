@@ -527,11 +527,12 @@ typedef struct {
 
 /* Random bits for stochastic rounding of the FP8 and BF16-SR write-back of
  * sample j (0..L-1) of window `batch`: one Philox block per window,
- * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}), then for element
- * group c = e >> 2 the words W_i = lowbias32(K_i ^ (j*0x9E3779B9 +
- * c*0x85EBCA77)), i = 0..3; W_0 holds P[4c], P[4c+1] (low, high 16 bits),
- * W_1 P[4c+2], P[4c+3], W_2 and W_3 the same elements of Q.  Sixteen bits
- * per element keep late-epoch updates far below one FP8 step unbiased. */
+ * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}).  Lane l of the lane
+ * group (elements [lE, lE+E), E = k/G) draws the seed
+ * W_0 = lowbias32(K[0] ^ (j*0x9E3779B9 + l*0x85EBCA77)) for its P elements
+ * (K[2] for Q) and W_{i+1} = sr_next(W_i); its element i uses the 16-bit half
+ * (i & 1) of W_{i>>1}.  Sixteen bits per element keep late-epoch updates far
+ * below one FP8 step unbiased. */
 static inline uint32_t lowbias32(uint32_t x)
 {
     x ^= x >> 16;
@@ -542,12 +543,22 @@ static inline uint32_t lowbias32(uint32_t x)
     return x;
 }
 
-static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e)
+/* xorshift32 (Marsaglia 2003) plus a Weyl increment. */
+static inline uint32_t sr_next(uint32_t x)
 {
-    uint32_t t = j * 0x9E3779B9u + (uint32_t)(e >> 2) * 0x85EBCA77u;
-    int i = (which << 1) | ((e >> 1) & 1);
-    uint32_t w = lowbias32(K[i] ^ t);
-    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x + 0x6D2B79F5u;
+}
+
+/* The 16 random bits of element e (0..k-1) of row `which` (0 P, 1 Q). */
+static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e, int k, int G)
+{
+    int E = k / G, lane = e / E, i = e % E;
+    uint32_t w = lowbias32(K[which ? 2 : 0] ^ (j * 0x9E3779B9u + (uint32_t)lane * 0x85EBCA77u));
+    for (int s = 0; s < (i >> 1); ++s) w = sr_next(w);
+    return (i & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
 /* FP8 rows with power-of-two row scales sp, sq: the bytes' values are used
@@ -580,8 +591,8 @@ static void sgd_update_one_e4m3(const orc_sgd_cfg* c, void* P, void* Q, float sp
     for (int j = 0; j < k; ++j) {
         float pn = fmaf(lqp, q[j], p[j] * cp);
         float qn = fmaf(lpq, p[j], q[j] * cq);
-        P8[bp + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(pn, sr_bits(K, sj, 0, j));
-        Q8[bq + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(qn, sr_bits(K, sj, 1, j));
+        P8[bp + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(pn, sr_bits(K, sj, 0, j, k, c->G));
+        Q8[bq + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(qn, sr_bits(K, sj, 1, j, k, c->G));
     }
 }
 
@@ -611,8 +622,8 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
         float qn = fmaf(le, p[j], q[j] * cq);
         uint32_t rp = 0, rq = 0;
         if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
-            rp = sr_bits(K, sj, 0, j);
-            rq = sr_bits(K, sj, 1, j);
+            rp = sr_bits(K, sj, 0, j, k, c->G);
+            rq = sr_bits(K, sj, 1, j, k, c->G);
         }
         store_elem(P, c->prec, bp + j, pn, ip, sp, rp);
         store_elem(Q, c->prec, bq + j, qn, iq, sq, rq);
