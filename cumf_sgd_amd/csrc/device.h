@@ -207,14 +207,17 @@ __device__ inline uint32_t e4m3_sr_clamped(float f, uint32_t rword, uint32_t old
     }
 }
 
-// Smallest power of two s with amax <= target * s.
+// Smallest power of two s with amax <= target * s, kept in [2^-126, 2^126]:
+// s and 1/s are then normal, and v_rcp_f32 returns 1/s exactly (an in-bounds
+// probe compared it with 1.0f / s on every power of two).
 __device__ inline float pow2_scale(float amax, float target)
 {
     if (!(amax > 0.0f) || __builtin_isinf(amax)) return 1.0f;
     int ea, et;
     float fa = frexpf(amax, &ea);
     float ft = frexpf(target, &et);
-    return ldexpf(1.0f, ea - et + (fa > ft ? 1 : 0));
+    const int n = ea - et + (fa > ft ? 1 : 0);
+    return ldexpf(1.0f, n < -126 ? -126 : (n > 126 ? 126 : n));
 }
 
 // ---------------------------------------------------------------------------

@@ -246,8 +246,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 // multiplier of q in p' and of p in q' (FP8: in the rows' units)
                 float lq_p = lerr, lp_q = lerr;
                 if constexpr (FP8) {
-                    lq_p = lerr * (qsc * (1.0f / psc));
-                    lp_q = lerr * (psc * (1.0f / qsc));
+                    // the scales are normal powers of two: v_rcp_f32 is exact
+                    lq_p = lerr * (qsc * __builtin_amdgcn_rcpf(psc));
+                    lp_q = lerr * (psc * __builtin_amdgcn_rcpf(qsc));
                 }
 
                 Words<NW> po, qo;

@@ -288,14 +288,17 @@ static inline float store_elem(void* base, int prec, int64_t i, float x, float i
 
 /* Smallest power of two s with amax <= target * s (s = 1 when amax is 0 or
  * not finite).  With amax = fa*2^ea and target = ft*2^et (fa, ft in
- * [0.5, 1)), that is s = 2^(ea - et + (fa > ft)). */
+ * [0.5, 1)), that is s = 2^(ea - et + (fa > ft)), exponent clamped to
+ * [-126, 126]. */
 EXPORT float orc_pow2_scale(float amax, float target)
 {
     if (!(amax > 0.0f) || isinf(amax)) return 1.0f;
     int ea, et;
     float fa = frexpf(amax, &ea);
     float ft = frexpf(target, &et);
-    return ldexpf(1.0f, ea - et + (fa > ft ? 1 : 0));
+    int n = ea - et + (fa > ft ? 1 : 0);
+    /* kept in [2^-126, 2^126], so that s and 1/s are normal */
+    return ldexpf(1.0f, n < -126 ? -126 : (n > 126 ? 126 : n));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -157,6 +157,15 @@ def test_pow2_scale_is_smallest_power_of_two(amax, target):
     assert np.float32(amax) > np.float32(target) * (s / 2)
 
 
+def test_pow2_scale_stays_normal():
+    """Degenerate rows (|row| near 0) get 2^-126, never a subnormal or zero
+    scale; the largest finite amax stays below the 2^126 cap."""
+    assert oracle.pow2_scale(1e-38, 56.0) == 2.0 ** -126
+    assert oracle.pow2_scale(float(np.finfo(np.float32).tiny) / 8, 14.0) == 2.0 ** -126
+    s = oracle.pow2_scale(float(np.finfo(np.float32).max), 14.0)
+    assert s == 2.0 ** 125 and np.isfinite(np.float32(1.0) / np.float32(s))
+
+
 # ---------------------------------------------------------------- schedule
 def test_lr_table_known_values():
     # alpha=0.08, beta=0.3: 0.08, 0.0615385, 0.0432777, 0.0312641, 0.0235294
