@@ -959,7 +959,9 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     sh->L = o->batch_len ? o->batch_len : 128;
     if (sh->L < 1 || sh->L > kSgdMaxBatch) return fail(CUMF_ERR_ARG, "batch_len=%d out of range 1..%d", sh->L, kSgdMaxBatch);
     if (sh->L % 2) return fail(CUMF_ERR_ARG, "batch_len=%d must be even", sh->L);
-    int D = o->pipeline_depth ? o->pipeline_depth : 4;
+    // two row pairs in flight per worker measured fastest or equal for every
+    // format (fewer registers, three waves per SIMD)
+    int D = o->pipeline_depth ? o->pipeline_depth : 2;
     if (D != 2 && D != 4 && D != 8) return fail(CUMF_ERR_ARG, "pipeline_depth=%d not 2, 4 or 8", D);
     while (sh->L % D) D >>= 1;   // the depth must divide the window length
     sh->D = D;

@@ -375,25 +375,17 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 wq[d] = qo;
                 wu[d] = uc;
                 wv[d] = vc;
-                if constexpr (decltype(more)::value) {
-                    if (FP8 ? j + D < L : true) request(pend[d], j + D);
-                }
+                if constexpr (decltype(more)::value) request(pend[d], j + D);
             }
         };
-        if constexpr (FP8) {
-            // the FP8 step is long and its registers are tight: one loop
-            for (int base = 0; base < L; base += D) round(base, std::true_type{});
-            fetch_next();
-        } else {
-            if (L > D) {
-                round(0, std::true_type{});
-                for (int base = D; base < L - D; base += D) round(base, std::true_type{});
-            }
-            // every record of the window has been read: the next window's
-            // records load during the last round
-            fetch_next();
-            round(L - D, std::false_type{});
+        if (L > D) {
+            round(0, std::true_type{});
+            for (int base = D; base < L - D; base += D) round(base, std::true_type{});
         }
+        // every record of the window has been read: the next window's
+        // records load during the last round
+        fetch_next();
+        round(L - D, std::false_type{});
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

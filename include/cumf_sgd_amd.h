@@ -68,7 +68,7 @@ typedef struct cumf_train_opts {
     int32_t schedule;          /* enum cumf_schedule */
     int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane */
     int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
-    int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 4 */
+    int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 2 */
     int32_t use_graph;         /* 1: run the epoch loop as a captured HIP graph, replayed per epoch */
     float lambda_p;            /* already divided by the rating scale (sgd.cu:673) */
     float lambda_q;
