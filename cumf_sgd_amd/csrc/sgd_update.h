@@ -14,11 +14,11 @@ namespace cumf {
 // A wave of 64 lanes holds S = 64/G workers ("slots"); the G lanes of a slot
 // share one rating update, lane l of the slot owning the E = K/G consecutive
 // elements [lE, lE+E) of the P and Q rows, so a row is one coalesced
-// K*bytes gather.  Per window, a slot stages its L records (u, v, r) in LDS
-// (the next window's records are loaded into registers while the current one
-// runs), then walks them in order as a D-deep software pipeline: the rows of
-// sample j + D are requested right after sample j is written back, so D row
-// pairs are in flight per slot.  A prefetched row may predate the write-back
+// K*bytes gather.  Per window, a slot's L records (u, v, r) go from memory
+// straight into LDS (global_load_lds; the next window's load during the
+// current window's last round), and the slot walks them in order as a D-deep
+// software pipeline: the rows of sample j + D are requested right after
+// sample j is written back, so D row pairs are in flight per slot.  A prefetched row may predate the write-back
 // of one of the D - 1 samples before it; those written rows are kept in a
 // register ring and forwarded when the ids match, so one slot is exactly a
 // serial SGD over its window sequence.
@@ -27,7 +27,9 @@ namespace cumf {
 // the lane's other elements; butterfly sum over the slot (lane l + lane l^d,
 // d = 1, 2, 4, ...); e = r - dot; with cp = fma(-lr, lp, 1), cq likewise and
 // le = lr*e: p' = fma(le, q, p*cp) and q' = fma(le, p, q*cq) with the old p;
-// store rounds p', q' to the storage format.
+// store rounds p', q' to the storage format.  FP8 rows are used in their own
+// units, their power-of-two scales entering once per sample (see the FP8
+// write-back below).
 // ---------------------------------------------------------------------------
 constexpr int kWavesPerBlock = kSgdWavesPerBlock;
 constexpr int kMaxBatch = kSgdMaxBatch;
