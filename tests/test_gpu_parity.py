@@ -62,7 +62,8 @@ def test_serial_epoch_matches_oracle(gpu, k, G, prec, D):
     """Serialized single-worker epochs: GPU factors == CPU FP32 oracle factors.
 
     The oracle restates the kernel's lane-group dot order, so the result is
-    expected bit-exact; the north-star bar (1e-5 relative) is asserted too."""
+    expected bit-exact; the looser bar of the serialized mode against the
+    reference's CPU SGD (1e-5 relative after one epoch) is asserted too."""
     import cumf_sgd_amd as cs
     import oracle
 
