@@ -263,12 +263,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3", 4: "bf16 (stochastic rounding)"}[prec]
-                 + " storage / f32 math",
+        # the update's arithmetic; the factor storage format is in config
+        "dtype": "f32",
         "data": "synthetic (seeded, generated on the host, resident in HBM)",
         "config": {"workload": "medium Netflix-shaped MF: 480189 x 17770, 99072112 train + 1408395 test, k=128, "
                                "lambda 0.05, alpha 0.08, beta 0.3, batch-Hogwild",
-                   "precision": args.precision, "schedule": sched_tag, "workers": info.num_workers,
+                   "precision": args.precision,
+                   "factor_storage": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3 + f32 row scale",
+                                      4: "bf16 (stochastic rounding)"}[prec],
+                   "schedule": sched_tag, "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
