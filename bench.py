@@ -191,14 +191,15 @@ def main() -> int:
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
             f"batch_len={opts.batch_len} workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
     tuned = (opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth, opts.batch_len)
-    for ep in range(args.warmup):
-        ctx.train(opts, ep, ep + 1)
+    # the epochs of each phase in one call, as the reference's training loop
+    # runs all its iterations inside sgd_update_k128 (singleGPU/sgd_kernel.cu:212-248)
+    if args.warmup:
+        ctx.train(opts, 0, args.warmup)
     ctx.sync()
     torch.cuda.synchronize()
     ctx.reset_kernel_time()
     t0 = time.perf_counter()
-    for ep in range(args.warmup, args.warmup + args.steps):
-        ctx.train(opts, ep, ep + 1)
+    ctx.train(opts, args.warmup, args.warmup + args.steps)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
