@@ -241,8 +241,17 @@ def main() -> int:
             g = json.load(open(gold))
             ref64 = g["fp64"][args.rmse_epochs - 1]
             ttt["fp64_oracle_rmse"] = round(ref64, 5)
-            for p in curves:
-                ttt[PREC_NAME[p]]["gap_to_fp64_oracle"] = round(curves[p][1][-1] - ref64, 5)
+            # the same rule anchored on the serialized FP32 oracle at the same
+            # epoch count instead of the FP32 GPU run
+            otarget = g["fp32"][args.rmse_epochs - 1] + 0.005
+            ttt["oracle_target_rmse"] = round(otarget, 5)
+            ttt["oracle_target_rule"] = f"serialized FP32 CPU oracle after {args.rmse_epochs} epochs + 0.005"
+            for p, (times, rmses) in curves.items():
+                ttt[PREC_NAME[p]]["gap_to_fp64_oracle"] = round(rmses[-1] - ref64, 5)
+                hit = next((i for i, x in enumerate(rmses) if x <= otarget), None)
+                ttt[PREC_NAME[p]]["epochs_to_oracle_target"] = None if hit is None else hit + 1
+                ttt[PREC_NAME[p]]["seconds_to_oracle_target"] = (None if hit is None
+                                                                 else round(float(sum(times[:hit + 1])), 4))
 
     # ---- CPU baseline (rank 0, N=1 only)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -349,9 +349,10 @@ __host__ __device__ inline uint32_t lowbias32(uint32_t x)
 
 // Random words for stochastic rounding: lane l of the worker's lane group
 // draws, for sample j of the window, the seed lowbias32(K ^ (j*0x9E3779B9 +
-// l*0x85EBCA77)) (K: Philox word 0 of the window for P, word 2 for Q) and
-// steps it with sr_next for its further words: one multiply-based hash per
-// row and lane, then three shift-xors and an add per word.
+// l*0x85EBCA77)) (K: Philox word 0 of the window) and steps it with sr_next
+// for its further words: one multiply-based hash per sample and lane, then
+// three shift-xors and an add per word.  Each word serves two elements of the
+// P row and, with its halves swapped, the same two elements of the Q row.
 __host__ __device__ inline uint32_t sr_lane_seed(uint32_t key, uint32_t j, uint32_t lane)
 {
     return lowbias32(key ^ (j * 0x9E3779B9u + lane * 0x85EBCA77u));

@@ -284,26 +284,24 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         }
                     }
                 } else if constexpr (FP8) {
-                    // Element e of the lane rounds with the 16-bit half (e & 1)
-                    // of word e >> 1 of the lane's random words (sr_lane_seed,
-                    // sr_next) as the top half of the instruction's word; values
-                    // clamp to +-448 (one v_med3_f32).
-                    uint32_t wp = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
-                    uint32_t wq = sr_lane_seed(srk.z, (uint32_t)j, (uint32_t)ls);
+                    // Element e of the lane's P row rounds with the 16-bit half
+                    // (e & 1) of word e >> 1 of the lane's random words
+                    // (sr_lane_seed, sr_next), element e of its Q row with the
+                    // other half, each as the top half of the instruction's
+                    // word; values clamp to +-448 (one v_med3_f32).
+                    uint32_t w = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
 #pragma unroll
                     for (int c = 0; c < NW; ++c) {
                         uint32_t op = 0u, oq = 0u;
 #pragma unroll
                         for (int b = 0; b < 4; ++b) {
                             const int e = 4 * c + b;
-                            if (e >= 2 && (e & 1) == 0) {
-                                wp = sr_next(wp);
-                                wq = sr_next(wq);
-                            }
+                            if (e >= 2 && (e & 1) == 0) w = sr_next(w);
                             const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
                             const float qn = __builtin_fmaf(lp_q, p[e], q[e] * cq);
-                            const uint32_t rp = (b & 1) ? (wp & 0xffff0000u) : (wp << 16);
-                            const uint32_t rq = (b & 1) ? (wq & 0xffff0000u) : (wq << 16);
+                            const uint32_t lo = w << 16, hi = w & 0xffff0000u;
+                            const uint32_t rp = (b & 1) ? hi : lo;
+                            const uint32_t rq = (b & 1) ? lo : hi;
                             op = e4m3_sr_clamped(pn, rp, op, b);
                             oq = e4m3_sr_clamped(qn, rq, oq, b);
                         }
@@ -320,11 +318,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         }
                     }
                 } else {
-                uint32_t wp = 0u, wq = 0u;   // BF16-SR: the lane's random words, as for FP8
-                if constexpr (SR) {
-                    wp = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
-                    wq = sr_lane_seed(srk.z, (uint32_t)j, (uint32_t)ls);
-                }
+                uint32_t wr = 0u;   // BF16-SR: the lane's random words, shared by P and Q as for FP8
+                if constexpr (SR) wr = sr_lane_seed(srk.x, (uint32_t)j, (uint32_t)ls);
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const float pn = __builtin_fmaf(lq_p, q[e], p[e] * cp);
@@ -340,17 +335,15 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         pb = f32_to_bf16(pn);
                         qb = f32_to_bf16(qn);
                     } else {
-                        // element e of the lane takes the 16-bit half (e & 1) of
-                        // the lane's random word e >> 1
-                        if (e >= 2 && (e & 1) == 0) {
-                            wp = sr_next(wp);
-                            wq = sr_next(wq);
-                        }
-                        const uint32_t bp = wp, bq = wq;
+                        // element e of the lane's P row takes the 16-bit half
+                        // (e & 1) of the lane's random word e >> 1, element e of
+                        // its Q row the other half
+                        if (e >= 2 && (e & 1) == 0) wr = sr_next(wr);
                         if constexpr (PREC == kBF16SR && E % 2 == 0) {
                             // the instruction writes half (e & 1) of the word directly
-                            const uint32_t rp = (e & 1) ? (bp & 0xffff0000u) : (bp << 16);
-                            const uint32_t rq = (e & 1) ? (bq & 0xffff0000u) : (bq << 16);
+                            const uint32_t lo = wr << 16, hi = wr & 0xffff0000u;
+                            const uint32_t rp = (e & 1) ? hi : lo;
+                            const uint32_t rq = (e & 1) ? lo : hi;
                             const uint32_t op = (e & 1) ? po.w[e >> 1] : 0u, oq = (e & 1) ? qo.w[e >> 1] : 0u;
                             if (e & 1) {
                                 po.w[e >> 1] = bf16_sr_insert<true>(pn, rp, op);
@@ -361,8 +354,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                             }
                         } else {
                             static_assert(PREC == kBF16SR, "FP8 has its own write-back above");
-                            pb = f32_to_bf16_sr(pn, (e & 1) ? (bp >> 16) : bp);
-                            qb = f32_to_bf16_sr(qn, (e & 1) ? (bq >> 16) : bq);
+                            pb = f32_to_bf16_sr(pn, (e & 1) ? (wr >> 16) : wr);
+                            qb = f32_to_bf16_sr(qn, (e & 1) ? wr : (wr >> 16));
                         }
                     }
                     if constexpr (!(PREC == kBF16SR && E % 2 == 0)) {

@@ -532,10 +532,10 @@ typedef struct {
  * sample j (0..L-1) of window `batch`: one Philox block per window,
  * K = Philox(ctr = {batch, epoch, worker, STREAM_ROUND}).  Lane l of the lane
  * group (elements [lE, lE+E), E = k/G) draws the seed
- * W_0 = lowbias32(K[0] ^ (j*0x9E3779B9 + l*0x85EBCA77)) for its P elements
- * (K[2] for Q) and W_{i+1} = sr_next(W_i); its element i uses the 16-bit half
- * (i & 1) of W_{i>>1}.  Sixteen bits per element keep late-epoch updates far
- * below one FP8 step unbiased. */
+ * W_0 = lowbias32(K[0] ^ (j*0x9E3779B9 + l*0x85EBCA77)) and
+ * W_{i+1} = sr_next(W_i); element i of its P row uses the 16-bit half (i & 1)
+ * of W_{i>>1} and element i of its Q row the other half.  Sixteen bits per
+ * element keep late-epoch updates far below one FP8 step unbiased. */
 static inline uint32_t lowbias32(uint32_t x)
 {
     x ^= x >> 16;
@@ -559,9 +559,9 @@ static inline uint32_t sr_next(uint32_t x)
 static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e, int k, int G)
 {
     int E = k / G, lane = e / E, i = e % E;
-    uint32_t w = lowbias32(K[which ? 2 : 0] ^ (j * 0x9E3779B9u + (uint32_t)lane * 0x85EBCA77u));
+    uint32_t w = lowbias32(K[0] ^ (j * 0x9E3779B9u + (uint32_t)lane * 0x85EBCA77u));
     for (int s = 0; s < (i >> 1); ++s) w = sr_next(w);
-    return (i & 1) ? (w >> 16) : (w & 0xffffu);
+    return ((i & 1) ^ (which != 0)) ? (w >> 16) : (w & 0xffffu);
 }
 
 /* FP8 rows with power-of-two row scales sp, sq: the bytes' values are used
