@@ -344,3 +344,67 @@ def test_ranking_criteria_match_oracle_and_mf_predict(gpu, tmp_path):
         assert line.startswith(label[crit] + " = "), line
         if oracle.have_mf_predict():
             assert line.split("=", 1)[1].strip() == oracle.mf_predict_eval(test, model, crit)
+
+
+def test_c1_config_cli_beside_reference_cpu_trainer(gpu, tmp_path):
+    """The small configuration as specified: 10,000 x 2,000, 900,000 / 100,000
+    ratings, k=128, FP32 factors, lambda 0.05, alpha 0.08, beta 0.3, 20
+    iterations, serialized, through the cumf_sgd command line
+    (singleGPU/main.cpp's flags).  The model file scored by the reference's
+    mf-predict gives the RMSE the command printed, within 0.005 of the
+    serialized FP32 oracle (the command permutes the ids first, so the two
+    runs visit the same ratings with other initial rows).  The reference's
+    CPU trainer (test/mf-train: LIBMF's FPSGD, -s 1) runs beside it on the
+    same files; its test RMSE and rate are printed (another optimizer, no
+    bar)."""
+    import time
+
+    import cumf_sgd_amd as cs
+    import oracle
+    from cumf_sgd_amd import datagen
+
+    R = datagen.small(seed=1)
+    u, v, r, tu, tv, tr = R.numpy()
+    tr_path, te_path = str(tmp_path / "train.bin"), str(tmp_path / "test.bin")
+    datagen.write_triplets(tr_path, u, v, r)
+    datagen.write_triplets(te_path, tu, tv, tr)
+    model = str(tmp_path / "model")
+    t0 = time.perf_counter()
+    out = subprocess.run([cs.CLI_PATH, "-g", "0", "-l", "0.05", "-k", "128", "-t", "20", "-a", "0.08", "-b", "0.3",
+                          "-s", "1", "--serial", "--precision", "fp32", "--seed", "1", "-p", te_path, tr_path, model],
+                         capture_output=True, text=True, timeout=300)
+    cli_s = time.perf_counter() - t0
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = [l for l in out.stdout.splitlines() if "test RMSE" in l]
+    assert len(lines) == 20
+    cli_rmse = lines[-1].split()[-1]
+    if oracle.have_mf_predict():
+        assert oracle.mf_predict_rmse(te_path, model) == cli_rmse
+
+    # the serialized FP32 oracle on the unpermuted ids
+    std = np.float32(np.sqrt((r.astype(np.float64) ** 2).mean() - np.float64(r.astype(np.float64).mean()) ** 2))
+    ave = np.float32(r.astype(np.float64).mean())
+    scale = max(np.float32(1e-4), std)
+    rs = (r * np.float32(1.0 / scale)).astype(np.float32)
+    P, _ = oracle.init_factors(R.m, 128, 0, 1, 0)
+    Q, _ = oracle.init_factors(R.n, 128, 0, 1, 1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 24)
+    lam = np.float32(0.05) / scale
+    P, Q, _, _ = oracle.sgd_serial(u, v, rs, P, Q, lr, 0, 20, k=128, G=64, prec=0, lambda_p=lam, lambda_q=lam,
+                                   seed=1)
+    fs = np.float32(np.sqrt(scale))
+    want = oracle.rmse(tu, tv, tr, (P * fs).astype(np.float32), (Q * fs).astype(np.float32), ave / std)
+    print(f"C1 cumf_sgd --serial fp32: test RMSE {cli_rmse} ({cli_s:.1f} s with file I/O); serialized oracle "
+          f"{want:.4f}")
+    assert abs(float(cli_rmse) - want) <= 0.005
+
+    if os.access(oracle.MF_TRAIN, os.X_OK):
+        ref = subprocess.run([oracle.MF_TRAIN, "-l2", "0.05", "-k", "128", "-t", "20", "-s", "1", "-p", te_path,
+                              tr_path, str(tmp_path / "ref_model")], capture_output=True, text=True, timeout=300)
+        assert ref.returncode == 0, ref.stdout + ref.stderr
+        rows = [l.split() for l in ref.stdout.splitlines() if l.split() and l.split()[0].isdigit()]
+        perf = [l for l in ref.stdout.splitlines() if l.startswith("perf:")]
+        va = float(rows[-1][2])
+        print(f"C1 test/mf-train -s 1: test RMSE {va:.4f} after {len(rows)} iterations, "
+              f"{perf[0] if perf else 'no perf line'}")
+        assert len(rows) == 20 and np.isfinite(va)
