@@ -64,13 +64,13 @@ def prepare(ctx, R, seed: int):
 
 
 def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, depth=0, batch_len=0,
-               rmse_every=True):
+               rmse_every=True, rounds=0):
     """Fresh factors, `epochs` epochs; returns (per-epoch train seconds, rmse list)."""
     import cumf_sgd_amd as cs
 
     ctx.init_factors(K, prec, seed)
     opts = cs.make_opts(k=K, schedule=schedule, num_workers=workers, lambda_p=float(lam), seed=seed,
-                        lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len)
+                        lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len, stratum_rounds=rounds)
     fs = np.float32(np.sqrt(scale))
     times, rmses = [], []
     for ep in range(epochs):
@@ -132,6 +132,9 @@ def main() -> int:
                     help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
                          "(default); hogwild: the reference's 1x1 mode")
     ap.add_argument("--strata", type=int, default=16, help="s for the stratum schedule (s x s blocks)")
+    ap.add_argument("--rounds", type=int, default=0,
+                    help="stratum schedule: passes over the diagonals per epoch (0 = library default)")
+    ap.add_argument("--batch-len", type=int, default=0, help="records per window (even, <= 128; 0 = 128)")
     ap.add_argument("--rmse-epochs", type=int, default=20)
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
@@ -184,7 +187,8 @@ def main() -> int:
     # ---- throughput: warmup + K timed epochs (one step = one epoch)
     ctx.init_factors(K, prec, args.seed)
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
-                        lanes_per_sample=args.lanes, pipeline_depth=args.depth)
+                        lanes_per_sample=args.lanes, pipeline_depth=args.depth, stratum_rounds=args.rounds,
+                        batch_len=args.batch_len)
     if args.autotune:
         t1 = time.perf_counter()
         opts, tuned_ms = ctx.autotune(opts)
@@ -221,8 +225,9 @@ def main() -> int:
         for p in (0, 1, 2, 4, 3):
             # the headline format trains with the measured shape, the others
             # with their defaults (or what the command line fixed)
-            w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, 0)
-            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl)
+            w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, args.batch_len)
+            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl,
+                                      rounds=args.rounds)
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005

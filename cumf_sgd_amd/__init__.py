@@ -62,6 +62,8 @@ class TrainOpts(ctypes.Structure):
         ("beta", ctypes.c_float),
         ("lrate", ctypes.c_float),
         ("seed", ctypes.c_uint64),
+        ("stratum_rounds", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -474,7 +476,7 @@ class Batch:
 def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambda_p: float = 0.05,
               lambda_q: float | None = None, alpha: float = 0.08, beta: float = 0.3, lrate: float = 0.0,
               seed: int = 1, batch_len: int = 0, lanes_per_sample: int = 0, coherent: int = 0,
-              pipeline_depth: int = 0, use_graph: int = 0) -> TrainOpts:
+              pipeline_depth: int = 0, use_graph: int = 0, stratum_rounds: int = 0) -> TrainOpts:
     o = TrainOpts()
     o.k, o.schedule, o.num_workers = k, schedule, num_workers
     o.lambda_p = lambda_p
@@ -483,6 +485,7 @@ def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambd
     o.batch_len, o.lanes_per_sample, o.coherent = batch_len, lanes_per_sample, coherent
     o.pipeline_depth = pipeline_depth
     o.use_graph = use_graph
+    o.stratum_rounds = stratum_rounds
     return o
 
 

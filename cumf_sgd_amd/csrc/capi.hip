@@ -1027,6 +1027,7 @@ struct LaunchRec {
     int64_t task_off;
     int32_t task_cnt;
     int32_t num_xcd;
+    int32_t blocks;   // workgroups of the launch; 0 = the shape's
 };
 
 // The launches of one epoch, appended to `tasks`.
@@ -1036,9 +1037,9 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
 {
     const int nb = c->ux * c->vy;
     *upd_epoch = 0;
-    auto push = [&](std::vector<SgdTask>& t, int nx) {
+    auto push = [&](std::vector<SgdTask>& t, int nx, int32_t blocks = 0) {
         if (t.empty()) return;
-        launches.push_back(LaunchRec{(int64_t)tasks.size(), (int32_t)t.size(), nx});
+        launches.push_back(LaunchRec{(int64_t)tasks.size(), (int32_t)t.size(), nx, blocks});
         for (const SgdTask& x : t) *upd_epoch += (int64_t)x.worker_count * x.update_count * sh.L;
         tasks.insert(tasks.end(), t.begin(), t.end());
     };
@@ -1058,11 +1059,19 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
         // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
         // which share no P and no Q rows.  Each block runs on one XCD; the
         // blocks of a diagonal go to XCDs largest first onto the least-loaded
-        // one (a launch lasts as long as its most loaded XCD).
+        // one (a launch lasts as long as its most loaded XCD).  With R rounds
+        // the epoch passes over the s diagonals R times, each pass running
+        // 1/R of every block's windows (window indices continue across the
+        // passes), so that a row meets every column range R times per epoch.
         const int s = c->ux;
+        const int R = std::max(1, o->stratum_rounds);
         const int per_xcd = sh.W / num_xcd;
+        struct Placed {
+            SgdTask t;
+            int x;   // XCD
+        };
+        std::vector<std::vector<Placed>> diag((size_t)s);
         for (int d = 0; d < s; ++d) {
-            std::vector<SgdTask> t;
             std::vector<int> order((size_t)s);
             for (int i = 0; i < s; ++i) order[(size_t)i] = i * s + (i + d) % s;
             auto size_of = [&](int blk) { return c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk]; };
@@ -1077,7 +1086,7 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
             for (int x = 0; x < num_xcd; ++x) {
                 const std::vector<int>& mine = assigned[(size_t)x];
                 const int64_t tot = load[(size_t)x];
-                int wb = x * per_xcd, left = per_xcd;
+                int left = per_xcd;
                 for (size_t j = 0; j < mine.size(); ++j) {
                     const int blk = mine[j];
                     const int64_t sz = c->offsets[blk + 1] - c->offsets[blk];
@@ -1088,15 +1097,49 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                     // no more workers than windows: a small block gets one
                     // window per worker instead of many redundant ones
                     w = (int)std::min<int64_t>(w, (sz + sh.L - 1) / sh.L);
-                    SgdTask tk = base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk);
-                    tk.worker_begin = wb;
-                    wb += w;
                     left -= w;
-                    t.push_back(tk);
+                    diag[(size_t)d].push_back(
+                        Placed{base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk), x});
                 }
             }
-            push(t, num_xcd);
         }
+        // A launch's grid holds just its workers: XCD x numbers its workers
+        // [x * P, x * P + its count), P the largest count rounded up to whole
+        // workgroups (the kernel numbers workers XCD-major over the grid).
+        const int gran = kSgdWavesPerBlock * sh.S;
+        auto emit = [&](const std::vector<Placed>& pl) {
+            std::vector<int64_t> off((size_t)num_xcd, 0);
+            for (const Placed& q : pl) off[(size_t)q.x] += q.t.worker_count;
+            const int64_t most = std::max<int64_t>(1, *std::max_element(off.begin(), off.end()));
+            const int64_t P = (most + gran - 1) / gran * gran;
+            std::fill(off.begin(), off.end(), 0);
+            std::vector<SgdTask> t;
+            for (const Placed& q : pl) {
+                SgdTask tk = q.t;
+                tk.worker_begin = (int32_t)(q.x * P + off[(size_t)q.x]);
+                off[(size_t)q.x] += tk.worker_count;
+                t.push_back(tk);
+            }
+            push(t, num_xcd, (int32_t)(num_xcd * (P / gran)));
+        };
+        for (int rr = 0; rr < R; ++rr)
+            for (int d = 0; d < s; ++d) {
+                std::vector<Placed> pl;
+                for (const Placed& full : diag[(size_t)d]) {
+                    Placed q = full;
+                    if (R > 1) {
+                        // this pass: ceil(N / R) of the block's N windows, on at
+                        // most its workers, windows [rr * uc, (rr + 1) * uc)
+                        const int64_t n_all = (int64_t)full.t.worker_count * full.t.update_count;
+                        const int64_t n_pass = (n_all + R - 1) / R;
+                        q.t.worker_count = (int32_t)std::min<int64_t>(full.t.worker_count, n_pass);
+                        q.t.update_count = (int32_t)((n_pass + q.t.worker_count - 1) / q.t.worker_count);
+                        q.t.window_begin = rr * q.t.update_count;
+                    }
+                    pl.push_back(q);
+                }
+                emit(pl);
+            }
     }
     return CUMF_OK;
 }
@@ -1124,7 +1167,8 @@ int queue_one(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const Laun
             if (!ev0 || !ev1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
             HIP_TRY(hipEventRecord(ev0, c->stream));
         }
-        HIP_TRY(launch_sgd(a, sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D, c->stream));
+        HIP_TRY(launch_sgd(a, l.blocks ? l.blocks : sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D,
+                           c->stream));
         if (timed) HIP_TRY(hipEventRecord(ev1, c->stream));
     }
     return CUMF_OK;
@@ -1230,6 +1274,8 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     if (e0 < 0 || e1 < e0 || e1 > 1 << 20) return fail(CUMF_ERR_ARG, "bad epoch range [%d,%d)", e0, e1);
     if (o->schedule < CUMF_SCHED_HOGWILD || o->schedule > CUMF_SCHED_STRATUM) return fail(CUMF_ERR_ARG, "bad schedule");
     if (o->use_graph != 0 && o->use_graph != 1) return fail(CUMF_ERR_ARG, "use_graph must be 0 or 1");
+    if (o->stratum_rounds < 0 || o->stratum_rounds > 64)
+        return fail(CUMF_ERR_ARG, "stratum_rounds=%d out of range 0..64", o->stratum_rounds);
     if (c->nnz == 0 || e1 == e0) return CUMF_OK;
 
     const int nb = c->ux * c->vy;
@@ -1273,6 +1319,8 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     std::vector<int64_t> soff;
     if (c->streamed) {
         if (!stratum) return fail(CUMF_ERR_STATE, "streamed ratings train with the stratum schedule");
+        if (o->stratum_rounds > 1)
+            return fail(CUMF_ERR_ARG, "streamed ratings make one pass over the diagonals per epoch (stratum_rounds 1)");
         // point every task at its block's place in its diagonal's staging buffer
         const int s = c->ux;
         soff.assign((size_t)s * s, 0);

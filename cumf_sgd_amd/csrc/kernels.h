@@ -27,7 +27,8 @@ struct SgdTask {
     uint32_t tag;             // block id folded into the window counter
     int32_t k;                // factor length (batched launches dispatch on it)
     int32_t wave_count;       // batched launches: waves owned by the task
-    int32_t pad;
+    int32_t window_begin;     // the task's first window index in the epoch (its windows
+                              // are window_begin + [0, update_count))
 };
 
 struct SgdLaunch {

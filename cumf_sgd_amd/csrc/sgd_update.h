@@ -108,7 +108,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     // (slot, ls) fetches record j = ls + G*c of its slot into word 64*c + lane,
     // so record j of a slot sits at (j / G) * 64 + slot * G + j % G.
     auto fetch_window = [&](int ep, int b) {
-        const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)b, nnz);
+        const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)(T.window_begin + b), nnz);
         // reads of the previous window's records have returned before they
         // are overwritten
         __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
@@ -161,7 +161,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         const float cp = __builtin_fmaf(-lr, T.lambda_p, 1.0f);
         const float cq = __builtin_fmaf(-lr, T.lambda_q, 1.0f);
         U4 srk = U4{0u, 0u, 0u, 0u};
-        if constexpr (SR) srk = philox(U4{(uint32_t)b, (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0, key.k1);
+        if constexpr (SR)
+            srk = philox(U4{(uint32_t)(T.window_begin + b), (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0,
+                         key.k1);
         stage_window();
         const bool more_windows = w + 1 < nwin;
         auto fetch_next = [&]() {

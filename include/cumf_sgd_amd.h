@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CUMF_ABI_VERSION 1
+#define CUMF_ABI_VERSION 2
 
 enum cumf_status {
     CUMF_OK = 0,
@@ -76,6 +76,9 @@ typedef struct cumf_train_opts {
     float beta;
     float lrate;
     uint64_t seed;             /* Philox key for windows and rounding */
+    int32_t stratum_rounds;    /* stratum schedule: passes over the s diagonals per epoch, each running
+                                  1/rounds of every block's windows; 0 = 1 */
+    int32_t reserved;          /* 0 */
 } cumf_train_opts;
 
 typedef struct cumf_ctx cumf_ctx;

@@ -6,7 +6,7 @@ tests/test_gpu_large.py::test_large_sparse_formats (1,000,990 x 624,961,
 host-generated with seed 3, srand(3) id maps, scale = std of the training
 ratings, lambda 0.05, alpha 0.08, beta 0.3), and writes the test RMSE after
 every epoch into large_oracle_rmse.json.  Takes 30-60 minutes on one core per
-precision; the two can run side by side:
+precision; run the two one after the other:
     python tests/golden/make_large_oracle.py [fp32|fp64]
 """
 import json
@@ -71,12 +71,12 @@ def main(which: str) -> None:
         curve.append(oracle.rmse(tu, tv, tr, Pe, Qe, b))
         print(f"{which} epoch {ep}: rmse {curve[-1]:.5f} ({time.time() - t0:.0f}s)", flush=True)
     path = os.path.join(HERE, "large_oracle_rmse.json")
-    out = {which: curve,
-           "workload": ("large sparse: 1000990 x 624961, 250272272 train + 2528003 test, seed 3 (host), k=128, "
-                        "lambda 0.05, alpha 0.08, beta 0.3, serialized single worker, windows of 128"),
-           "script": "tests/golden/make_large_oracle.py"}
-    with open(path + f".{which}", "w") as f:
-        json.dump(out, f, indent=1)
+    data = json.load(open(path)) if os.path.exists(path) else {}
+    data[which] = curve
+    data["workload"] = ("large sparse: 1000990 x 624961, 250272272 train + 2528003 test, seed 3 (host), k=128, "
+                        "lambda 0.05, alpha 0.08, beta 0.3, serialized single worker, windows of 128")
+    data["script"] = "tests/golden/make_large_oracle.py"
+    json.dump(data, open(path, "w"), indent=1)
 
 
 if __name__ == "__main__":
