@@ -1122,23 +1122,31 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
             }
             push(t, num_xcd, (int32_t)(num_xcd * (P / gran)));
         };
+        // pass rr runs N*(rr+1)/R - N*rr/R (rounded up to whole windows per
+        // worker) of a block's N windows on at most its workers; a worker's
+        // window indices continue from pass to pass
+        std::vector<std::vector<int32_t>> next_window((size_t)s);
+        for (int d = 0; d < s; ++d) next_window[(size_t)d].assign(diag[(size_t)d].size(), 0);
         for (int rr = 0; rr < R; ++rr)
             for (int d = 0; d < s; ++d) {
                 std::vector<Placed> pl;
-                for (const Placed& full : diag[(size_t)d]) {
+                for (size_t j = 0; j < diag[(size_t)d].size(); ++j) {
+                    const Placed& full = diag[(size_t)d][j];
                     Placed q = full;
                     if (R > 1) {
-                        // this pass: ceil(N / R) of the block's N windows, on at
-                        // most its workers, windows [rr * uc, (rr + 1) * uc)
                         const int64_t n_all = (int64_t)full.t.worker_count * full.t.update_count;
-                        const int64_t n_pass = (n_all + R - 1) / R;
-                        q.t.worker_count = (int32_t)std::min<int64_t>(full.t.worker_count, n_pass);
-                        q.t.update_count = (int32_t)((n_pass + q.t.worker_count - 1) / q.t.worker_count);
-                        q.t.window_begin = rr * q.t.update_count;
+                        const int64_t n_pass = n_all * (rr + 1) / R - n_all * rr / R;
+                        if (n_pass == 0) continue;
+                        // fewest windows per worker, then fewest workers:
+                        // w * uc - n_pass < uc
+                        q.t.update_count = (int32_t)((n_pass + full.t.worker_count - 1) / full.t.worker_count);
+                        q.t.worker_count = (int32_t)((n_pass + q.t.update_count - 1) / q.t.update_count);
+                        q.t.window_begin = next_window[(size_t)d][j];
+                        next_window[(size_t)d][j] += q.t.update_count;
                     }
                     pl.push_back(q);
                 }
-                emit(pl);
+                if (!pl.empty()) emit(pl);
             }
     }
     return CUMF_OK;

@@ -440,6 +440,8 @@ def test_streamed_ratings_equal_resident_stratum(gpu):
             c.set_test(tu, tv, tr)
             out.append((off, c.get_factors(0)[0], c.get_factors(1)[0], c.test_rmse()))
             if streamed:
+                with pytest.raises(cs.CumfError):   # one pass over the diagonals per epoch
+                    c.train(cs.make_opts(k=64, schedule=cs.STRATUM, seed=4, stratum_rounds=2), 3, 4)
                 gu, gv, gr = c.get_ratings()
                 assert np.array_equal(np.sort(gu * 1000 + gv), np.sort(u.astype(np.int64) * 1000 + v))
                 with pytest.raises(cs.CumfError):
@@ -447,6 +449,45 @@ def test_streamed_ratings_equal_resident_stratum(gpu):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
     assert out[0][3] == out[1][3]
+
+
+def test_stratum_rounds_split_windows_exactly(gpu):
+    """Passes over the diagonals (stratum_rounds) split each block's windows
+    across the passes without adding any.  With at most one window per block
+    (one worker each) a stratum epoch is deterministic, and only the last
+    pass holds a block's window, so every round count trains exactly like one
+    pass: bit-equal factors and equal updates per epoch.  On blocks of many
+    windows the passes still cover the epoch's windows once (updates per
+    epoch within 5% of a single pass's) and the model trains."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=8, m=320, n=200, nnz=6_000, n_test=600)
+    out = {}
+    for R in (1, 2, 3, 8):
+        with _ctx(u, v, r, m, n) as c:
+            off = c.partition_strata(8)
+            assert np.diff(off).max() <= 128
+            c.init_factors(64, cs.FP16, seed=4)
+            c.train(cs.make_opts(k=64, schedule=cs.STRATUM, seed=4, lambda_p=0.05, stratum_rounds=R), 0, 3)
+            out[R] = (c.get_factors(0)[0], c.get_factors(1)[0], c.last_launch().updates_per_epoch)
+    for R in (2, 3, 8):
+        assert out[R][2] == out[1][2]
+        assert np.array_equal(out[R][0], out[1][0]) and np.array_equal(out[R][1], out[1][1])
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=9, m=2_000, n=900, nnz=300_000, n_test=20_000)
+    upd = {}
+    for R in (1, 4):
+        with _ctx(u, v, r, m, n) as c:
+            c.set_test(tu, tv, tr)
+            c.partition_strata(4)
+            c.init_factors(64, cs.FP32, seed=4)
+            before = c.test_rmse(1.0, 3.0)
+            c.train(cs.make_opts(k=64, schedule=cs.STRATUM, seed=4, lambda_p=0.05, num_workers=256,
+                                 stratum_rounds=R), 0, 4)
+            upd[R] = c.last_launch().updates_per_epoch
+            after = c.test_rmse(1.0, 3.0)
+            assert np.isfinite(after) and after < before
+    assert len(u) <= upd[4] <= upd[1] * 1.05, upd
 
 
 def test_streamed_ratings_train_the_medium_workload(gpu):
