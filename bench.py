@@ -6,7 +6,7 @@ k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated (seeded) on the host
 and copied to HBM.  One
 step = one epoch of batch-Hogwild SGD over all training ratings (the
 reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
-by default under the stratum schedule (16 x 16 nnz-balanced blocks, the 16
+by default under the stratum schedule (24 x 24 nnz-balanced blocks, the 24
 conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
 the least-loaded one, one launch per diagonal; --schedule hogwild runs the
 reference's 1x1 mode).
@@ -131,7 +131,7 @@ def main() -> int:
     ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum"],
                     help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
                          "(default); hogwild: the reference's 1x1 mode")
-    ap.add_argument("--strata", type=int, default=16, help="s for the stratum schedule (s x s blocks)")
+    ap.add_argument("--strata", type=int, default=24, help="s for the stratum schedule (s x s blocks)")
     ap.add_argument("--rounds", type=int, default=0,
                     help="stratum schedule: passes over the diagonals per epoch (0 = library default)")
     ap.add_argument("--batch-len", type=int, default=0, help="records per window (even, <= 128; 0 = 128)")
