@@ -479,7 +479,9 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
     cumf_train_opts o = opts_from(p, extra, scale, seed);
     if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID;
     if (o.schedule == CUMF_SCHED_STRATUM) {
-        const int s = ux == vy && ux > 1 ? ux : 8;
+        // a square grid gives s; otherwise 24 x 24 blocks, the measured
+        // fastest for 16-bit factors on a Netflix-shaped problem
+        const int s = ux == vy && ux > 1 ? ux : 24;
         CUMF_TRY(cumf_partition_strata(ctx, s, nullptr));
     }
     for (int ep = 0; ep < p.num_iters; ++ep) {

@@ -282,7 +282,8 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
  * save-ready output: statistics, id permutation (srand(seed) +
  * std::random_shuffle, sgd.cu:39-60), grid, scaling, init, training, rescale
  * and inverse permutation.  R is read only.  opts (may be NULL) overrides
- * the schedule, lanes_per_sample, coherent and batch_len; its k, lambdas,
+ * the schedule (the stratum schedule takes s from a square u x v grid, else
+ * 24), lanes_per_sample, coherent, batch_len and stratum_rounds; its k, lambdas,
  * alpha, beta, lrate and seed are taken from para and seed.  With a 1x1 grid
  * everything runs on the device; with more blocks the id shuffle and the
  * cycle-swap partition run on the host first so that block contents and

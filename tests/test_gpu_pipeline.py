@@ -54,6 +54,29 @@ def test_sgd_train_pipeline_and_model_file(gpu, tmp_path, grid):
         assert oracle.mf_predict_rmse(test, model) == f"{rmse[-1]:.4f}"
 
 
+@pytest.mark.parametrize("rounds", [0, 4])
+def test_sgd_train_stratum_schedule(gpu, rounds):
+    """cumf_sgd_train with the stratum schedule on a 1x1 grid (24 x 24
+    blocks), one pass or four passes over the diagonals per epoch: the model
+    trains, stays finite and ends no worse than the Hogwild pipeline's (on a
+    problem this small every XCD's L2 holds its own copy of every row for a
+    whole Hogwild epoch, while stratum blocks never share a row across XCDs
+    within a launch)."""
+    import cumf_sgd_amd as cs
+
+    R = _data()
+    u, v, r, tu, tv, tr = R.numpy()
+    m, n = int(u.max()) + 1, int(v.max()) + 1
+    p = cs.make_param(k=64, num_workers=256, num_iters=8)
+    _, _, _, hog = cs.sgd_train(p, _nodes(u, v, r), m, n, precision=cs.FP16, seed=3, test=_nodes(tu, tv, tr))
+    o = cs.make_opts(k=64, schedule=cs.STRATUM, stratum_rounds=rounds)
+    P, Q, b, st = cs.sgd_train(p, _nodes(u, v, r), m, n, precision=cs.FP16, seed=3, test=_nodes(tu, tv, tr),
+                               opts=o)
+    print(f"rounds={rounds} stratum {st[-1]:.4f} hogwild {hog[-1]:.4f}")
+    assert np.all(np.isfinite(P)) and np.all(np.isfinite(Q))
+    assert st[-1] < 0.95 * st[0] and st[-1] <= hog[-1] + 0.005
+
+
 def test_sgd_update_k128_seam(gpu):
     """The reference's host pipeline (restated by the oracle: maps, grid,
     scale with id localisation, FP16 init) feeding the drop-in for
