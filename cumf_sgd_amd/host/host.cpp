@@ -477,7 +477,11 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
     const float fs = std::sqrt(scale);   // scale_model, sgd.cu:598
 
     cumf_train_opts o = opts_from(p, extra, scale, seed);
-    if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID;
+    // A 1x1 grid trains the conflict-free stratum schedule: under Hogwild
+    // every XCD's L2 keeps its own copy of the rows it updates until the
+    // kernel ends, which on problems whose rows fit in the L2s loses most
+    // updates.  Gridded runs keep the reference's block order.
+    if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID;
     if (o.schedule == CUMF_SCHED_STRATUM) {
         // a square grid gives s; otherwise 24 x 24 blocks, the measured
         // fastest for 16-bit factors on a Netflix-shaped problem

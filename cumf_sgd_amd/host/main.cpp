@@ -8,7 +8,7 @@
 //   --seed N        Philox key and srand seed (reference: time(NULL))
 //   --precision P   fp32 | fp16 (default, the reference's) | bf16 | bf16sr | fp8
 //   --serial        one worker, deterministic window order
-//   --schedule S    hogwild | grid | stratum
+//   --schedule S    hogwild | grid | stratum (default: stratum for a 1x1 grid, grid otherwise)
 //   --lanes G       lanes per update (8..64)
 //   --coherent      device-scope coherent factor accesses
 // With -p the test RMSE is printed after every epoch.  A 1x1 grid streams the
@@ -167,7 +167,9 @@ int main(int argc, char** argv)
     }
     cumf_train_opts o;
     std::memset(&o, 0, sizeof(o));
-    o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_HOGWILD : CUMF_SCHED_GRID);
+    // 1x1 grid: the conflict-free stratum schedule (--schedule hogwild for the
+    // reference's own 1x1 mode); gridded runs keep the reference's block order
+    o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID);
     o.lanes_per_sample = a.lanes;
     o.coherent = a.coherent;
     std::vector<double> rmse((size_t)(a.p.num_iters > 0 ? a.p.num_iters : 1), 0.0);

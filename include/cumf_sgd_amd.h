@@ -285,9 +285,10 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
  * the schedule (the stratum schedule takes s from a square u x v grid, else
  * 24), lanes_per_sample, coherent, batch_len and stratum_rounds; its k, lambdas,
  * alpha, beta, lrate and seed are taken from para and seed.  With a 1x1 grid
- * everything runs on the device; with more blocks the id shuffle and the
- * cycle-swap partition run on the host first so that block contents and
- * order are the reference's.  P_out (m*k) and Q_out (n*k) receive the final model in original id order,
+ * everything runs on the device, under the stratum schedule unless opts
+ * says otherwise; with more blocks the id shuffle and the cycle-swap
+ * partition run on the host first so that block contents and order are the
+ * reference's.  P_out (m*k) and Q_out (n*k) receive the final model in original id order,
  * *b_out the model's b (ave/std_dev, sgd.cu:679).  precision selects the
  * factor format (CUMF_FP16 is the reference's).  test (may be NULL, test_nnz
  * records, original ids) is scored after every epoch into rmse_out[epoch]

@@ -431,3 +431,37 @@ def test_c1_config_cli_beside_reference_cpu_trainer(gpu, tmp_path):
         print(f"C1 test/mf-train -s 1: test RMSE {va:.4f} after {len(rows)} iterations, "
               f"{perf[0] if perf else 'no perf line'}")
         assert len(rows) == 20 and np.isfinite(va)
+
+
+def test_cli_default_schedule_on_tiny_and_small_inputs(gpu, tmp_path):
+    """The command line's 1x1 default (the stratum schedule, 24 x 24 blocks)
+    on a 7-rating file with fewer ids than strata and on a small problem
+    whose rows fit in the L2s: both train and write the model file; on the
+    small problem the stratum run ends below the Hogwild run of the same
+    command (--schedule hogwild)."""
+    import cumf_sgd_amd as cs
+    from cumf_sgd_amd import datagen
+
+    tiny = str(tmp_path / "tiny.bin")
+    datagen.write_triplets(tiny, np.array([0, 1, 2, 0, 1, 2, 1], np.int32), np.array([0, 0, 1, 1, 2, 2, 0], np.int32),
+                           np.array([1, 2, 3, 4, 5, 4, 3], np.float32))
+    out = subprocess.run([cs.CLI_PATH, "-k", "32", "-t", "3", "-s", "8", "--seed", "2", tiny, str(tmp_path / "tm")],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert os.path.getsize(str(tmp_path / "tm")) == 20 + 4 * (3 + 3) * 32
+
+    R = _data(nnz=150_000, n_test=15_000)
+    u, v, r, tu, tv, tr = R.numpy()
+    tr_path, te_path = str(tmp_path / "train.bin"), str(tmp_path / "test.bin")
+    datagen.write_triplets(tr_path, u, v, r)
+    datagen.write_triplets(te_path, tu, tv, tr)
+    last = {}
+    for sched in ("default", "hogwild"):
+        extra = [] if sched == "default" else ["--schedule", "hogwild"]
+        out = subprocess.run([cs.CLI_PATH, "-l", "0.05", "-a", "0.08", "-b", "0.3", "-s", "256", "-k", "64", "-t", "8",
+                              "--seed", "4", *extra, "-p", te_path, tr_path, str(tmp_path / f"m_{sched}")],
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        last[sched] = float([l for l in out.stdout.splitlines() if "test RMSE" in l][-1].split()[-1])
+    print(f"default (stratum) {last['default']:.4f}, hogwild {last['hogwild']:.4f}")
+    assert last["default"] <= last["hogwild"] + 0.005
