@@ -339,7 +339,10 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
     o.lrate = p.lrate;
     o.seed = seed;
     if (p.u_grid * p.v_grid == 1) {
-        o.schedule = CUMF_SCHED_HOGWILD;
+        // the 1x1 grid trains conflict-free 24 x 24 strata (Hogwild loses
+        // updates to the per-XCD L2 copies of rows on small problems)
+        CUMF_TRY(cumf_partition_strata(g.c, 24, nullptr));
+        o.schedule = CUMF_SCHED_STRATUM;
         CUMF_TRY(cumf_train(g.c, &o, 0, p.num_iters, nullptr, 0));
     } else {
         o.schedule = CUMF_SCHED_GRID;

@@ -270,9 +270,11 @@ typedef struct cumf_param {
  * scaled, id-localised ratings (R2D[ux*vy] block pointers with gridSize
  * counts), the FP16 factor bits halfp (ux*u_seg*k) and halfq (vy*v_seg*k),
  * updated in place, and receives FP32 copies in floatp/floatq.  Schedule:
- * 1x1 -> Hogwild on the whole matrix; otherwise blocks in the reference's
- * order, driven by the C library rand() exactly as the reference does
- * (sgd_kernel.cu:253-267, 399-451).  seed is the Philox key. */
+ * 1x1 -> conflict-free 24 x 24 strata over the whole matrix (the
+ * reference's 1x1 Hogwild would keep per-XCD L2 copies of rows for a whole
+ * epoch); otherwise blocks in the reference's order, driven by the C
+ * library rand() exactly as the reference does (sgd_kernel.cu:253-267,
+ * 399-451).  seed is the Philox key. */
 int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u_seg, int64_t v_seg,
                          const cumf_node* const* R2D, const int64_t* grid_size,
                          uint16_t* halfp, uint16_t* halfq, float* floatp, float* floatq,

@@ -77,10 +77,13 @@ def test_sgd_train_stratum_schedule(gpu, rounds):
     assert st[-1] < 0.95 * st[0] and st[-1] <= hog[-1] + 0.005
 
 
-def test_sgd_update_k128_seam(gpu):
+@pytest.mark.parametrize("g", [1, 2])
+def test_sgd_update_k128_seam(gpu, g):
     """The reference's host pipeline (restated by the oracle: maps, grid,
     scale with id localisation, FP16 init) feeding the drop-in for
-    sgd_update_k128; outputs are consistent and the model trains."""
+    sgd_update_k128, on a 1x1 grid (stratum schedule inside) and a 2x2 grid
+    (the reference's block order); outputs are consistent and the model
+    trains."""
     import cumf_sgd_amd as cs
     import oracle
 
@@ -92,7 +95,7 @@ def test_sgd_update_k128_seam(gpu):
     scale = max(np.float32(1e-4), np.float32(std))
     pm, qm = oracle.gen_random_map(7, m), oracle.gen_random_map(7, n)
     shuf = oracle.shuffle_problem(nodes, pm, qm)
-    ux = vy = 2
+    ux = vy = g
     grid, gs, off = oracle.grid_problem(shuf, m, n, ux, vy)
     us, vs = oracle.grid_segments(m, n, ux, vy)
     local = oracle.scale_problem(grid, np.float32(1.0 / scale), ux, vy, us, vs)
@@ -100,7 +103,7 @@ def test_sgd_update_k128_seam(gpu):
     k = 128
     hp, _ = oracle.init_factors(ux * us, k, oracle.F16, 7, 0)
     hq, _ = oracle.init_factors(vy * vs, k, oracle.F16, 7, 1)
-    p = cs.make_param(k=k, num_workers=128, u_grid=2, v_grid=2, num_iters=5, lambda_p=0.05 / scale)
+    p = cs.make_param(k=k, num_workers=128, u_grid=g, v_grid=g, num_iters=5, lambda_p=0.05 / scale)
     hp2, hq2 = hp.copy(), hq.copy()
     fp, fq = cs.sgd_update_k128(p, m, n, us, vs, blocks, hp2, hq2, seed=7)
     assert not np.array_equal(hp2, hp)
