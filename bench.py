@@ -101,7 +101,8 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
         if line.startswith("perf:"):
             return {"value": float(line.split()[1]), "unit": "rating-updates/s", "cores": threads,
                     "kind": "reference",
-                    "sample": f"test/mf-train (LIBMF FPSGD, SSE+OpenMP) on the first {sample} training ratings "
+                    "sample": f"test/mf-train (LIBMF FPSGD, SSE+OpenMP) on "
+                              f"{'all' if sample == R.nnz else 'the first'} {sample} training ratings "
                               f"of the same workload, {iters} iterations, k={K}, -s {threads}; its optimizer is "
                               f"AdaGrad-style FPSGD, not the GPU kernel's plain SGD"}
     return None
@@ -137,8 +138,9 @@ def main() -> int:
     ap.add_argument("--batch-len", type=int, default=0, help="records per window (even, <= 128; 0 = 128)")
     ap.add_argument("--rmse-epochs", type=int, default=20)
     ap.add_argument("--no-rmse", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="training ratings for the CPU baseline (-1 = all of them, 0 = skip)")
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--depth", type=int, default=0, help="pipeline depth (2, 4, 8; 0 = library default)")
@@ -261,9 +263,10 @@ def main() -> int:
     # ---- CPU baseline (rank 0, N=1 only)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample != 0:
         try:
-            cpu = cpu_baseline(R, threads, args.cpu_sample, args.cpu_iters)
+            sample = R.nnz if args.cpu_sample < 0 else min(args.cpu_sample, R.nnz)
+            cpu = cpu_baseline(R, threads, sample, args.cpu_iters)
         except Exception as e:  # the baseline never decides the GPU result
             log(f"cpu baseline failed: {e}")
 
