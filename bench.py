@@ -9,12 +9,15 @@ reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
 by default under the stratum schedule (24 x 24 nnz-balanced blocks, the 24
 conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
 the least-loaded one, one launch per diagonal; --schedule hogwild runs the
-reference's 1x1 mode).
+reference's 1x1 mode).  Each epoch's launches are captured once as a HIP
+graph and replayed (--no-graph launches them one by one).
 Inputs are resident in HBM before the timed region starts.
 
 Printed: one JSON line with value (rating updates/s of the timed epochs),
 roofline of the update kernel (algorithmic bytes per launch over its
-HIP-event duration), time-to-target RMSE for each factor format against the
+HIP-event duration; with graph replay the events bracket each epoch's graph,
+so the average launch includes the short gaps between launches), time-to-target
+RMSE for each factor format against the
 FP32 GPU run, and the reference's CPU trainer (test/mf-train, built from the
 reference sources into oracle/_ref) timed on a bounded sample on the host.
 """
@@ -64,13 +67,14 @@ def prepare(ctx, R, seed: int):
 
 
 def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule, depth=0, batch_len=0,
-               rmse_every=True, rounds=0):
+               rmse_every=True, rounds=0, graph=0):
     """Fresh factors, `epochs` epochs; returns (per-epoch train seconds, rmse list)."""
     import cumf_sgd_amd as cs
 
     ctx.init_factors(K, prec, seed)
     opts = cs.make_opts(k=K, schedule=schedule, num_workers=workers, lambda_p=float(lam), seed=seed,
-                        lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len, stratum_rounds=rounds)
+                        lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len, stratum_rounds=rounds,
+                        use_graph=graph)
     fs = np.float32(np.sqrt(scale))
     times, rmses = [], []
     for ep in range(epochs):
@@ -136,6 +140,8 @@ def main() -> int:
     ap.add_argument("--rounds", type=int, default=0,
                     help="stratum schedule: passes over the diagonals per epoch (0 = library default)")
     ap.add_argument("--batch-len", type=int, default=0, help="records per window (even, <= 128; 0 = 128)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each epoch's kernels one by one instead of replaying them as a captured HIP graph")
     ap.add_argument("--rmse-epochs", type=int, default=20)
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=-1,
@@ -190,7 +196,7 @@ def main() -> int:
     ctx.init_factors(K, prec, args.seed)
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
                         lanes_per_sample=args.lanes, pipeline_depth=args.depth, stratum_rounds=args.rounds,
-                        batch_len=args.batch_len)
+                        batch_len=args.batch_len, use_graph=int(not args.no_graph))
     if args.autotune:
         t1 = time.perf_counter()
         opts, tuned_ms = ctx.autotune(opts)
@@ -229,7 +235,7 @@ def main() -> int:
             # with their defaults (or what the command line fixed)
             w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, args.batch_len)
             times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl,
-                                      rounds=args.rounds)
+                                      rounds=args.rounds, graph=int(not args.no_graph))
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005
@@ -289,7 +295,7 @@ def main() -> int:
                    "precision": args.precision,
                    "factor_storage": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3 + f32 row scale",
                                       4: "bf16 (stochastic rounding)"}[prec],
-                   "schedule": sched_tag, "workers": info.num_workers,
+                   "schedule": sched_tag, "graph_replay": not args.no_graph, "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
