@@ -485,6 +485,9 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
     // kernel ends, which on problems whose rows fit in the L2s loses most
     // updates.  Gridded runs keep the reference's block order.
     if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID;
+    // each epoch's launches replay as one captured graph (gridded runs with a
+    // new block order per epoch launch one by one)
+    if (!extra) o.use_graph = 1;
     if (o.schedule == CUMF_SCHED_STRATUM) {
         // a square grid gives s; otherwise 24 x 24 blocks, the measured
         // fastest for 16-bit factors on a Netflix-shaped problem

@@ -172,6 +172,7 @@ int main(int argc, char** argv)
     o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID);
     o.lanes_per_sample = a.lanes;
     o.coherent = a.coherent;
+    o.use_graph = 1;   // each epoch's launches replay as one captured graph
     std::vector<double> rmse((size_t)(a.p.num_iters > 0 ? a.p.num_iters : 1), 0.0);
     float b = 0.0f;
     if (a.p.ux * a.p.vy == 1) {
