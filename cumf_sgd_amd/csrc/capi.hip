@@ -1323,6 +1323,10 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         if ((rc = plan_epoch(c, o, sh, num_xcd, e0 + i, e0, block_order, order_len, tasks, plan.back(), &upd))) return rc;
     }
     c->last.updates_per_epoch = upd;
+    // stratum launches carry their own, smaller grids: report the largest
+    int32_t most_blocks = 0;
+    for (const LaunchRec& l : plan[0]) most_blocks = std::max(most_blocks, l.blocks);
+    if (most_blocks > 0) c->last.blocks = most_blocks;
     if (tasks.empty()) return CUMF_OK;
     std::vector<int64_t> soff;
     if (c->streamed) {

@@ -221,7 +221,7 @@ typedef struct cumf_launch_info {
     int32_t num_workers;
     int32_t lanes_per_sample;
     int32_t waves_per_block;
-    int32_t blocks;
+    int32_t blocks;            /* workgroups per launch (stratum: of the largest launch) */
     int32_t batch_len;
     int32_t pipeline_depth;
     int64_t updates_per_epoch;
