@@ -58,6 +58,8 @@ _sig = {
     "orc_scale_problem": ([P, I64, F, ctypes.c_int, ctypes.c_int, I64, I64], None),
     "orc_tree_dot": ([P, P, ctypes.c_int, ctypes.c_int], F),
     "orc_sgd_serial": ([ctypes.POINTER(SgdCfg), P, P, P, I64, P, P, P, P, P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "orc_sgd_block_windows": ([ctypes.POINTER(SgdCfg), U32, P, P, P, I64, P, P, P, P, P, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int], ctypes.c_int),
     "orc_sgd_serial_f64": ([ctypes.c_int, ctypes.c_int, U64, U32, ctypes.c_double, ctypes.c_double, P, P, P, I64, P, P,
                             P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "orc_widen": ([P, P, I64, ctypes.c_int, ctypes.c_int, P], None),
@@ -195,6 +197,23 @@ def sgd_serial(u, v, r, P_, Q_, lr, epoch_begin, epoch_end, *, k, G, prec, lambd
     _lib.orc_sgd_serial(ctypes.byref(cfg), _p(u), _p(v), _p(r), len(u), _p(P2), _p(Q2), _p(ps2), _p(qs2), _p(lr),
                         epoch_begin, epoch_end)
     return P2, Q2, ps2, qs2
+
+
+def sgd_block_windows(u, v, r, P_, Q_, lr, epoch, tag, window_begin, windows, *, k, G, prec, lambda_p, lambda_q,
+                      seed, batch_len=128, ps=None, qs=None, problem=0):
+    """Windows [window_begin, window_begin + windows) of worker 0 of one rating
+    block (tag = block id) in `epoch`, updating P_/Q_ (and FP8 scales) in
+    place; they must be C-contiguous arrays of the storage format."""
+    for a in (P_, Q_) + tuple(x for x in (ps, qs) if x is not None):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("factor arrays must be C-contiguous")
+    cfg = SgdCfg(k, G, prec, batch_len, 1, problem, seed, lambda_p, lambda_q)
+    u = np.ascontiguousarray(u, np.int32)
+    v = np.ascontiguousarray(v, np.int32)
+    r = np.ascontiguousarray(r, np.float32)
+    lr = np.ascontiguousarray(lr, np.float32)
+    _lib.orc_sgd_block_windows(ctypes.byref(cfg), tag, _p(u), _p(v), _p(r), len(u), _p(P_), _p(Q_), _p(ps), _p(qs),
+                               _p(lr), epoch, window_begin, windows)
 
 
 def sgd_serial_f64(u, v, r, P_, Q_, lr, epoch_begin, epoch_end, *, k, lambda_p, lambda_q, seed, batch_len=128,

@@ -670,6 +670,44 @@ EXPORT int orc_sgd_serial(const orc_sgd_cfg* c, const int32_t* ru, const int32_t
     return 0;
 }
 
+/* Windows [window_begin, window_begin + windows) of worker 0 of one rating
+ * block in epoch `epoch`: the block's task in a gridded or stratum launch
+ * (singleGPU/sgd_kernel.cu:249-372 trains one block per launch), whose
+ * window draws and rounding words carry the block id as `tag`.  Each window
+ * is L consecutive records of the block modulo its size (h:50-56).  With at
+ * most one worker per block the stratum schedule's epoch is exactly: the
+ * diagonals in order, and within a diagonal each block's windows in turn
+ * (its blocks share no P or Q rows, so their order does not matter).
+ * Factors are updated in place. */
+EXPORT int orc_sgd_block_windows(const orc_sgd_cfg* c, uint32_t tag, const int32_t* ru, const int32_t* rv,
+                                 const float* rr, int64_t nnz, void* P, void* Q, float* ps, float* qs,
+                                 const float* lr_table, int epoch, int window_begin, int windows)
+{
+    if (nnz <= 0) return 0;
+    float* p = (float*)malloc(sizeof(float) * (size_t)c->k);
+    float* q = (float*)malloc(sizeof(float) * (size_t)c->k);
+    float lanes[256];
+    const float lr = lr_table[epoch];
+    for (int w = 0; w < windows; ++w) {
+        const uint32_t b = (uint32_t)(window_begin + w);
+        int64_t start = orc_window_start(c->seed, c->problem, tag, 0, (uint32_t)epoch, b, nnz);
+        uint32_t K[4] = {0, 0, 0, 0};
+        if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
+            uint32_t ctr[4] = {b, (uint32_t)epoch, 0u, STREAM_ROUND | (tag << 8)};
+            uint32_t key[2];
+            key_of(c->seed, c->problem, key);
+            orc_philox4x32_10(ctr, key, K);
+        }
+        for (int i = 0; i < c->batch_len; ++i) {
+            int64_t off = (start + i) % nnz;
+            sgd_update_one(c, P, Q, ps, qs, ru[off], rv[off], rr[off], lr, K, (uint32_t)i, p, q, lanes);
+        }
+    }
+    free(p);
+    free(q);
+    return 0;
+}
+
 /* FP64 serialized oracle: same windows, everything in double, sequential dot. */
 EXPORT int orc_sgd_serial_f64(int k, int batch_len, uint64_t seed, uint32_t problem,
                               double lambda_p, double lambda_q, const int32_t* ru,

@@ -451,6 +451,50 @@ def test_streamed_ratings_equal_resident_stratum(gpu):
     assert out[0][3] == out[1][3]
 
 
+@pytest.mark.parametrize("prec,rounds", [(0, 1), (1, 1), (1, 3), (4, 2), (3, 1)])
+def test_stratum_epochs_match_oracle_when_deterministic(gpu, prec, rounds):
+    """With at most one window per block (one worker each) a stratum epoch is
+    deterministic and equals its restatement: the diagonals in order, each
+    block's window drawn with the block id as tag (the blocks of a diagonal
+    share no rows), the FP8 rescale after each epoch.  GPU factors after
+    three epochs equal the oracle's bit for bit, for one and several passes
+    over the diagonals (the window then runs in the last pass)."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=8, m=320, n=200, nnz=6_000, n_test=600)
+    k, s, seed, lp, lq = 64, 8, 4, 0.05, 0.04
+    with _ctx(u, v, r, m, n) as c:
+        off = c.partition_strata(s)
+        assert np.diff(off).max() <= 128
+        gu, gv, gr = c.get_ratings()
+        c.init_factors(k, prec, seed=seed)
+        P, ps = c.get_factors(0)
+        Q, qs = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.STRATUM, seed=seed, lambda_p=lp, lambda_q=lq, stratum_rounds=rounds),
+                0, 3)
+        G = c.last_launch().lanes_per_sample
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 8)
+    P, Q = np.ascontiguousarray(P).copy(), np.ascontiguousarray(Q).copy()
+    for ep in range(3):
+        for d in range(s):
+            for i in range(s):
+                blk = i * s + (i + d) % s
+                lo, hi = int(off[blk]), int(off[blk + 1])
+                if hi > lo:
+                    oracle.sgd_block_windows(gu[lo:hi], gv[lo:hi], gr[lo:hi], P, Q, lr, ep, blk, 0, 1, k=k, G=G,
+                                             prec=prec, lambda_p=lp, lambda_q=lq, seed=seed, ps=ps, qs=qs)
+        if prec == 3:
+            P, ps = oracle.fp8_rescale(P, ps)
+            Q, qs = oracle.fp8_rescale(Q, qs)
+    assert np.array_equal(P1.view(np.uint8), P.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Q.view(np.uint8))
+    if prec == 3:
+        assert np.array_equal(ps1, ps) and np.array_equal(qs1, qs)
+
+
 def test_stratum_rounds_split_windows_exactly(gpu):
     """Passes over the diagonals (stratum_rounds) split each block's windows
     across the passes without adding any.  With at most one window per block

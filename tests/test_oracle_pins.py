@@ -340,6 +340,35 @@ def test_serial_windows_follow_the_window_draws():
     assert len(set(starts)) > 1
 
 
+@pytest.mark.parametrize("prec", [0, 1, 4])
+def test_block_windows_restate_the_serial_epoch(prec):
+    """One rating block's windows (the stratum schedule's unit) against the
+    serialized epoch: with tag 0 the windows [0, ceil(nnz/L)) are that epoch
+    bit for bit (same draws, same rounding words), split anywhere into two
+    calls they are too, and another block id draws other windows."""
+    rng = np.random.default_rng(11)
+    m, n, k, nnz, L = 40, 30, 16, 700, 32
+    u = rng.integers(0, m, nnz).astype(np.int32)
+    v = rng.integers(0, n, nnz).astype(np.int32)
+    r = rng.uniform(1, 5, nnz).astype(np.float32)
+    P0, _ = oracle.init_factors(m, k, prec, 3, 0)
+    Q0, _ = oracle.init_factors(n, k, prec, 3, 1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 4)
+    kw = dict(k=k, G=4, prec=prec, lambda_p=0.05, lambda_q=0.04, seed=3, batch_len=L)
+    nb = math.ceil(nnz / L)
+    Ps, Qs, _, _ = oracle.sgd_serial(u, v, r, P0, Q0, lr, 1, 2, **kw)
+    P, Q = P0.copy(), Q0.copy()
+    oracle.sgd_block_windows(u, v, r, P, Q, lr, 1, 0, 0, nb, **kw)
+    assert np.array_equal(P, Ps) and np.array_equal(Q, Qs)
+    P2, Q2 = P0.copy(), Q0.copy()
+    oracle.sgd_block_windows(u, v, r, P2, Q2, lr, 1, 0, 0, 9, **kw)
+    oracle.sgd_block_windows(u, v, r, P2, Q2, lr, 1, 0, 9, nb - 9, **kw)
+    assert np.array_equal(P2, Ps) and np.array_equal(Q2, Qs)
+    P3, Q3 = P0.copy(), Q0.copy()
+    oracle.sgd_block_windows(u, v, r, P3, Q3, lr, 1, 5, 0, nb, **kw)
+    assert not np.array_equal(P3, Ps)
+
+
 # ---------------------------------------------------------------- evaluation vs the reference binary
 @pytest.mark.skipif(not oracle.have_mf_predict(), reason="reference mf-predict not built")
 def test_rmse_and_model_file_match_reference_mf_predict(tmp_path):
