@@ -495,6 +495,41 @@ def test_stratum_epochs_match_oracle_when_deterministic(gpu, prec, rounds):
         assert np.array_equal(ps1, ps) and np.array_equal(qs1, qs)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_grid_epochs_match_oracle_with_one_worker(gpu, prec):
+    """The reference's gridded schedule (blocks one after another, here a
+    2 x 3 grid in a caller-given order per epoch, singleGPU/sgd_kernel.cu:
+    249-372) with one worker is deterministic: each block's ceil(size/L)
+    windows, drawn with the block id as tag, in the given block order.  GPU
+    factors after two epochs equal the oracle's bit for bit."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=12, m=500, n=240, nnz=20_000, n_test=0)
+    k, seed, L = 64, 6, 128
+    order = np.array([4, 0, 5, 2, 1, 3, 2, 3, 0, 1, 5, 4], np.int32)   # two epochs of a 2 x 3 grid
+    with _ctx(u, v, r, m, n) as c:
+        off = c.partition(2, 3)
+        gu, gv, gr = c.get_ratings()
+        c.init_factors(k, prec, seed=seed)
+        P, _ = c.get_factors(0)
+        Q, _ = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.GRID, num_workers=1, seed=seed, lambda_p=0.05), 0, 2, order)
+        G = c.last_launch().lanes_per_sample
+        P1, _ = c.get_factors(0)
+        Q1, _ = c.get_factors(1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 6)
+    P, Q = np.ascontiguousarray(P).copy(), np.ascontiguousarray(Q).copy()
+    for ep in range(2):
+        for blk in order[ep * 6:(ep + 1) * 6]:
+            lo, hi = int(off[blk]), int(off[blk + 1])
+            if hi > lo:
+                oracle.sgd_block_windows(gu[lo:hi], gv[lo:hi], gr[lo:hi], P, Q, lr, ep, int(blk), 0, -(-(hi - lo) // L),
+                                         k=k, G=G, prec=prec, lambda_p=0.05, lambda_q=0.05, seed=seed)
+    assert np.array_equal(P1.view(np.uint8), P.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Q.view(np.uint8))
+
+
 def test_stratum_rounds_split_windows_exactly(gpu):
     """Passes over the diagonals (stratum_rounds) split each block's windows
     across the passes without adding any.  With at most one window per block
