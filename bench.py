@@ -112,17 +112,26 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
-def measured_traffic(prec: int, info, schedule: str):
+def measured_traffic(prec: int, info, schedule: str, upl: float | None = None):
     """HBM bytes per launch of the update kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload and schedule (profiles/),
-    when it was measured on the same kernel shape; None otherwise."""
+    when it was measured on the same kernel and launch shape (window length
+    and updates per launch, when the summary records them; the worker budget
+    alone does not change a stratum launch once it exceeds the windows of a
+    diagonal); None otherwise."""
     name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8", 4: "bf16sr"}[prec]
     path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
         return None
     d = json.load(open(path))
     shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}>"
-    return d["traffic_bytes_per_launch"] if shape in d["kernel"] else None
+    if shape not in d["kernel"]:
+        return None
+    if d.get("batch_len", 128) != info.batch_len:   # summaries without it used the default 128
+        return None
+    if upl is not None and "updates_per_launch" in d and abs(d["updates_per_launch"] - upl) > 1e-3 * upl:
+        return None
+    return d["traffic_bytes_per_launch"]
 
 
 def main() -> int:
@@ -151,7 +160,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--depth", type=int, default=0, help="pipeline depth (2, 4, 8; 0 = library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0)
-    ap.add_argument("--autotune", action="store_true", help="pick lanes / depth / workers by timing trial epochs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shape instead of the autotuner's (lanes, depth, "
+                         "window length, workers picked by timing trial epochs before the timed region)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -197,7 +208,7 @@ def main() -> int:
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
                         lanes_per_sample=args.lanes, pipeline_depth=args.depth, stratum_rounds=args.rounds,
                         batch_len=args.batch_len, use_graph=int(not args.no_graph))
-    if args.autotune:
+    if not args.no_autotune:
         t1 = time.perf_counter()
         opts, tuned_ms = ctx.autotune(opts)
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
@@ -299,7 +310,7 @@ def main() -> int:
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info, sched_tag),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info, sched_tag, upd_per_launch),
                      "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upd_per_launch},
         "time_to_target": ttt,
