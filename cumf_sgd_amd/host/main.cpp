@@ -10,6 +10,8 @@
 //   --serial        one worker, deterministic window order
 //   --schedule S    hogwild | grid | stratum (default: stratum for a 1x1 grid, grid otherwise)
 //   --lanes G       lanes per update (8..64)
+//   --rounds R      stratum schedule: passes over the diagonals per epoch
+//                   (s comes from a square -u/-v grid, else 24)
 //   --coherent      device-scope coherent factor accesses
 // With -p the test RMSE is printed after every epoch.  A 1x1 grid streams the
 // files straight into device memory (cumf_sgd_train_file); gridded modes keep
@@ -34,6 +36,7 @@ struct Args {
     int schedule = -1;
     int lanes = 0;
     int coherent = 0;
+    int rounds = 0;
 };
 
 bool is_numerical(const char* s)
@@ -111,6 +114,7 @@ Args parse(int argc, char** argv)
             else throw std::invalid_argument("unknown schedule " + v);
         }
         else if (s == "--lanes") { need("a number after --lanes"); num("--lanes"); a.lanes = std::atoi(argv[i]); }
+        else if (s == "--rounds") { need("a number after --rounds"); num("--rounds"); a.rounds = std::atoi(argv[i]); }
         else if (s == "--coherent") a.coherent = 1;
         else break;
     }
@@ -172,6 +176,7 @@ int main(int argc, char** argv)
     o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID);
     o.lanes_per_sample = a.lanes;
     o.coherent = a.coherent;
+    o.stratum_rounds = a.rounds;
     o.use_graph = 1;   // each epoch's launches replay as one captured graph
     std::vector<double> rmse((size_t)(a.p.num_iters > 0 ? a.p.num_iters : 1), 0.0);
     float b = 0.0f;

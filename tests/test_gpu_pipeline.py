@@ -440,8 +440,9 @@ def test_cli_default_schedule_on_tiny_and_small_inputs(gpu, tmp_path):
     """The command line's 1x1 default (the stratum schedule, 24 x 24 blocks)
     on a 7-rating file with fewer ids than strata and on a small problem
     whose rows fit in the L2s: both train and write the model file; on the
-    small problem the stratum run ends below the Hogwild run of the same
-    command (--schedule hogwild)."""
+    small problem the stratum run, with one or two passes over its diagonals
+    per epoch (--rounds 2), ends below the Hogwild run of the same command
+    (--schedule hogwild)."""
     import cumf_sgd_amd as cs
     from cumf_sgd_amd import datagen
 
@@ -459,12 +460,12 @@ def test_cli_default_schedule_on_tiny_and_small_inputs(gpu, tmp_path):
     datagen.write_triplets(tr_path, u, v, r)
     datagen.write_triplets(te_path, tu, tv, tr)
     last = {}
-    for sched in ("default", "hogwild"):
-        extra = [] if sched == "default" else ["--schedule", "hogwild"]
+    for sched in ("default", "hogwild", "rounds"):
+        extra = {"default": [], "hogwild": ["--schedule", "hogwild"], "rounds": ["--rounds", "2"]}[sched]
         out = subprocess.run([cs.CLI_PATH, "-l", "0.05", "-a", "0.08", "-b", "0.3", "-s", "256", "-k", "64", "-t", "8",
                               "--seed", "4", *extra, "-p", te_path, tr_path, str(tmp_path / f"m_{sched}")],
                              capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, out.stdout + out.stderr
         last[sched] = float([l for l in out.stdout.splitlines() if "test RMSE" in l][-1].split()[-1])
-    print(f"default (stratum) {last['default']:.4f}, hogwild {last['hogwild']:.4f}")
-    assert last["default"] <= last["hogwild"] + 0.005
+    print(f"default (stratum) {last['default']:.4f}, two passes {last['rounds']:.4f}, hogwild {last['hogwild']:.4f}")
+    assert last["default"] <= last["hogwild"] + 0.005 and last["rounds"] <= last["hogwild"] + 0.005
