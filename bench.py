@@ -241,10 +241,20 @@ def main() -> int:
     if not args.no_rmse:
         ttt = {}
         curves = {}
+        shapes = {}
         for p in (0, 1, 2, 4, 3):
-            # the headline format trains with the measured shape, the others
-            # with their defaults (or what the command line fixed)
+            # the headline format trains with the measured shape; each other
+            # format with its own autotuned shape (or the defaults with
+            # --no-autotune, or what the command line fixed)
             w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, args.batch_len)
+            if p != prec and not args.no_autotune:
+                ctx.init_factors(K, p, args.seed)
+                o_p, _ = ctx.autotune(cs.make_opts(k=K, schedule=sched, num_workers=w, lambda_p=float(lam),
+                                                   seed=args.seed, lanes_per_sample=l, pipeline_depth=d,
+                                                   stratum_rounds=args.rounds, batch_len=bl,
+                                                   use_graph=int(not args.no_graph)))
+                w, l, d, bl = o_p.num_workers, o_p.lanes_per_sample, o_p.pipeline_depth, o_p.batch_len
+            shapes[p] = {"lanes_per_sample": l, "pipeline_depth": d, "batch_len": bl or 128}   # 0 = default
             times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl,
                                       rounds=args.rounds, graph=int(not args.no_graph))
             curves[p] = (times, rmses)
@@ -257,6 +267,7 @@ def main() -> int:
                 "epochs_to_target": None if hit is None else hit + 1,
                 "seconds_to_target": None if hit is None else round(float(sum(times[:hit + 1])), 4),
                 "train_seconds_total": round(float(sum(times)), 4),
+                "shape": shapes[p],
             }
         ttt["target_rmse"] = round(target, 5)
         ttt["target_rule"] = f"FP32 GPU test RMSE after {args.rmse_epochs} epochs + 0.005"
