@@ -1,0 +1,30 @@
+"""Host logic of bench.py that runs without a GPU: the algorithmic bytes per
+update and the lookup of measured HBM traffic, which must only be reported
+for the launch shape its profile was taken on."""
+from types import SimpleNamespace
+
+import bench
+
+
+def _info(lanes=16, depth=2, batch_len=32):
+    return SimpleNamespace(lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len)
+
+
+def test_bytes_per_update_at_k128():
+    # 12-byte rating record; P and Q rows of 128 elements read and written
+    assert bench.BYTES_PER_UPDATE[0] == 12 + 2 * 2 * 128 * 4
+    assert bench.BYTES_PER_UPDATE[1] == bench.BYTES_PER_UPDATE[2] == bench.BYTES_PER_UPDATE[4] == 12 + 2 * 2 * 128 * 2
+    # FP8: one byte per element plus the two FP32 row scales read
+    assert bench.BYTES_PER_UPDATE[3] == 12 + 2 * 2 * 128 + 2 * 4
+
+
+def test_traffic_reported_only_for_the_profiled_shape():
+    upl = 4128389.3333333335      # updates per launch of the profiled FP16 stratum-24 run
+    t = bench.measured_traffic(1, _info(), "stratum24", upl)
+    assert t is not None and 5e8 < t < 8e8
+    assert bench.measured_traffic(1, _info(), "stratum24") == t
+    assert bench.measured_traffic(1, _info(lanes=32), "stratum24", upl) is None
+    assert bench.measured_traffic(1, _info(depth=4), "stratum24", upl) is None
+    assert bench.measured_traffic(1, _info(batch_len=128), "stratum24", upl) is None
+    assert bench.measured_traffic(1, _info(), "stratum24", upl * 1.01) is None
+    assert bench.measured_traffic(1, _info(), "stratum7", upl) is None
