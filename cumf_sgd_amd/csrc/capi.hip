@@ -16,6 +16,7 @@
 #include "errors.h"
 #include "loader.h"
 #include "kernels.h"
+#include "train_plan.h"
 
 using namespace cumf;
 
@@ -926,9 +927,9 @@ int cumf_last_launch(cumf_ctx* c, cumf_launch_info* info)
 // ---------------------------------------------------------------------------
 namespace {
 
-struct Shape {
-    int G, S, L, W, blocks, D;
-};
+// (Shape and ShapeInput: train_plan.h)
+
+}  // namespace
 
 // waves_per_cu waves per CU (the default 128, 512 for FP8, measured
 // fastest: short per-worker chains, little tail), then the
@@ -937,10 +938,10 @@ struct Shape {
 constexpr int kDefaultWavesPerCu = 128;
 constexpr int kDefaultWavesPerCuFp8 = 512;   // FP8 and stratum launches: more, shorter workers
 
-int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
+int cumf::default_workers(const ShapeInput& in, int S, int L, int waves_per_cu)
 {
-    const int64_t w0 = (int64_t)c->cu_count * waves_per_cu * S;
-    const int64_t nnz = std::max<int64_t>(1, c->nnz);
+    const int64_t w0 = (int64_t)in.cu_count * waves_per_cu * S;
+    const int64_t nnz = std::max<int64_t>(1, in.nnz);
     const int64_t uc = std::max<int64_t>(1, (nnz + w0 * L - 1) / (w0 * L));
     int64_t w = (nnz + uc * L - 1) / (uc * L);
     const int64_t gran = (int64_t)kSgdWavesPerBlock * S;
@@ -948,12 +949,12 @@ int default_workers(const cumf_ctx* c, int S, int L, int waves_per_cu)
     return (int)std::min<int64_t>(w, 1 << 24);
 }
 
-int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
+int cumf::resolve_shape(const ShapeInput& in, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
 {
-    sh->G = o->lanes_per_sample ? o->lanes_per_sample : sgd_default_group(c->k, c->prec);
-    if (!sgd_shape_supported(c->k, c->prec, sh->G))
-        return fail(CUMF_ERR_ARG, "lanes_per_sample=%d unsupported for k=%d precision=%d", sh->G, c->k, c->prec);
-    if (o->coherent && (c->k / sh->G) * prec_bytes(c->prec) < 4)
+    sh->G = o->lanes_per_sample ? o->lanes_per_sample : sgd_default_group(in.k, in.prec);
+    if (!sgd_shape_supported(in.k, in.prec, sh->G))
+        return fail(CUMF_ERR_ARG, "lanes_per_sample=%d unsupported for k=%d precision=%d", sh->G, in.k, in.prec);
+    if (o->coherent && (in.k / sh->G) * prec_bytes(in.prec) < 4)
         return fail(CUMF_ERR_ARG, "coherent mode needs >= 4 bytes per lane");
     sh->S = 64 / sh->G;
     sh->L = o->batch_len ? o->batch_len : 128;
@@ -973,8 +974,8 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     } else {
         // stratum launches are short: one window per worker (the per-block
         // cap) measured fastest; FP8 likewise
-        const bool one_window = o->schedule == CUMF_SCHED_STRATUM || c->prec == CUMF_FP8_E4M3;
-        W = default_workers(c, sh->S, sh->L, one_window ? kDefaultWavesPerCuFp8 : kDefaultWavesPerCu);
+        const bool one_window = o->schedule == CUMF_SCHED_STRATUM || in.prec == CUMF_FP8_E4M3;
+        W = default_workers(in, sh->S, sh->L, one_window ? kDefaultWavesPerCuFp8 : kDefaultWavesPerCu);
     }
     const int per_block = kSgdWavesPerBlock * sh->S;
     int blocks = (W + per_block - 1) / per_block;
@@ -984,15 +985,31 @@ int resolve_shape(cumf_ctx* c, const cumf_train_opts* o, Shape* sh, int xcd_mult
     return CUMF_OK;
 }
 
-int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
+bool cumf::confine_to_one_xcd(int schedule, int coherent, int64_t m, int64_t n, int k, int prec)
+{
+    if (schedule != CUMF_SCHED_HOGWILD && schedule != CUMF_SCHED_GRID) return false;
+    if (coherent) return false;   // coherent accesses already see one copy of every row
+    return (m + n) * k * prec_bytes(prec) <= kOneXcdFactorBytes;
+}
+
+void cumf::lr_table(const cumf_train_opts* o, int n, float* out)
 {
     // singleGPU/sgd_kernel.cu:137-154: lr[i] = float(alpha/(1+beta*i^1.5) + lrate)
-    std::vector<float> h((size_t)n);
     const float alpha = o->alpha, beta = o->beta, lrate = o->lrate;
     for (int i = 0; i < n; ++i) {
         double t = alpha / (1 + beta * std::pow(i, 1.5)) + lrate;
-        h[i] = (float)t;
+        out[i] = (float)t;
     }
+}
+
+namespace {
+
+ShapeInput shape_input(const cumf_ctx* c) { return ShapeInput{c->k, c->prec, c->cu_count, c->nnz}; }
+
+int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
+{
+    std::vector<float> h((size_t)n);
+    lr_table(o, n, h.data());
     int rc = ensure(&c->lr, &c->lr_cap, (size_t)n);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(c->lr, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
@@ -1035,6 +1052,12 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                const int32_t* block_order, int64_t order_len, std::vector<SgdTask>& tasks,
                std::vector<LaunchRec>& launches, int64_t* upd_epoch)
 {
+    // a launch confined to XCD 0: the grid spans all XCDs (workgroups go
+    // round-robin over them), workers are numbered XCD-major, and the
+    // workgroups of the other XCDs find no task and exit
+    const bool one_xcd = confine_to_one_xcd(o->schedule, o->coherent, c->m, c->n, c->k, c->prec);
+    const int spread = one_xcd ? kNumXcd : 1;
+    const int32_t spread_blocks = one_xcd ? kNumXcd * sh.blocks : 0;
     const int nb = c->ux * c->vy;
     *upd_epoch = 0;
     auto push = [&](std::vector<SgdTask>& t, int nx, int32_t blocks = 0) {
@@ -1045,7 +1068,7 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
     };
     if (o->schedule == CUMF_SCHED_HOGWILD || o->schedule == CUMF_SCHED_SERIAL) {
         std::vector<SgdTask> t{base_task(c, o, 0, c->nnz, sh.W, sh.L, 0)};
-        push(t, 1);
+        push(t, spread, spread_blocks);
     } else if (o->schedule == CUMF_SCHED_GRID) {
         for (int i = 0; i < nb; ++i) {
             int blk = i;
@@ -1053,7 +1076,7 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
             const int64_t beg = c->offsets[blk], end = c->offsets[blk + 1];
             if (end == beg) continue;
             std::vector<SgdTask> t{base_task(c, o, beg, end, sh.W, sh.L, (uint32_t)blk)};
-            push(t, 1);
+            push(t, spread, spread_blocks);
         }
     } else {
         // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
@@ -1084,16 +1107,21 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                 load[(size_t)x] += size_of(blk);
             }
             for (int x = 0; x < num_xcd; ++x) {
-                const std::vector<int>& mine = assigned[(size_t)x];
+                std::vector<int> mine;   // the XCD's non-empty blocks, largest first
+                for (int blk : assigned[(size_t)x])
+                    if (size_of(blk) > 0) mine.push_back(blk);
                 const int64_t tot = load[(size_t)x];
-                int left = per_xcd;
+                // every non-empty block gets at least one worker, also when
+                // the XCD holds more blocks than its share of the workers
+                const int budget = std::max(per_xcd, (int)mine.size());
+                int left = budget;
                 for (size_t j = 0; j < mine.size(); ++j) {
                     const int blk = mine[j];
-                    const int64_t sz = c->offsets[blk + 1] - c->offsets[blk];
+                    const int64_t sz = size_of(blk);
                     int w = j + 1 == mine.size() ? left
-                                                  : (int)std::max<int64_t>(1, tot ? (int64_t)per_xcd * sz / tot : 1);
-                    w = std::min(w, left - (int)(mine.size() - 1 - j));
-                    if (w < 1 || sz == 0) continue;
+                                                  : (int)std::max<int64_t>(1, tot ? (int64_t)budget * sz / tot : 1);
+                    // leave one worker for each block still to place
+                    w = std::max(1, std::min(w, left - (int)(mine.size() - 1 - j)));
                     // no more workers than windows: a small block gets one
                     // window per worker instead of many redundant ones
                     w = (int)std::min<int64_t>(w, (sz + sh.L - 1) / sh.L);
@@ -1102,6 +1130,12 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                         Placed{base_task(c, o, c->offsets[blk], c->offsets[blk + 1], w, sh.L, (uint32_t)blk), x});
                 }
             }
+            // each non-empty block of the diagonal is trained exactly once
+            size_t nonempty = 0;
+            for (int blk : order) nonempty += size_of(blk) > 0;
+            if (diag[(size_t)d].size() != nonempty)
+                return fail(CUMF_ERR_STATE, "stratum plan placed %zu of %zu blocks of diagonal %d",
+                            diag[(size_t)d].size(), nonempty, d);
         }
         // A launch's grid holds just its workers: XCD x numbers its workers
         // [x * P, x * P + its count), P the largest count rounded up to whole
@@ -1292,7 +1326,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         return fail(CUMF_ERR_STATE, "stratum schedule needs an s x s partition (have %d x %d)", c->ux, c->vy);
     const int num_xcd = stratum ? 8 : 1;
     Shape sh;
-    if ((rc = resolve_shape(c, o, &sh, num_xcd))) return rc;
+    if ((rc = resolve_shape(shape_input(c), o, &sh, num_xcd))) return rc;
     if (stratum) sh.W = sh.blocks * kSgdWavesPerBlock * sh.S;   // every wave of every XCD works
     if ((rc = upload_lr(c, o, e1 + 4))) return rc;
 
@@ -1360,12 +1394,23 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     if (c->streamed) {
         if ((rc = train_streamed(c, o, sh, plan[0], tasks, soff, e0, e1))) return rc;
     } else if (o->use_graph && !per_epoch_order) {
-        // the epoch loop as a graph: captured once per plan, replayed per epoch
+        // the epoch loop as a graph: captured once per plan, replayed per epoch.
+        // The key covers everything the captured launches hold by value: the
+        // task table's contents and its device address (a larger plan
+        // reallocates it), the epoch counter's address, the launch shapes,
+        // the factor format and length and the row counts the FP8 rescale
+        // walks.
+        if ((rc = ensure(&c->epoch_dev, &c->epoch_dev_cap, 1))) return rc;
         uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
-        const int32_t shape[8] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec};
+        const int32_t shape[11] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec, c->k, c->m, c->n};
         key = fnv1a(shape, sizeof(shape), key);
         key = fnv1a(&o->seed, sizeof(o->seed), key);
-        if ((rc = ensure(&c->epoch_dev, &c->epoch_dev_cap, 1))) return rc;
+        const void* addr[7] = {c->tasks, c->epoch_dev, c->P, c->Q, c->ps, c->qs, c->lr};
+        key = fnv1a(addr, sizeof(addr), key);
+        for (const LaunchRec& l : plan[0]) {
+            const int64_t f[4] = {l.task_off, l.task_cnt, l.num_xcd, l.blocks};
+            key = fnv1a(f, sizeof(f), key);
+        }
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) {
                 (void)hipGraphExecDestroy(c->graph_exec);
@@ -1468,7 +1513,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
                     o.pipeline_depth = D;
                     o.batch_len = L;
                     o.num_workers = base->num_workers ? base->num_workers
-                                                      : default_workers(c, 64 / G, L, waves_per_cu[wi]);
+                                                      : default_workers(shape_input(c), 64 / G, L, waves_per_cu[wi]);
                     o.use_graph = 0;
                     if (!warm) {   // first launch of the code object is not timed
                         if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;

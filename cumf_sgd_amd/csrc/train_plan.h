@@ -1,0 +1,50 @@
+// Launch-shape resolution and the learning-rate table, shared by the
+// context's training entry points (capi.hip) and the allocation-free one
+// (noalloc.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/cumf_sgd_amd.h"
+
+namespace cumf {
+
+// The resolved shape of an update launch: G lanes per sample, S = 64/G
+// samples per wave, L records per window, W workers, `blocks` workgroups,
+// D row pairs in flight per worker.
+struct Shape {
+    int G, S, L, W, blocks, D;
+};
+
+// What the shape depends on besides the options: the factor length and
+// format, the device's CU count and the rating count.
+struct ShapeInput {
+    int k, prec, cu_count;
+    int64_t nnz;
+};
+
+// The default worker count for waves_per_cu waves per CU: the fewest workers
+// that keep the same windows per worker (update_count = ceil(nnz / (W * L)),
+// singleGPU/sgd_kernel.cu:203), in whole workgroups.
+int default_workers(const ShapeInput& in, int S, int L, int waves_per_cu);
+
+// Resolve opts' zero fields to the defaults and check the shape; the grid is
+// rounded up to a multiple of xcd_multiple workgroups.  Returns CUMF_OK or
+// CUMF_ERR_ARG (with the last-error text set).
+int resolve_shape(const ShapeInput& in, const cumf_train_opts* o, Shape* sh, int xcd_multiple);
+
+// Hogwild and gridded launches whose factors (P and Q together) fit in the
+// XCDs' L2s run on one XCD.  Device memory is not kept coherent between the
+// eight XCD L2s within a kernel: each L2 keeps its own copy of a row it has
+// touched until the kernel ends, so on factors that small the workers of
+// different XCDs would train diverging copies of the hot rows for a whole
+// launch.  Larger factors are evicted long before that and train on all XCDs.
+constexpr int kNumXcd = 8;
+constexpr int64_t kOneXcdFactorBytes = 32ll << 20;   // 8 XCDs x 4 MB of L2
+bool confine_to_one_xcd(int schedule, int coherent, int64_t m, int64_t n, int k, int prec);
+
+// lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
+// (singleGPU/sgd_kernel.cu:137-154).
+void lr_table(const cumf_train_opts* o, int n, float* out);
+
+}  // namespace cumf

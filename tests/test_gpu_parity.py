@@ -669,3 +669,76 @@ def test_empty_ratings_and_test_set(gpu):
         assert c.kernel_time().launches == 0
         c.set_test(e, e, np.zeros(0, np.float32))
         assert c.test_rmse() == 0.0
+
+
+def test_graph_cache_follows_factor_length_and_task_table(gpu):
+    """The captured epoch graph holds the factor length, the row counts and the
+    task table's device address by value, so a graph captured at k = 128 must
+    not be replayed after the factors are re-initialised at k = 64 with the
+    same explicit launch shape, nor after a larger plan (a gridded call)
+    reallocated the task table: graph replays equal eager epochs bit for bit
+    on both sequences (serialized schedule, deterministic)."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=20_000)
+
+    def opts(k, graph):
+        return cs.make_opts(k=k, schedule=cs.SERIAL, seed=3, lanes_per_sample=16, batch_len=32, use_graph=graph)
+
+    # k changes under an unchanged allocation
+    out = {}
+    for graph in (0, 1):
+        with _ctx(u, v, r, m, n) as c:
+            c.init_factors(128, cs.FP16, seed=4)
+            c.train(opts(128, graph), 0, 1)
+            c.init_factors(64, cs.FP16, seed=5)
+            c.train(opts(64, graph), 0, 2)
+            out[graph] = (c.get_factors(0)[0].copy(), c.get_factors(1)[0].copy())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+    # the task table grows between two graph calls whose own tables are equal
+    nb = 4
+    offs = np.linspace(0, len(u), nb + 1).astype(np.int64)
+    out = {}
+    for graph in (0, 1):
+        with _ctx(u, v, r, m, n) as c:
+            c.init_factors(64, cs.FP32, seed=6)
+            c.train(opts(64, graph), 0, 1)
+            c.set_blocks(2, 2, offs)
+            c.train(cs.make_opts(k=64, schedule=cs.GRID, num_workers=64, seed=3, batch_len=32), 1, 2)
+            P0, _ = c.get_factors(0)
+            Q0, _ = c.get_factors(1)
+            c.set_factors(0, P0)
+            c.set_factors(1, Q0)
+            c.train(opts(64, graph), 2, 4)
+            out[graph] = (P0, Q0, c.get_factors(0)[0].copy(), c.get_factors(1)[0].copy())
+    # both runs start the last call from their own (Hogwild) state: replay it eagerly
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(64, cs.FP32, seed=6)
+        c.set_factors(0, out[1][0])
+        c.set_factors(1, out[1][1])
+        c.train(opts(64, 0), 2, 4)
+        assert np.array_equal(c.get_factors(0)[0], out[1][2]) and np.array_equal(c.get_factors(1)[0], out[1][3])
+
+
+def test_stratum_plan_trains_every_block_with_few_workers(gpu):
+    """More blocks per XCD than workers per XCD (one-lane-group waves, 32
+    workers, 40 x 40 strata: five blocks of each diagonal per XCD, four
+    workers): every non-empty block still gets a worker, so an epoch performs
+    at least one update per rating, and training converges."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(nnz=40_000)
+    with _ctx(u, v, r, m, n) as c:
+        c.set_test(tu, tv, tr)
+        off = c.partition_strata(40)
+        c.init_factors(64, cs.FP32, seed=8)
+        before = c.test_rmse(1.0, 3.0)
+        o = cs.make_opts(k=64, schedule=cs.STRATUM, num_workers=32, lanes_per_sample=64, seed=8, batch_len=32)
+        c.train(o, 0, 3)
+        info = c.last_launch()
+        nonempty = int(np.count_nonzero(np.diff(off)))
+        assert info.updates_per_epoch >= len(u)
+        # one worker of one window at most per block beyond its records
+        assert info.updates_per_epoch < len(u) + nonempty * 32 * 2
+        assert c.test_rmse(1.0, 3.0) < before - 0.05

@@ -323,9 +323,10 @@ def test_serial_sgd_fits_a_rank_one_problem():
 
 
 def test_serial_windows_follow_the_window_draws():
-    """Each epoch walks ceil(nnz/L) windows of L records at the drawn starts:
-    with lr = 0 nothing changes; with one record per window the update count
-    per record equals the number of windows starting on it."""
+    """With lr = 0 an epoch changes nothing (the decay 1 - lr*lambda is then
+    exactly 1), and the window starts of an epoch differ.  Which records a
+    window visits is brute-forced in
+    test_serial_windows_visit_L_consecutive_records_modulo_nnz."""
     rng = np.random.default_rng(10)
     m, n, k, nnz = 20, 10, 4, 50
     u = rng.integers(0, m, nnz).astype(np.int32)
@@ -439,3 +440,131 @@ def test_evaluation_criteria_match_reference_mf_predict(tmp_path):
         for crit in (0, 1, 2, 5, 6):
             assert oracle.mf_predict_eval(test, model, crit) == _fmt4(
                 oracle.evaluate(crit, tu, tv, tr, P, Q, np.float32(2.5))), (trial, crit)
+
+
+# ---------------------------------------------------------------- window semantics, remap, rescale, model scale
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 7])
+def test_serial_windows_visit_L_consecutive_records_modulo_nnz(L):
+    """Brute force of the window semantics (singleGPU/sgd_k128_kernel_hogwild_warp32.h:41-56,
+    worker count 1, update_count = ceil(nnz/L), sgd_kernel.cu:197-208): every
+    record has its own user and item, q = 0, r = 0, lambda = 1 and lr = 0.5,
+    so an update only multiplies the record's p by 1 - lr*lambda = 1/2 and
+    p ends at exactly 2^-(number of updates of that record).  The counts
+    must equal those of the drawn windows: window b of epoch ep covers the
+    records (start_b + i) mod nnz, i = 0..L-1."""
+    nnz, k, seed, epochs = 23, 2, 7 + L, 2
+    u = np.arange(nnz, dtype=np.int32)
+    v = np.arange(nnz, dtype=np.int32)
+    r = np.zeros(nnz, np.float32)
+    P0 = np.ones((nnz, k), np.float32)
+    Q0 = np.zeros((nnz, k), np.float32)
+    lr = np.full(epochs, 0.5, np.float32)
+    P1, Q1, _, _ = oracle.sgd_serial(u, v, r, P0, Q0, lr, 0, epochs, k=k, G=2, prec=oracle.F32, lambda_p=1.0,
+                                     lambda_q=1.0, seed=seed, batch_len=L)
+    assert np.array_equal(Q1, Q0)
+    assert np.all(P1[:, 0] == P1[:, 1])
+    got = -np.log2(P1[:, 0].astype(np.float64))
+    assert np.array_equal(got, np.round(got))
+    want = np.zeros(nnz, np.int64)
+    for ep in range(epochs):
+        for b in range(-(-nnz // L)):
+            s = oracle.window_start(seed, 0, 0, 0, ep, b, nnz)
+            for i in range(L):
+                want[(s + i) % nnz] += 1
+    assert want.sum() == epochs * L * -(-nnz // L)
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+def test_serial_windows_walk_records_in_order():
+    """Within a window the records are applied in index order: two records of
+    the same user and item, r = 1 then r = 3 (lambda 0, lr 1/2, k = 1 lane
+    group of 2 with q = (1, 0), p = (0, 0)): the first update gives
+    p0 = 1/2, q unchanged (p was 0); the second sees the dot 1/2, e = 5/2
+    and moves p0 by 5/4 and q0 by 5/8.  The reverse order ends elsewhere."""
+    u = np.zeros(2, np.int32)
+    v = np.zeros(2, np.int32)
+    P0 = np.zeros((1, 2), np.float32)
+    Q0 = np.array([[1.0, 0.0]], np.float32)
+    lr = np.full(1, 0.5, np.float32)
+    kw = dict(k=2, G=2, prec=oracle.F32, lambda_p=0.0, lambda_q=0.0, batch_len=2)
+    # pick a seed whose single window starts at record 0 (order 0, 1)
+    seed = next(s for s in range(1000) if oracle.window_start(s, 0, 0, 0, 0, 0, 2) == 0)
+    P1, Q1, _, _ = oracle.sgd_serial(u, v, np.float32([1, 3]), P0, Q0, lr, 0, 1, seed=seed, **kw)
+    assert P1[0].tolist() == [0.5 + 1.25, 0.0] and Q1[0].tolist() == [1.0 + 0.625, 0.0]
+    P2, Q2, _, _ = oracle.sgd_serial(u, v, np.float32([3, 1]), P0, Q0, lr, 0, 1, seed=seed, **kw)
+    # r = 3 first: p0 = 3/2, then dot 3/2, e = -1/2: p0 = 3/2 - 1/4, q0 = 1 - 3/8
+    assert P2[0].tolist() == [1.25, 0.0] and Q2[0].tolist() == [0.625, 0.0]
+
+
+def test_shuffle_problem_is_the_gather_by_the_maps():
+    """shuffle_problem (singleGPU/sgd.cu:126-134) against numpy's gather
+    u <- p_map[u], v <- q_map[v], values untouched."""
+    rng = np.random.default_rng(21)
+    m, n, nnz = 1000, 300, 20_000
+    nodes = np.zeros(nnz, oracle.NODE_DTYPE)
+    nodes["u"] = rng.integers(0, m, nnz)
+    nodes["v"] = rng.integers(0, n, nnz)
+    nodes["rate"] = rng.uniform(1, 5, nnz).astype(np.float32)
+    pm = rng.permutation(m).astype(np.int32)
+    qm = rng.permutation(n).astype(np.int32)
+    out = oracle.shuffle_problem(nodes, pm, qm)
+    assert np.array_equal(out["u"], pm[nodes["u"]])
+    assert np.array_equal(out["v"], qm[nodes["v"]])
+    assert np.array_equal(out["rate"].view(np.uint32), nodes["rate"].view(np.uint32))
+
+
+def test_scale_model_multiplies_by_float_sqrt_of_scale():
+    """scale_model (singleGPU/sgd.cu:593-606): every element times
+    float(sqrt(scale)), against numpy's correctly rounded float32 sqrt."""
+    rng = np.random.default_rng(22)
+    vec = rng.normal(0, 1, (37, 16)).astype(np.float32)
+    for scale in (1.0, 1.08918, 34.6, 1e-4, 2.0):
+        fs = np.sqrt(np.float32(scale))
+        assert np.array_equal(oracle.scale_model(vec, scale), vec * fs)
+
+
+def _e4m3_values():
+    torch = pytest.importorskip("torch")
+    codes = np.arange(256, dtype=np.uint8)
+    vals = torch.from_numpy(codes).view(torch.float8_e4m3fn).float().numpy()
+    return torch, codes, vals
+
+
+def test_fp8_rescale_against_torch_e4m3():
+    """The between-epoch FP8 rescale against facts outside the oracle: the new
+    power-of-two scale puts the row's largest magnitude in (28, 56]; the new
+    bytes are torch's round-to-nearest-even float8_e4m3fn of old value / new
+    scale; when the scale does not grow every value is kept exactly (a power
+    of two times an E4M3 value below 56 is an E4M3 value); when it grows each
+    value moves by at most half an E4M3 quantum at the new scale; all-zero
+    rows are left alone."""
+    torch, codes, vals = _e4m3_values()
+    rng = np.random.default_rng(23)
+    rows, k = 400, 32
+    finite = codes[np.isfinite(vals)]
+    data = rng.choice(finite, (rows, k)).astype(np.uint8)
+    # rows of small magnitude too (scale must grow or shrink by several octaves)
+    small = np.abs(vals[finite]) <= 2
+    data[::3] = rng.choice(finite[small], (len(range(0, rows, 3)), k))
+    data[5] = 0                                  # an all-zero row
+    scales = np.ldexp(np.float32(1), rng.integers(-20, 8, rows)).astype(np.float32)
+    old = vals[data] * scales[:, None]
+    new_d, new_s = oracle.fp8_rescale(data, scales)
+    assert new_s[5] == scales[5] and np.array_equal(new_d[5], data[5])
+    live = np.abs(old).max(1) > 0
+    amax = np.abs(old).max(1)
+    assert np.all(np.log2(new_s[live]) == np.round(np.log2(new_s[live])))
+    assert np.all(amax[live] / new_s[live] <= 56) and np.all(amax[live] / new_s[live] > 28)
+    want = torch.from_numpy((old / new_s[:, None]).astype(np.float32)).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(new_d[live], want[live])
+    got = vals[new_d] * new_s[:, None]
+    shrink = live & (new_s <= scales)
+    assert np.array_equal(got[shrink], old[shrink])
+    grow = live & (new_s > scales)
+    assert grow.sum() > 50 and shrink.sum() > 50
+    # half a quantum at the new scale: E4M3 has 3 mantissa bits, subnormal
+    # quantum 2^-9
+    x = np.abs(old[grow] / new_s[grow, None])
+    e = np.maximum(np.floor(np.log2(np.where(x > 0, x, 1))), -6)
+    quantum = np.ldexp(1.0, (e - 3).astype(int)) * new_s[grow, None]
+    assert np.all(np.abs(got[grow] - old[grow]) <= quantum / 2)
