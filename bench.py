@@ -10,16 +10,21 @@ by default under the stratum schedule (24 x 24 nnz-balanced blocks, the 24
 conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
 the least-loaded one, one launch per diagonal; --schedule hogwild runs the
 reference's 1x1 mode).  Each epoch's launches are captured once as a HIP
-graph and replayed (--no-graph launches them one by one).
+graph and replayed (--no-graph launches them one by one).  The launch shape
+is fixed (the library's defaults for the format with 32-record windows, which
+is what cumf_autotune picks on this workload), so that the profiled kernel is
+the timed one; --autotune runs the autotuner first instead.
 Inputs are resident in HBM before the timed region starts.
 
 Printed: one JSON line with value (rating updates/s of the timed epochs),
 roofline of the update kernel (algorithmic bytes per launch over its
-HIP-event duration; with graph replay the events bracket each epoch's graph,
-so the average launch includes the short gaps between launches), time-to-target
-RMSE for each factor format against the
-FP32 GPU run, and the reference's CPU trainer (test/mf-train, built from the
-reference sources into oracle/_ref) timed on a bounded sample on the host.
+HIP-event duration against the L2 bandwidth that serves them, and the
+measured HBM-side traffic of the same launch shape against the HBM peak;
+with graph replay the events bracket each epoch's graph, so the average
+launch includes the short gaps between launches), time-to-target RMSE for
+each factor format against the FP32 GPU run, and the reference's CPU trainer
+(test/mf-train, built from the reference sources into oracle/_ref) timed on
+the host's cores, or the CPU oracle when that build is absent.
 """
 from __future__ import annotations
 
@@ -37,6 +42,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Aggregate L2 bandwidth of MI355X: 8 XCDs x 16 L2 channels x 128 bytes per
+# clock x 2.4 GHz peak engine clock (16 channels of 128 B/clk per XCD L2: AMD
+# CDNA 3 architecture white paper; CDNA 4 keeps the 4 MB, 16-channel L2 per
+# XCD).  Every row load and store of an update is an L2 request, so this is
+# the level that serves the algorithmic bytes.
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4
 K = 128
 # algorithmic bytes per update at k=128: 12-byte rating + P and Q rows read and
 # written (+ the two FP32 row scales read, FP8)
@@ -86,12 +97,42 @@ def run_epochs(ctx, prec, epochs, scale, b, lam, seed, workers, lanes, schedule,
     return times, rmses
 
 
+def host_threads() -> tuple[int, dict]:
+    """Threads for the CPU baseline: the CPUs this process may run on (its
+    affinity mask), capped by OMP_NUM_THREADS when the host sets it (the
+    share of the host's cores a one-GPU lease gets)."""
+    aff = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(aff, env) if env > 0 else aff
+    return max(1, threads), {"nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": env or None}
+
+
+def cpu_baseline_port(R, sample: int, epochs: int):
+    """No reference build on this host: the CPU oracle's serialized FP32 SGD
+    (the GPU kernel's update rule, one thread) on a bounded sample."""
+    import oracle
+
+    u, v, r = (t[:sample].cpu().numpy() for t in (R.u, R.v, R.r))
+    m, n = R.m, R.n
+    P, _ = oracle.init_factors(m, K, oracle.F32, 2, 0)
+    Q, _ = oracle.init_factors(n, K, oracle.F32, 2, 1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, epochs + 4)
+    t0 = time.perf_counter()
+    oracle.sgd_serial(u, v, r, P, Q, lr, 0, epochs, k=K, G=16, prec=oracle.F32, lambda_p=0.05, lambda_q=0.05,
+                      seed=2, batch_len=128)
+    dt = time.perf_counter() - t0
+    return {"value": sample * epochs / dt, "unit": "rating-updates/s", "cores": 1, "kind": "port",
+            "sample": f"the CPU oracle's serialized FP32 SGD (this kernel's update rule, one thread) on the first "
+                      f"{sample} training ratings, {epochs} epochs, k={K} ({dt:.1f} s); the reference's "
+                      f"test/mf-train was not built on this host"}
+
+
 def cpu_baseline(R, threads: int, sample: int, iters: int):
     """The reference's CPU trainer on a bounded sample of the training set."""
     import oracle
 
     if not os.access(oracle.MF_TRAIN, os.X_OK):
-        return None
+        return cpu_baseline_port(R, min(sample, 20_000_000), 1)
     from cumf_sgd_amd import datagen
 
     u, v, r = (t[:sample].cpu().numpy() for t in (R.u, R.v, R.r))
@@ -112,13 +153,12 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
-def measured_traffic(prec: int, info, schedule: str, upl: float | None = None):
-    """HBM bytes per launch of the update kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload and schedule (profiles/),
-    when it was measured on the same kernel and launch shape (window length
-    and updates per launch, when the summary records them; the worker budget
-    alone does not change a stratum launch once it exceeds the windows of a
-    diagonal); None otherwise."""
+def profile_summary(prec: int, info, schedule: str, upl: float | None = None) -> dict | None:
+    """The committed rocprofv3 counter summary (profiles/) of the update
+    kernel on this workload and schedule, when it was measured on the same
+    kernel and launch shape (window length and updates per launch, when the
+    summary records them; the worker budget alone does not change a stratum
+    launch once it exceeds the windows of a diagonal); None otherwise."""
     name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8", 4: "bf16sr"}[prec]
     path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
@@ -131,7 +171,42 @@ def measured_traffic(prec: int, info, schedule: str, upl: float | None = None):
         return None
     if upl is not None and "updates_per_launch" in d and abs(d["updates_per_launch"] - upl) > 1e-3 * upl:
         return None
-    return d["traffic_bytes_per_launch"]
+    return d
+
+
+def measured_traffic(prec: int, info, schedule: str, upl: float | None = None):
+    """HBM-side bytes per launch of the update kernel (calibrated FETCH_SIZE +
+    WRITE_SIZE) from the matching profile summary; None otherwise."""
+    d = profile_summary(prec, info, schedule, upl)
+    return None if d is None else d["traffic_bytes_per_launch"]
+
+
+def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float) -> dict:
+    """Roofline of the update kernel.  `achieved` is the algorithmic bytes
+    (every row read and written, the rating record) per launch over the
+    launch's HIP-event duration.  Those bytes are L2 requests: at the L2 the
+    kernel's fraction is achieved / L2 peak.  The HBM side moves only what
+    the L2s miss or write back: its fraction is the measured traffic (the
+    profile of the same launch shape) over the 8 TB/s HBM peak.  `bound` is
+    the level with the larger fraction."""
+    algo = BYTES_PER_UPDATE[prec] * upl / avg_launch_s / 1e9
+    prof = profile_summary(prec, info, schedule, upl)
+    traffic = None if prof is None else prof["traffic_bytes_per_launch"]
+    levels = {"l2": {"achieved": algo, "peak": L2_PEAK_GBS, "frac": algo / L2_PEAK_GBS,
+                     "bytes": "algorithmic (12-byte record + P and Q rows read and written)"}}
+    if traffic is not None:
+        hbm = traffic / avg_launch_s / 1e9
+        levels["hbm"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS,
+                         "bytes": "measured: calibrated FETCH_SIZE + WRITE_SIZE per launch"}
+    if prof is not None and "l2_request_bytes_per_launch" in prof:
+        req = prof["l2_request_bytes_per_launch"] / avg_launch_s / 1e9
+        levels["l2"]["measured_requests_gbs"] = req
+    bound = max(levels, key=lambda k: levels[k]["frac"])
+    return {"bound": bound, "achieved": levels[bound]["achieved"], "peak": levels[bound]["peak"], "unit": "GB/s",
+            "frac": levels[bound]["frac"], "traffic": traffic, "levels": levels, "effective_algorithmic_gbs": algo,
+            "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+            "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upl,
+            "profile": None if prof is None else prof.get("source")}
 
 
 def main() -> int:
@@ -148,7 +223,9 @@ def main() -> int:
     ap.add_argument("--strata", type=int, default=24, help="s for the stratum schedule (s x s blocks)")
     ap.add_argument("--rounds", type=int, default=0,
                     help="stratum schedule: passes over the diagonals per epoch (0 = library default)")
-    ap.add_argument("--batch-len", type=int, default=0, help="records per window (even, <= 128; 0 = 128)")
+    ap.add_argument("--batch-len", type=int, default=32,
+                    help="records per window (even, <= 128; 0 = the library's 128); 32 is what cumf_autotune "
+                         "picks on this workload")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each epoch's kernels one by one instead of replaying them as a captured HIP graph")
     ap.add_argument("--rmse-epochs", type=int, default=20)
@@ -160,9 +237,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--depth", type=int, default=0, help="pipeline depth (2, 4, 8; 0 = library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0)
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default launch shape instead of the autotuner's (lanes, depth, "
-                         "window length, workers picked by timing trial epochs before the timed region)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick lanes, depth, window length and workers with cumf_autotune (trial epochs before the "
+                         "timed region) instead of the fixed shape: the library's defaults with --batch-len, which "
+                         "is the autotuner's pick on this workload and keeps the profiled kernel shape fixed")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -208,7 +286,7 @@ def main() -> int:
     opts = cs.make_opts(k=K, schedule=sched, num_workers=args.workers, lambda_p=float(lam), seed=args.seed,
                         lanes_per_sample=args.lanes, pipeline_depth=args.depth, stratum_rounds=args.rounds,
                         batch_len=args.batch_len, use_graph=int(not args.no_graph))
-    if not args.no_autotune:
+    if args.autotune:
         t1 = time.perf_counter()
         opts, tuned_ms = ctx.autotune(opts)
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
@@ -244,10 +322,10 @@ def main() -> int:
         shapes = {}
         for p in (0, 1, 2, 4, 3):
             # the headline format trains with the measured shape; each other
-            # format with its own autotuned shape (or the defaults with
-            # --no-autotune, or what the command line fixed)
+            # format with its own autotuned shape (with --autotune), else with
+            # the library's defaults for that format and the fixed window length
             w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, args.batch_len)
-            if p != prec and not args.no_autotune:
+            if p != prec and args.autotune:
                 ctx.init_factors(K, p, args.seed)
                 o_p, _ = ctx.autotune(cs.make_opts(k=K, schedule=sched, num_workers=w, lambda_p=float(lam),
                                                    seed=args.seed, lanes_per_sample=l, pipeline_depth=d,
@@ -289,12 +367,15 @@ def main() -> int:
                                                                  else round(float(sum(times[:hit + 1])), 4))
 
     # ---- CPU baseline (rank 0, N=1 only)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
+    threads = args.cpu_threads or threads
     cpu = None
     if args.cpu_sample != 0:
         try:
             sample = R.nnz if args.cpu_sample < 0 else min(args.cpu_sample, R.nnz)
             cpu = cpu_baseline(R, threads, sample, args.cpu_iters)
+            if cpu is not None:
+                cpu.update(host)
         except Exception as e:  # the baseline never decides the GPU result
             log(f"cpu baseline failed: {e}")
 
@@ -317,13 +398,13 @@ def main() -> int:
                    "precision": args.precision,
                    "factor_storage": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3 + f32 row scale",
                                       4: "bf16 (stochastic rounding)"}[prec],
-                   "schedule": sched_tag, "graph_replay": not args.no_graph, "workers": info.num_workers,
+                   "schedule": sched_tag, "graph_replay": not args.no_graph,
+                   "shape_source": "cumf_autotune" if args.autotune else
+                   "fixed: library defaults with the given window length (cumf_autotune's pick on this workload)",
+                   "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(prec, info, sched_tag, upd_per_launch),
-                     "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
-                     "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upd_per_launch},
+        "roofline": roofline(prec, info, sched_tag, avg_launch_s, upd_per_launch),
         "time_to_target": ttt,
         "cpu_baseline": cpu,
     }

@@ -29,7 +29,7 @@ PREDICT_PATH = os.path.join(_HERE, "cumf_predict")
 
 FP32, FP16, BF16, FP8_E4M3, BF16_SR = 0, 1, 2, 3, 4
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3, "bf16sr": BF16_SR}
-HOGWILD, SERIAL, GRID, STRATUM = 0, 1, 2, 3
+AUTO, HOGWILD, SERIAL, GRID, STRATUM = -1, 0, 1, 2, 3
 # mf-predict -e criteria (test/mf.h:30-40)
 EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
 EVAL_ROW_MPR, EVAL_COL_MPR, EVAL_ROW_AUC, EVAL_COL_AUC = 10, 11, 12, 13
@@ -156,6 +156,7 @@ def _load():
         "cumf_batch_autotune": ([P, ctypes.POINTER(TrainOpts), ctypes.POINTER(TrainOpts), D], ctypes.c_int),
         "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
         "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
+        "cumf_default_schedule": ([P, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -376,6 +377,14 @@ class Context:
         la, up = ctypes.c_int64(), ctypes.c_int64()
         _check(lib.cumf_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)), "cumf_kernel_time")
         return KernelTime(ms.value, la.value, up.value)
+
+    def default_schedule(self, k: int, precision: int) -> tuple[int, int]:
+        """cumf_default_schedule: (schedule, stratum_rounds) that a 1x1 grid
+        trains by default for these ratings."""
+        out, rounds = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.cumf_default_schedule(self._h, k, precision, ctypes.byref(out), ctypes.byref(rounds)),
+               "cumf_default_schedule")
+        return out.value, rounds.value
 
     def autotune(self, base: TrainOpts):
         """Fastest launch shape for this problem (factors are left unchanged);

@@ -618,6 +618,32 @@ int cumf_partition(cumf_ctx* c, int32_t ux, int32_t vy, int64_t* offsets)
     return partition_with(c, bu, bv, offsets);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Ratings per row of P (hu) and of Q (hv), counted on the device.
+int rating_degrees(cumf_ctx* c, std::vector<int32_t>& hu, std::vector<int32_t>& hv)
+{
+    DevTmp du, dv;
+    HIP_TRY(hipMalloc(&du.p, sizeof(int32_t) * (size_t)std::max(1, c->m)));
+    HIP_TRY(hipMalloc(&dv.p, sizeof(int32_t) * (size_t)std::max(1, c->n)));
+    HIP_TRY(hipMemsetAsync(du.p, 0, sizeof(int32_t) * (size_t)c->m, c->stream));
+    HIP_TRY(hipMemsetAsync(dv.p, 0, sizeof(int32_t) * (size_t)c->n, c->stream));
+    HIP_TRY(launch_degree(c->u, c->nnz, (int32_t*)du.p, c->m, c->stream));
+    HIP_TRY(launch_degree(c->v, c->nnz, (int32_t*)dv.p, c->n, c->stream));
+    hu.assign((size_t)c->m, 0);
+    hv.assign((size_t)c->n, 0);
+    HIP_TRY(hipMemcpyAsync(hu.data(), du.p, sizeof(int32_t) * c->m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hv.data(), dv.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CUMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
 {
     int rc = check_ctx(c);
@@ -625,18 +651,46 @@ int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
     if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (s < 1 || s > 64) return fail(CUMF_ERR_ARG, "strata %d out of range 1..64", s);
-    DevTmp du, dv;
-    HIP_TRY(hipMalloc(&du.p, sizeof(int32_t) * (size_t)c->m));
-    HIP_TRY(hipMalloc(&dv.p, sizeof(int32_t) * (size_t)c->n));
-    HIP_TRY(hipMemsetAsync(du.p, 0, sizeof(int32_t) * (size_t)c->m, c->stream));
-    HIP_TRY(hipMemsetAsync(dv.p, 0, sizeof(int32_t) * (size_t)c->n, c->stream));
-    HIP_TRY(launch_degree(c->u, c->nnz, (int32_t*)du.p, c->m, c->stream));
-    HIP_TRY(launch_degree(c->v, c->nnz, (int32_t*)dv.p, c->n, c->stream));
-    std::vector<int32_t> hu(c->m), hv(c->n);
-    HIP_TRY(hipMemcpyAsync(hu.data(), du.p, sizeof(int32_t) * c->m, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hv.data(), dv.p, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> hu, hv;
+    if ((rc = rating_degrees(c, hu, hv))) return rc;
     return partition_with(c, balanced_bounds(hu, s, c->nnz), balanced_bounds(hv, s, c->nnz), offsets);
+}
+
+int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* schedule, int32_t* stratum_rounds)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!schedule || !stratum_rounds) return fail(CUMF_ERR_ARG, "null output");
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (k != 32 && k != 64 && k != 128) return fail(CUMF_ERR_ARG, "k=%d unsupported (32, 64 or 128)", k);
+    if (precision < CUMF_FP32 || precision > CUMF_BF16_SR) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
+    *stratum_rounds = 1;
+    // streamed ratings train only as strata, one pass
+    if (c->streamed) {
+        *schedule = CUMF_SCHED_STRATUM;
+        return CUMF_OK;
+    }
+    // factors that fit in the L2s: conflict-free strata (no row is cached
+    // by two XCDs within a launch) with four passes over the diagonals per
+    // epoch, so that a row meets every range of the other axis four times
+    // per epoch; a small problem's launches are short, the passes cost little
+    if (confine_to_one_xcd(CUMF_SCHED_HOGWILD, 0, c->m, c->n, k, precision)) {
+        *schedule = CUMF_SCHED_STRATUM;
+        *stratum_rounds = kSmallFactorRounds;
+        return CUMF_OK;
+    }
+    // a row and a column that each carry a large share of the ratings: one
+    // stratum diagonal at a time pulls both towards one range of the other
+    // axis per sub-epoch, while Hogwild interleaves them (and on factors this
+    // large the per-XCD copies of a row are evicted long before a launch
+    // ends); otherwise the conflict-free strata in one pass
+    std::vector<int32_t> hu, hv;
+    if ((rc = rating_degrees(c, hu, hv))) return rc;
+    const int64_t mu = hu.empty() ? 0 : *std::max_element(hu.begin(), hu.end());
+    const int64_t mv = hv.empty() ? 0 : *std::max_element(hv.begin(), hv.end());
+    const bool both_skewed = mu * kSkewedShareInv >= c->nnz && mv * kSkewedShareInv >= c->nnz;
+    *schedule = both_skewed ? CUMF_SCHED_HOGWILD : CUMF_SCHED_STRATUM;
+    return CUMF_OK;
 }
 
 int cumf_stream_ratings(cumf_ctx* c, const cumf_node* R, int64_t nnz, int32_t m, int32_t n, int32_t s,
@@ -1004,7 +1058,13 @@ void cumf::lr_table(const cumf_train_opts* o, int n, float* out)
 
 namespace {
 
-ShapeInput shape_input(const cumf_ctx* c) { return ShapeInput{c->k, c->prec, c->cu_count, c->nnz}; }
+// A launch confined to one XCD sizes its default worker count for that XCD's
+// CUs.
+ShapeInput shape_input(const cumf_ctx* c, const cumf_train_opts* o)
+{
+    const bool one = confine_to_one_xcd(o->schedule, o->coherent, c->m, c->n, c->k, c->prec);
+    return ShapeInput{c->k, c->prec, one ? std::max(1, c->cu_count / kNumXcd) : c->cu_count, c->nnz};
+}
 
 int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
 {
@@ -1326,7 +1386,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         return fail(CUMF_ERR_STATE, "stratum schedule needs an s x s partition (have %d x %d)", c->ux, c->vy);
     const int num_xcd = stratum ? 8 : 1;
     Shape sh;
-    if ((rc = resolve_shape(shape_input(c), o, &sh, num_xcd))) return rc;
+    if ((rc = resolve_shape(shape_input(c, o), o, &sh, num_xcd))) return rc;
     if (stratum) sh.W = sh.blocks * kSgdWavesPerBlock * sh.S;   // every wave of every XCD works
     if ((rc = upload_lr(c, o, e1 + 4))) return rc;
 
@@ -1513,7 +1573,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
                     o.pipeline_depth = D;
                     o.batch_len = L;
                     o.num_workers = base->num_workers ? base->num_workers
-                                                      : default_workers(shape_input(c), 64 / G, L, waves_per_cu[wi]);
+                                                      : default_workers(shape_input(c, base), 64 / G, L, waves_per_cu[wi]);
                     o.use_graph = 0;
                     if (!warm) {   // first launch of the code object is not timed
                         if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;

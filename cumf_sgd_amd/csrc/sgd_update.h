@@ -131,13 +131,16 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
     const int rbase = slot * G;
-    auto request = [&](Pending<NW>& pd, int j) {
+    // Request sample j's rows.  A row of the sample before it (prev_u,
+    // prev_v: -1 at a window's start) is not loaded: that sample's written
+    // row is forwarded from the register ring instead (below).
+    auto request = [&](Pending<NW>& pd, int j, int prev_u, int prev_v) {
         const int at = (j / G) * 64 + rbase + j % G;
         pd.u = su[at];
         pd.v = sv[at];
         pd.r = sr[at];
-        pd.p = load_raw<TB, COH>(Pb + (int64_t)pd.u * kRow);
-        pd.q = load_raw<TB, COH>(Qb + (int64_t)pd.v * kRow);
+        if (pd.u != prev_u) pd.p = load_raw<TB, COH>(Pb + (int64_t)pd.u * kRow);
+        if (pd.v != prev_v) pd.q = load_raw<TB, COH>(Qb + (int64_t)pd.v * kRow);
         if constexpr (FP8) {
             pd.ps = Ps[pd.u];
             pd.qs = Qs[pd.v];
@@ -178,7 +181,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         for (int d = 0; d < D; ++d) {
             wu[d] = -1;
             wv[d] = -1;
-            request(pend[d], d);
+            request(pend[d], d, d ? pend[d - 1].u : -1, d ? pend[d - 1].v : -1);
             // keep the requests in consumption order, so the wait for the
             // oldest one is a partial wait (the loop header merges this state)
             __builtin_amdgcn_sched_barrier(0);
@@ -366,13 +369,18 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                     }
                 }
                 }
-                store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
-                store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
+                // a row that the next sample of this worker (in pend[(d + 1) % D],
+                // requested already) rewrites is not stored: that sample takes
+                // it from the ring and stores its own result
+                const bool has_next = decltype(more)::value || d + 1 < D;
+                const Pending<NW>& nx = pend[(d + 1) % D];
+                if (!(has_next && nx.u == uc)) store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
+                if (!(has_next && nx.v == vc)) store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
                 wp[d] = po;
                 wq[d] = qo;
                 wu[d] = uc;
                 wv[d] = vc;
-                if constexpr (decltype(more)::value) request(pend[d], j + D);
+                if constexpr (decltype(more)::value) request(pend[d], j + D, pend[(d + D - 1) % D].u, pend[(d + D - 1) % D].v);
             }
         };
         if (L > D) {

@@ -43,6 +43,13 @@ constexpr int kNumXcd = 8;
 constexpr int64_t kOneXcdFactorBytes = 32ll << 20;   // 8 XCDs x 4 MB of L2
 bool confine_to_one_xcd(int schedule, int coherent, int64_t m, int64_t n, int k, int prec);
 
+// The default schedule counts an axis as skewed when its busiest row holds at
+// least 1/kSkewedShareInv of all ratings.
+constexpr int64_t kSkewedShareInv = 100;
+// Stratum passes per epoch that the default schedule gives factors small
+// enough for one XCD's launch placement.
+constexpr int kSmallFactorRounds = 4;
+
 // lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
 // (singleGPU/sgd_kernel.cu:137-154).
 void lr_table(const cumf_train_opts* o, int n, float* out);

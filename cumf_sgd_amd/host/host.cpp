@@ -339,10 +339,11 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
     o.lrate = p.lrate;
     o.seed = seed;
     if (p.u_grid * p.v_grid == 1) {
-        // the 1x1 grid trains conflict-free 24 x 24 strata (Hogwild loses
-        // updates to the per-XCD L2 copies of rows on small problems)
-        CUMF_TRY(cumf_partition_strata(g.c, 24, nullptr));
-        o.schedule = CUMF_SCHED_STRATUM;
+        // the 1x1 grid trains the library's default schedule for these
+        // ratings: conflict-free 24 x 24 strata, or Hogwild on data skewed
+        // on both axes
+        CUMF_TRY(cumf_default_schedule(g.c, p.k, CUMF_FP16, &o.schedule, &o.stratum_rounds));
+        if (o.schedule == CUMF_SCHED_STRATUM) CUMF_TRY(cumf_partition_strata(g.c, 24, nullptr));
         CUMF_TRY(cumf_train(g.c, &o, 0, p.num_iters, nullptr, 0));
     } else {
         o.schedule = CUMF_SCHED_GRID;
@@ -480,11 +481,18 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
     const float fs = std::sqrt(scale);   // scale_model, sgd.cu:598
 
     cumf_train_opts o = opts_from(p, extra, scale, seed);
-    // A 1x1 grid trains the conflict-free stratum schedule: under Hogwild
-    // every XCD's L2 keeps its own copy of the rows it updates until the
-    // kernel ends, which on problems whose rows fit in the L2s loses most
-    // updates.  Gridded runs keep the reference's block order.
-    if (!extra) o.schedule = ux * vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID;
+    // A 1x1 grid trains the library's default schedule for these ratings
+    // (conflict-free strata, or Hogwild on data skewed on both axes);
+    // gridded runs keep the reference's block order.
+    if (!extra || extra->schedule == CUMF_SCHED_AUTO) {
+        if (ux * vy == 1) {
+            int32_t rounds = 1;
+            CUMF_TRY(cumf_default_schedule(ctx, p.k, precision, &o.schedule, &rounds));
+            if (!extra || extra->stratum_rounds == 0) o.stratum_rounds = rounds;
+        } else {
+            o.schedule = CUMF_SCHED_GRID;
+        }
+    }
     // each epoch's launches replay as one captured graph (gridded runs with a
     // new block order per epoch launch one by one)
     if (!extra) o.use_graph = 1;

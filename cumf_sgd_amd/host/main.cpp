@@ -8,9 +8,12 @@
 //   --seed N        Philox key and srand seed (reference: time(NULL))
 //   --precision P   fp32 | fp16 (default, the reference's) | bf16 | bf16sr | fp8
 //   --serial        one worker, deterministic window order
-//   --schedule S    hogwild | grid | stratum (default: stratum for a 1x1 grid, grid otherwise)
+//   --schedule S    hogwild | grid | stratum (default: a 1x1 grid trains the
+//                   library's choice for the ratings, cumf_default_schedule;
+//                   a larger grid trains its blocks in the reference's order)
 //   --lanes G       lanes per update (8..64)
-//   --rounds R      stratum schedule: passes over the diagonals per epoch
+//   --rounds R      stratum schedule (implied on a 1x1 grid): passes over the
+//                   diagonals per epoch
 //                   (s comes from a square -u/-v grid, else 24)
 //   --coherent      device-scope coherent factor accesses
 // With -p the test RMSE is printed after every epoch.  A 1x1 grid streams the
@@ -171,9 +174,11 @@ int main(int argc, char** argv)
     }
     cumf_train_opts o;
     std::memset(&o, 0, sizeof(o));
-    // 1x1 grid: the conflict-free stratum schedule (--schedule hogwild for the
-    // reference's own 1x1 mode); gridded runs keep the reference's block order
-    o.schedule = a.schedule >= 0 ? a.schedule : (a.p.ux * a.p.vy == 1 ? CUMF_SCHED_STRATUM : CUMF_SCHED_GRID);
+    // without --schedule a 1x1 grid trains the library's default for the
+    // ratings read (cumf_default_schedule); gridded runs keep the reference's
+    // block order
+    o.schedule = a.schedule >= 0 ? a.schedule : CUMF_SCHED_AUTO;
+    if (a.schedule < 0 && a.rounds > 0 && a.p.ux * a.p.vy == 1) o.schedule = CUMF_SCHED_STRATUM;
     o.lanes_per_sample = a.lanes;
     o.coherent = a.coherent;
     o.stratum_rounds = a.rounds;

@@ -47,7 +47,11 @@ enum cumf_precision {
 
 /* Which rating blocks run in one launch. */
 enum cumf_schedule {
-    CUMF_SCHED_HOGWILD = 0,  /* all workers on the whole matrix (reference 1x1 mode) */
+    CUMF_SCHED_AUTO = -1,    /* cumf_sgd_train / cumf_sgd_train_file options only: a 1x1 grid
+                                trains cumf_default_schedule's choice, a larger grid GRID */
+    CUMF_SCHED_HOGWILD = 0,  /* all workers on the whole matrix (reference 1x1 mode); factors of
+                                at most 32 MB (P and Q) train on one XCD, whose L2 then holds
+                                the only L2 copy of every row */
     CUMF_SCHED_SERIAL = 1,   /* one worker, deterministic window order */
     CUMF_SCHED_GRID = 2,     /* blocks one after another, all workers each (u x v / x x y) */
     CUMF_SCHED_STRATUM = 3   /* s x s blocks, a conflict-free diagonal per launch */
@@ -199,6 +203,22 @@ int cumf_test_eval(cumf_ctx* ctx, int32_t criterion, float mult, float b, double
  * With ratings present, m and n must match them; training still needs k in
  * {32, 64, 128}. */
 int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* P, const float* Q);
+
+/* The schedule a 1x1 grid trains by default (cumf_sgd_train and
+ * cumf_sgd_train_file with no options or CUMF_SCHED_AUTO, the command line
+ * without --schedule, cumf_sgd_update_k128), chosen from the context's
+ * ratings for factors of length k in the given format:
+ *  - P and Q together at most 32 MB: CUMF_SCHED_STRATUM with 4 passes over
+ *    the diagonals per epoch;
+ *  - a row and a column that each hold at least 1% of the ratings:
+ *    CUMF_SCHED_HOGWILD (one diagonal at a time would pull both towards one
+ *    range of the other axis per sub-epoch);
+ *  - otherwise CUMF_SCHED_STRATUM, one pass.
+ * It replaces the reference's fixed choice of Hogwild for a 1x1 grid
+ * (singleGPU/sgd_kernel.cu:212-248), made for one coherent L2; MI355X has
+ * eight XCDs with an L2 each. */
+int cumf_default_schedule(cumf_ctx* ctx, int32_t k, int32_t precision, int32_t* schedule,
+                          int32_t* stratum_rounds);
 
 /* Pick the launch shape: time one epoch of every supported combination of
  * lanes_per_sample {64, 32, 16, 8}, pipeline_depth {2, 4, 8}, batch_len
