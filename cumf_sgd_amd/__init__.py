@@ -157,6 +157,8 @@ def _load():
         "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
         "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "cumf_default_schedule": ([P, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32)], ctypes.c_int),
+        "cumf_train_noalloc": ([ctypes.c_int, P, ctypes.POINTER(TrainOpts), I32, P, P, P, I64, I32, I32, P, P, P, P,
+                                I32, I32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -480,6 +482,20 @@ class Batch:
         _check(lib.cumf_batch_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)),
                "cumf_batch_kernel_time")
         return KernelTime(ms.value, la.value, up.value)
+
+
+NOALLOC_MAX_EPOCHS = 4028   # CUMF_NOALLOC_MAX_EPOCHS
+
+
+def train_noalloc(opts: TrainOpts, precision: int, u, v, r, m: int, n: int, P, Q, epoch_begin: int, epoch_end: int,
+                  ps=None, qs=None, device: int = 0, stream: int | None = None) -> None:
+    """cumf_train_noalloc: epochs of the Hogwild or serialized schedule over
+    ratings and factors in the caller's device memory (torch tensors or raw
+    device addresses), updated in place; the library allocates nothing.
+    `stream` is a hipStream_t address (None = the null stream)."""
+    _check(lib.cumf_train_noalloc(device, stream, ctypes.byref(opts), precision, _ptr(u), _ptr(v), _ptr(r),
+                                  int(u.numel() if hasattr(u, "numel") else len(u)), m, n, _ptr(P), _ptr(Q),
+                                  _ptr(ps), _ptr(qs), epoch_begin, epoch_end), "cumf_train_noalloc")
 
 
 def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambda_p: float = 0.05,

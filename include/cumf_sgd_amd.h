@@ -330,6 +330,26 @@ int cumf_sgd_train_file(const cumf_param* para, const cumf_train_opts* opts, con
                         const char* test_path, int32_t precision, uint64_t seed, float** P_out, float** Q_out,
                         int32_t* m, int32_t* n, int64_t* nnz, float* b_out, double* rmse_out);
 
+/* ---- allocation-free training -------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): epochs [epoch_begin, epoch_end) of opts->schedule
+ * (CUMF_SCHED_HOGWILD or CUMF_SCHED_SERIAL; use_graph 0) over ratings u, v, r
+ * (nnz, ids in [0,m) x [0,n), not checked) and factors P (m x opts->k), Q
+ * (n x opts->k) in the given format (FP8: with row scales ps, qs), all in
+ * the caller's device memory on `device`, updated in place.  Same kernels,
+ * shapes and results as cumf_train on a context holding the same ratings and
+ * factors.  The launch's task record and learning-rate table live in a fixed
+ * 16 KiB device array of the library (one per GPU, kept until the process
+ * ends), hence epoch_end <= CUMF_NOALLOC_MAX_EPOCHS; the call allocates
+ * nothing, queues its work on `stream` (a hipStream_t; NULL = the null
+ * stream) and returns.  Calls that overlap on one GPU share that array and
+ * race. */
+#define CUMF_NOALLOC_MAX_EPOCHS 4028
+int cumf_train_noalloc(int device, void* stream, const cumf_train_opts* opts, int32_t precision,
+                       const int32_t* u, const int32_t* v, const float* r, int64_t nnz, int32_t m, int32_t n,
+                       void* P, void* Q, float* ps, float* qs, int32_t epoch_begin, int32_t epoch_end);
+
 /* ---- host helpers (the reference's host pipeline, bit-exact) ------------- */
 /* gen_random_map (singleGPU/sgd.cu:39-53) with srand(seed) in place of
  * srand(time(NULL)). */
