@@ -8,8 +8,8 @@ step = one epoch of batch-Hogwild SGD over all training ratings (the
 reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
 by default under the stratum schedule (24 x 24 nnz-balanced blocks, the 24
 conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
-the least-loaded one, one launch per diagonal; --schedule hogwild runs the
-reference's 1x1 mode).  Each epoch's launches are captured once as a HIP
+the least-loaded one, one launch per diagonal, each block's records ordered
+by item; --schedule hogwild runs the reference's 1x1 mode).  Each epoch's launches are captured once as a HIP
 graph and replayed (--no-graph launches them one by one).  The launch shape
 is fixed (the library's defaults for the format with 32-record windows, which
 is what cumf_autotune picks on this workload), so that the profiled kernel is
@@ -54,6 +54,14 @@ K = 128
 BYTES_PER_UPDATE = {0: 12 + 4 * K * 4, 1: 12 + 4 * K * 2, 2: 12 + 4 * K * 2, 3: 12 + 4 * K * 1 + 2 * 4,
                     4: 12 + 4 * K * 2}
 PREC_NAME = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8_e4m3", 4: "bf16_sr"}
+# How each format's write-back rounds, for reading its time to target: the
+# round-to-nearest formats drop every step smaller than half a unit in the
+# last place (FP16 from about 2^-12 of a value, BF16 from 2^-9), which moves
+# their trajectory away from FP32's; the stochastically rounded ones keep
+# every step in expectation.
+WRITE_BACK = {0: "fp32 (exact)", 1: "round to nearest", 2: "round to nearest (drops steps below half a BF16 ulp: "
+              "not FP32's trajectory)", 3: "stochastic rounding to E4M3 (3 mantissa bits)",
+              4: "stochastic rounding (FP32's trajectory in expectation)"}
 
 
 def log(msg: str) -> None:
@@ -153,12 +161,13 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
-def profile_summary(prec: int, info, schedule: str, upl: float | None = None) -> dict | None:
+def profile_summary(prec: int, info, schedule: str, upl: float | None = None, order: str | None = None) -> dict | None:
     """The committed rocprofv3 counter summary (profiles/) of the update
     kernel on this workload and schedule, when it was measured on the same
     kernel and launch shape (window length and updates per launch, when the
     summary records them; the worker budget alone does not change a stratum
-    launch once it exceeds the windows of a diagonal); None otherwise."""
+    launch once it exceeds the windows of a diagonal) and, when given, the
+    same record order inside the blocks; None otherwise."""
     name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8", 4: "bf16sr"}[prec]
     path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
@@ -171,17 +180,19 @@ def profile_summary(prec: int, info, schedule: str, upl: float | None = None) ->
         return None
     if upl is not None and "updates_per_launch" in d and abs(d["updates_per_launch"] - upl) > 1e-3 * upl:
         return None
+    if order is not None and d.get("block_order", "input") != order:
+        return None
     return d
 
 
-def measured_traffic(prec: int, info, schedule: str, upl: float | None = None):
+def measured_traffic(prec: int, info, schedule: str, upl: float | None = None, order: str | None = None):
     """HBM-side bytes per launch of the update kernel (calibrated FETCH_SIZE +
     WRITE_SIZE) from the matching profile summary; None otherwise."""
-    d = profile_summary(prec, info, schedule, upl)
+    d = profile_summary(prec, info, schedule, upl, order)
     return None if d is None else d["traffic_bytes_per_launch"]
 
 
-def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float) -> dict:
+def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, order: str) -> dict:
     """Roofline of the update kernel.  `achieved` is the algorithmic bytes
     (every row read and written, the rating record) per launch over the
     launch's HIP-event duration.  Those bytes are L2 requests: at the L2 the
@@ -190,7 +201,7 @@ def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float) ->
     profile of the same launch shape) over the 8 TB/s HBM peak.  `bound` is
     the level with the larger fraction."""
     algo = BYTES_PER_UPDATE[prec] * upl / avg_launch_s / 1e9
-    prof = profile_summary(prec, info, schedule, upl)
+    prof = profile_summary(prec, info, schedule, upl, order)
     traffic = None if prof is None else prof["traffic_bytes_per_launch"]
     levels = {"l2": {"achieved": algo, "peak": L2_PEAK_GBS, "frac": algo / L2_PEAK_GBS,
                      "bytes": "algorithmic (12-byte record + P and Q rows read and written)"}}
@@ -221,6 +232,9 @@ def main() -> int:
                     help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
                          "(default); hogwild: the reference's 1x1 mode")
     ap.add_argument("--strata", type=int, default=24, help="s for the stratum schedule (s x s blocks)")
+    ap.add_argument("--order", default="columns", choices=["columns", "rows", "input"],
+                    help="record order inside each stratum block (cumf_order_blocks): by column (default, the "
+                         "library's default for this workload), by row, or as generated")
     ap.add_argument("--rounds", type=int, default=0,
                     help="stratum schedule: passes over the diagonals per epoch (0 = library default)")
     ap.add_argument("--batch-len", type=int, default=32,
@@ -278,6 +292,8 @@ def main() -> int:
     scale, b, lam = prepare(ctx, R, args.seed)
     if sched == cs.STRATUM:
         ctx.partition_strata(args.strata)
+        if args.order != "input":
+            ctx.order_blocks(cs.ORDER_COLUMNS if args.order == "columns" else cs.ORDER_ROWS)
     torch.cuda.synchronize()
     log(f"preprocessed: scale={scale:.5f} b={b:.5f}")
 
@@ -345,6 +361,8 @@ def main() -> int:
                 "epochs_to_target": None if hit is None else hit + 1,
                 "seconds_to_target": None if hit is None else round(float(sum(times[:hit + 1])), 4),
                 "train_seconds_total": round(float(sum(times)), 4),
+                "meets_target": hit is not None,
+                "write_back": WRITE_BACK[p],
                 "shape": shapes[p],
             }
         ttt["target_rmse"] = round(target, 5)
@@ -399,12 +417,14 @@ def main() -> int:
                    "factor_storage": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3 + f32 row scale",
                                       4: "bf16 (stochastic rounding)"}[prec],
                    "schedule": sched_tag, "graph_replay": not args.no_graph,
+                   "block_order": args.order if sched == cs.STRATUM else "input",
                    "shape_source": "cumf_autotune" if args.autotune else
                    "fixed: library defaults with the given window length (cumf_autotune's pick on this workload)",
                    "workers": info.num_workers,
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
-        "roofline": roofline(prec, info, sched_tag, avg_launch_s, upd_per_launch),
+        "roofline": roofline(prec, info, sched_tag, avg_launch_s, upd_per_launch,
+                             args.order if sched == cs.STRATUM else "input"),
         "time_to_target": ttt,
         "cpu_baseline": cpu,
     }

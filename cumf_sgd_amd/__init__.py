@@ -30,6 +30,7 @@ PREDICT_PATH = os.path.join(_HERE, "cumf_predict")
 FP32, FP16, BF16, FP8_E4M3, BF16_SR = 0, 1, 2, 3, 4
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3, "bf16sr": BF16_SR}
 AUTO, HOGWILD, SERIAL, GRID, STRATUM = -1, 0, 1, 2, 3
+ORDER_ROWS, ORDER_COLUMNS = 1, 2
 # mf-predict -e criteria (test/mf.h:30-40)
 EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
 EVAL_ROW_MPR, EVAL_COL_MPR, EVAL_ROW_AUC, EVAL_COL_AUC = 10, 11, 12, 13
@@ -156,7 +157,9 @@ def _load():
         "cumf_batch_autotune": ([P, ctypes.POINTER(TrainOpts), ctypes.POINTER(TrainOpts), D], ctypes.c_int),
         "cumf_batch_get_factors": ([P, I32, I32, P, P], ctypes.c_int),
         "cumf_batch_kernel_time": ([P, D, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
-        "cumf_default_schedule": ([P, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32)], ctypes.c_int),
+        "cumf_order_blocks": ([P, I32], ctypes.c_int),
+        "cumf_default_schedule": ([P, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32)],
+                                  ctypes.c_int),
         "cumf_train_noalloc": ([ctypes.c_int, P, ctypes.POINTER(TrainOpts), I32, P, P, P, I64, I32, I32, P, P, P, P,
                                 I32, I32], ctypes.c_int),
     }
@@ -298,6 +301,11 @@ class Context:
         _check(lib.cumf_partition_strata(self._h, s, _ptr(off)), "cumf_partition_strata")
         return off
 
+    def order_blocks(self, by: int) -> None:
+        """cumf_order_blocks: stable sort of each block's records by column
+        (ORDER_COLUMNS) or row (ORDER_ROWS) id."""
+        _check(lib.cumf_order_blocks(self._h, by), "cumf_order_blocks")
+
     def set_blocks(self, ux: int, vy: int, offsets: np.ndarray) -> None:
         _check(lib.cumf_set_blocks(self._h, ux, vy, _ptr(np.ascontiguousarray(offsets, np.int64))), "cumf_set_blocks")
 
@@ -380,13 +388,14 @@ class Context:
         _check(lib.cumf_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)), "cumf_kernel_time")
         return KernelTime(ms.value, la.value, up.value)
 
-    def default_schedule(self, k: int, precision: int) -> tuple[int, int]:
-        """cumf_default_schedule: (schedule, stratum_rounds) that a 1x1 grid
-        trains by default for these ratings."""
-        out, rounds = ctypes.c_int32(), ctypes.c_int32()
-        _check(lib.cumf_default_schedule(self._h, k, precision, ctypes.byref(out), ctypes.byref(rounds)),
-               "cumf_default_schedule")
-        return out.value, rounds.value
+    def default_schedule(self, k: int, precision: int) -> tuple[int, int, int]:
+        """cumf_default_schedule: (schedule, stratum_rounds, order) that a
+        1x1 grid trains by default for these ratings (order 0: the records
+        keep their order)."""
+        out, rounds, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.cumf_default_schedule(self._h, k, precision, ctypes.byref(out), ctypes.byref(rounds),
+                                         ctypes.byref(order)), "cumf_default_schedule")
+        return out.value, rounds.value, order.value
 
     def autotune(self, base: TrainOpts):
         """Fastest launch shape for this problem (factors are left unchanged);

@@ -47,7 +47,7 @@ def build_library() -> str:
     csrc = os.path.join(PKG, "csrc")
     host = os.path.join(PKG, "host")
     hip_units = ["sgd_kernels", "sgd_update_f32", "sgd_update_f16", "sgd_update_bf16", "sgd_update_e4m3", "sgd_update_bf16sr", "capi",
-                 "batch", "loader", "rank", "noalloc"]
+                 "batch", "loader", "rank", "noalloc", "order"]
     jobs = [[HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, f"{u}.hip"), "-o", os.path.join(BUILD, f"{u}.o")]
             for u in hip_units]
     jobs.append(["g++", *HOST_FLAGS, "-c", os.path.join(host, "host.cpp"), "-o", os.path.join(BUILD, "host.o")])

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -656,11 +658,50 @@ int cumf_partition_strata(cumf_ctx* c, int32_t s, int64_t* offsets)
     return partition_with(c, balanced_bounds(hu, s, c->nnz), balanced_bounds(hv, s, c->nnz), offsets);
 }
 
-int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* schedule, int32_t* stratum_rounds)
+int cumf_order_blocks(cumf_ctx* c, int32_t by)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if (!schedule || !stratum_rounds) return fail(CUMF_ERR_ARG, "null output");
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (by != CUMF_ORDER_ROWS && by != CUMF_ORDER_COLUMNS) return fail(CUMF_ERR_ARG, "bad order %d", by);
+    const int64_t nnz = c->nnz;
+    if (nnz > (int64_t)UINT32_MAX) return fail(CUMF_ERR_ARG, "%lld ratings: ordering supports up to 2^32 - 1",
+                                                (long long)nnz);
+    if (nnz == 0) return CUMF_OK;
+    const int nb = (int)c->offsets.size() - 1;
+    const uint64_t stride = (uint64_t)std::max(1, by == CUMF_ORDER_COLUMNS ? c->n : c->m);
+    const size_t kb = order_key_bytes(nb, stride);
+    DevTmp dof, keys, keys_out, idx, idx_out, uo, vo, ro, temp;
+    size_t temp_bytes = 0;
+    HIP_TRY(hipMalloc(&dof.p, sizeof(int64_t) * (size_t)(nb + 1)));
+    HIP_TRY(hipMalloc(&keys.p, kb * (size_t)nnz));
+    HIP_TRY(hipMalloc(&keys_out.p, kb * (size_t)nnz));
+    HIP_TRY(hipMalloc(&idx.p, sizeof(uint32_t) * (size_t)nnz));
+    HIP_TRY(hipMalloc(&idx_out.p, sizeof(uint32_t) * (size_t)nnz));
+    HIP_TRY(hipMalloc(&uo.p, sizeof(int32_t) * (size_t)nnz));
+    HIP_TRY(hipMalloc(&vo.p, sizeof(int32_t) * (size_t)nnz));
+    HIP_TRY(hipMalloc(&ro.p, sizeof(float) * (size_t)nnz));
+    HIP_TRY(hipMemcpyAsync(dof.p, c->offsets.data(), sizeof(int64_t) * (size_t)(nb + 1), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(launch_order_blocks(nnz, by == CUMF_ORDER_COLUMNS ? c->v : c->u, (const int64_t*)dof.p, nb, stride,
+                                keys.p, keys_out.p, (uint32_t*)idx.p, (uint32_t*)idx_out.p, c->u, c->v, c->r,
+                                (int32_t*)uo.p, (int32_t*)vo.p, (float*)ro.p, &temp.p, &temp_bytes, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::swap(*(void**)&c->u, uo.p);
+    std::swap(*(void**)&c->v, vo.p);
+    std::swap(*(void**)&c->r, ro.p);
+    c->u_cap = c->v_cap = c->r_cap = (size_t)nnz;
+    return CUMF_OK;
+}
+
+int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* schedule, int32_t* stratum_rounds,
+                          int32_t* order)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!schedule || !stratum_rounds || !order) return fail(CUMF_ERR_ARG, "null output");
+    *order = 0;
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
     if (k != 32 && k != 64 && k != 128) return fail(CUMF_ERR_ARG, "k=%d unsupported (32, 64 or 128)", k);
     if (precision < CUMF_FP32 || precision > CUMF_BF16_SR) return fail(CUMF_ERR_ARG, "bad precision %d", precision);
@@ -690,6 +731,10 @@ int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* sc
     const int64_t mv = hv.empty() ? 0 : *std::max_element(hv.begin(), hv.end());
     const bool both_skewed = mu * kSkewedShareInv >= c->nnz && mv * kSkewedShareInv >= c->nnz;
     *schedule = both_skewed ? CUMF_SCHED_HOGWILD : CUMF_SCHED_STRATUM;
+    // one-pass strata train each block's records grouped by column, so that
+    // consecutive samples of a worker share the column's row (kept in
+    // registers, not stored and reloaded); the ratings fit 32-bit indices
+    if (*schedule == CUMF_SCHED_STRATUM && c->nnz <= (int64_t)UINT32_MAX) *order = CUMF_ORDER_COLUMNS;
     return CUMF_OK;
 }
 

@@ -92,5 +92,14 @@ hipError_t launch_partition(const int32_t* u, const int32_t* v, const float* r, 
                             int64_t* off, int32_t* uo, int32_t* vo, float* ro, hipStream_t st, int64_t* host_tot);
 hipError_t launch_seg_table(int32_t* seg, int64_t rows, const int64_t* bnd, int ns, hipStream_t st);
 hipError_t launch_degree(const int32_t* ids, int64_t nnz, int32_t* deg, int32_t id_range, hipStream_t st);
+// Stable sort of each block's records (blocks = prefix offsets off[0..nb]) by
+// ids (< stride), gathered into uo/vo/ro.  keys/keys_out hold
+// order_key_bytes(nb, stride) bytes per record, idx/idx_out 4; temp is grown
+// (hipMalloc) as the sort needs.
+size_t order_key_bytes(int nb, uint64_t stride);
+hipError_t launch_order_blocks(int64_t nnz, const int32_t* ids, const int64_t* off, int nb, uint64_t stride, void* keys,
+                               void* keys_out, uint32_t* idx, uint32_t* idx_out, const int32_t* u, const int32_t* v,
+                               const float* r, int32_t* uo, int32_t* vo, float* ro, void** temp, size_t* temp_bytes,
+                               hipStream_t st);
 
 }  // namespace cumf

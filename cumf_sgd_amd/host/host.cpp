@@ -342,8 +342,10 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
         // the 1x1 grid trains the library's default schedule for these
         // ratings: conflict-free 24 x 24 strata, or Hogwild on data skewed
         // on both axes
-        CUMF_TRY(cumf_default_schedule(g.c, p.k, CUMF_FP16, &o.schedule, &o.stratum_rounds));
+        int32_t order = 0;
+        CUMF_TRY(cumf_default_schedule(g.c, p.k, CUMF_FP16, &o.schedule, &o.stratum_rounds, &order));
         if (o.schedule == CUMF_SCHED_STRATUM) CUMF_TRY(cumf_partition_strata(g.c, 24, nullptr));
+        if (order) CUMF_TRY(cumf_order_blocks(g.c, order));
         CUMF_TRY(cumf_train(g.c, &o, 0, p.num_iters, nullptr, 0));
     } else {
         o.schedule = CUMF_SCHED_GRID;
@@ -484,10 +486,11 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
     // A 1x1 grid trains the library's default schedule for these ratings
     // (conflict-free strata, or Hogwild on data skewed on both axes);
     // gridded runs keep the reference's block order.
+    int32_t block_order = 0;   // record order inside the stratum blocks (0: as given)
     if (!extra || extra->schedule == CUMF_SCHED_AUTO) {
         if (ux * vy == 1) {
             int32_t rounds = 1;
-            CUMF_TRY(cumf_default_schedule(ctx, p.k, precision, &o.schedule, &rounds));
+            CUMF_TRY(cumf_default_schedule(ctx, p.k, precision, &o.schedule, &rounds, &block_order));
             if (!extra || extra->stratum_rounds == 0) o.stratum_rounds = rounds;
         } else {
             o.schedule = CUMF_SCHED_GRID;
@@ -501,6 +504,7 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
         // fastest for 16-bit factors on a Netflix-shaped problem
         const int s = ux == vy && ux > 1 ? ux : 24;
         CUMF_TRY(cumf_partition_strata(ctx, s, nullptr));
+        if (block_order) CUMF_TRY(cumf_order_blocks(ctx, block_order));
     }
     for (int ep = 0; ep < p.num_iters; ++ep) {
         const int32_t* ord = nullptr;

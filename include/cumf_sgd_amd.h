@@ -155,6 +155,18 @@ int cumf_partition(cumf_ctx* ctx, int32_t ux, int32_t vy, int64_t* offsets);
 /* Partition into s x s blocks whose row and column ranges hold about equal
  * rating counts (for CUMF_SCHED_STRATUM), stable like cumf_partition. */
 int cumf_partition_strata(cumf_ctx* ctx, int32_t s, int64_t* offsets);
+/* Record order inside every block of the current partition: a stable sort
+ * of each block's records by column id (CUMF_ORDER_COLUMNS) or row id
+ * (CUMF_ORDER_ROWS), ties kept in their order.  No counterpart in the
+ * reference (its blocks keep the order grid_problem's cycle-swap leaves,
+ * singleGPU/sgd.cu:465-482).  With the records of a column adjacent, a
+ * window's consecutive samples mostly share that column's row, which the
+ * update kernel then keeps in registers instead of storing and reloading
+ * (99M-rating Netflix shape, 24 x 24 strata: FP16 1.52e10 -> 2.03e10
+ * updates/s, test RMSE unchanged).  Up to 2^32 - 1 ratings; not for
+ * streamed ratings. */
+enum cumf_order { CUMF_ORDER_ROWS = 1, CUMF_ORDER_COLUMNS = 2 };
+int cumf_order_blocks(cumf_ctx* ctx, int32_t by);
 /* Declare that the uploaded ratings are already grouped into ux x vy blocks
  * with the given prefix offsets (as the host cycle-swap leaves them). */
 int cumf_set_blocks(cumf_ctx* ctx, int32_t ux, int32_t vy, const int64_t* offsets);
@@ -213,12 +225,15 @@ int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* 
  *  - a row and a column that each hold at least 1% of the ratings:
  *    CUMF_SCHED_HOGWILD (one diagonal at a time would pull both towards one
  *    range of the other axis per sub-epoch);
- *  - otherwise CUMF_SCHED_STRATUM, one pass.
+ *  - otherwise CUMF_SCHED_STRATUM, one pass, each block's records ordered
+ *    by column (cumf_order_blocks, CUMF_ORDER_COLUMNS) when there are at
+ *    most 2^32 - 1 ratings.
+ * *order receives CUMF_ORDER_COLUMNS or 0 (keep the records' order).
  * It replaces the reference's fixed choice of Hogwild for a 1x1 grid
  * (singleGPU/sgd_kernel.cu:212-248), made for one coherent L2; MI355X has
  * eight XCDs with an L2 each. */
 int cumf_default_schedule(cumf_ctx* ctx, int32_t k, int32_t precision, int32_t* schedule,
-                          int32_t* stratum_rounds);
+                          int32_t* stratum_rounds, int32_t* order);
 
 /* Pick the launch shape: time one epoch of every supported combination of
  * lanes_per_sample {64, 32, 16, 8}, pipeline_depth {2, 4, 8}, batch_len

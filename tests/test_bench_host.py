@@ -28,3 +28,24 @@ def test_traffic_reported_only_for_the_profiled_shape():
     assert bench.measured_traffic(1, _info(batch_len=128), "stratum24", upl) is None
     assert bench.measured_traffic(1, _info(), "stratum24", upl * 1.01) is None
     assert bench.measured_traffic(1, _info(), "stratum7", upl) is None
+    # the profile was taken with each block's records ordered by item
+    assert bench.measured_traffic(1, _info(), "stratum24", upl, "columns") == t
+    assert bench.measured_traffic(1, _info(), "stratum24", upl, "input") is None
+
+
+def test_roofline_fractions_stay_below_one():
+    """Both levels of the bench's roofline at the profiled shape, from the
+    committed summary's own timing: the algorithmic bytes against the
+    aggregate L2 bandwidth and the measured HBM-side bytes against the HBM
+    peak are fractions of at most one, and the bound is the larger one."""
+    import json
+    import os
+
+    d = json.load(open(os.path.join(bench.ROOT, "profiles", "sgd_update_fp16_medium_stratum24.json")))
+    r = bench.roofline(1, _info(), "stratum24", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
+                       d["block_order"])
+    assert r["traffic"] == d["traffic_bytes_per_launch"]
+    assert set(r["levels"]) == {"l2", "hbm"}
+    assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values())
+    assert r["frac"] == max(lv["frac"] for lv in r["levels"].values())
+    assert abs(bench.L2_PEAK_GBS - 39321.6) < 1e-6

@@ -160,8 +160,9 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
 def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
     """Netflix-shaped 99M-rating workload, 20 epochs, under Hogwild (the
     reference's 1x1 mode, default launch shape) and under the library's
-    default for it (cumf_default_schedule: 24 x 24 strata, one pass; with
-    the benchmark's 32-record windows and graph replay): the test RMSE is
+    default for it (cumf_default_schedule: 24 x 24 strata, one pass, each
+    block's records ordered by item; with the benchmark's 32-record windows
+    and graph replay): the test RMSE is
     at most 0.005 above the serialized FP32 CPU oracle's at 20 epochs
     (tests/golden/medium_oracle_rmse.json, produced on exactly this data and
     preprocessing)."""
@@ -183,9 +184,10 @@ def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
         if schedule == "hogwild":
             times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.HOGWILD)
         else:
-            sched, rounds = c.default_schedule(128, prec)
-            assert (sched, rounds) == (cs.STRATUM, 1)
+            sched, rounds, order = c.default_schedule(128, prec)
+            assert (sched, rounds, order) == (cs.STRATUM, 1, cs.ORDER_COLUMNS)
             c.partition_strata(24)
+            c.order_blocks(order)
             times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, sched, batch_len=32, rounds=rounds,
                                             graph=1)
     del R

@@ -273,7 +273,7 @@ def _c1_rmse_curve(prec, schedule, epochs, workers=0, strata=0, coherent=0, lane
         if strata:
             c.partition_strata(strata)
         if schedule == cs.AUTO:
-            schedule, rounds = c.default_schedule(128, prec)
+            schedule, rounds, _ = c.default_schedule(128, prec)
             if schedule == cs.STRATUM and not strata:
                 c.partition_strata(24)
         c.init_factors(128, prec, seed=1)
@@ -324,7 +324,7 @@ def test_default_schedule_on_small_config_within_0005_of_serial_oracle(gpu, c1_o
     import cumf_sgd_amd as cs
 
     with _ctx(*_c1_data()) as c:
-        assert c.default_schedule(128, prec) == (cs.STRATUM, 4)
+        assert c.default_schedule(128, prec) == (cs.STRATUM, 4, 0)
     curve, _ = _c1_rmse_curve(prec, cs.AUTO, 20)
     print(f"prec={prec} default (strata, 4 passes) rmse={curve[-1]:.5f} oracle={c1_oracle_rmse:.5f}")
     assert curve[-1] <= c1_oracle_rmse + 0.005
@@ -496,14 +496,18 @@ def test_streamed_ratings_equal_resident_stratum(gpu):
     assert out[0][3] == out[1][3]
 
 
-@pytest.mark.parametrize("prec,rounds", [(0, 1), (1, 1), (1, 3), (4, 2), (3, 1)])
-def test_stratum_epochs_match_oracle_when_deterministic(gpu, prec, rounds):
+@pytest.mark.parametrize("prec,rounds,order", [(0, 1, 0), (1, 1, 0), (1, 3, 0), (4, 2, 0), (3, 1, 0), (0, 1, 2),
+                                               (1, 1, 2), (1, 1, 1), (3, 1, 2), (4, 2, 2)])
+def test_stratum_epochs_match_oracle_when_deterministic(gpu, prec, rounds, order):
     """With at most one window per block (one worker each) a stratum epoch is
     deterministic and equals its restatement: the diagonals in order, each
     block's window drawn with the block id as tag (the blocks of a diagonal
     share no rows), the FP8 rescale after each epoch.  GPU factors after
     three epochs equal the oracle's bit for bit, for one and several passes
-    over the diagonals (the window then runs in the last pass)."""
+    over the diagonals (the window then runs in the last pass), and with
+    each block's records ordered by column (2) or row (1), where most
+    consecutive samples share a row: the kernel keeps that row in registers
+    instead of storing and reloading it, which must not change the result."""
     import cumf_sgd_amd as cs
     import oracle
 
@@ -512,7 +516,13 @@ def test_stratum_epochs_match_oracle_when_deterministic(gpu, prec, rounds):
     with _ctx(u, v, r, m, n) as c:
         off = c.partition_strata(s)
         assert np.diff(off).max() <= 128
+        if order:
+            c.order_blocks(order)
         gu, gv, gr = c.get_ratings()
+        if order:
+            blk = np.repeat(np.arange(len(off) - 1), np.diff(off))
+            key = gv if order == cs.ORDER_COLUMNS else gu
+            assert np.all(np.diff(blk * (m + n) + key) >= 0)
         c.init_factors(k, prec, seed=seed)
         P, ps = c.get_factors(0)
         Q, qs = c.get_factors(1)
@@ -783,3 +793,62 @@ def test_stratum_plan_trains_every_block_with_few_workers(gpu):
         # one worker of one window at most per block beyond its records
         assert info.updates_per_epoch < len(u) + nonempty * 32 * 2
         assert c.test_rmse(1.0, 3.0) < before - 0.05
+
+
+@pytest.mark.parametrize("by", [1, 2])
+def test_order_blocks_is_a_stable_sort_within_blocks(gpu, by):
+    """cumf_order_blocks against numpy: the records of every block of a
+    partition sorted by column (2) or row (1) id, ties in their previous
+    order (a stable lexsort by (block, id)); offsets unchanged."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=9, m=900, n=500, nnz=70_000, n_test=0)
+    with _ctx(u, v, r, m, n) as c:
+        off = c.partition_strata(6)
+        pu, pv, pr = c.get_ratings()
+        c.order_blocks(by)
+        gu, gv, gr = c.get_ratings()
+    blk = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    idx = np.lexsort((pv if by == cs.ORDER_COLUMNS else pu, blk))
+    assert np.array_equal(gu, pu[idx]) and np.array_equal(gv, pv[idx])
+    assert np.array_equal(gr.view(np.uint32), pr[idx].view(np.uint32))
+
+
+@pytest.mark.parametrize("prec,D", [(0, 2), (1, 2), (1, 4), (1, 8), (3, 2), (4, 4)])
+def test_serial_epochs_on_column_ordered_ratings_match_oracle(gpu, prec, D):
+    """Serialized epochs over ratings ordered by column (one block, then
+    cumf_order_blocks): nearly every sample shares its column row with the
+    one before, so the kernel skips that row's load and store and forwards it
+    in registers; the factors must equal the oracle's serial SGD on the same
+    ratings bit for bit, for every pipeline depth."""
+    import cumf_sgd_amd as cs
+    import oracle
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=10, m=400, n=60, nnz=12_000, n_test=0)
+    k = 64
+    with _ctx(u, v, r, m, n) as c:
+        c.partition(1, 1)
+        c.order_blocks(cs.ORDER_COLUMNS)
+        gu, gv, gr = c.get_ratings()
+        assert np.all(np.diff(gv) >= 0)
+        c.init_factors(k, prec, seed=6)
+        P0, ps0 = c.get_factors(0)
+        Q0, qs0 = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.SERIAL, seed=6, pipeline_depth=D, batch_len=64), 0, 2)
+        G = c.last_launch().lanes_per_sample
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+    lr = oracle.lr_table(0.08, 0.3, 0.0, 6)
+    Pw, Qw, psw, qsw = oracle.sgd_serial(gu, gv, gr, P0, Q0, lr, 0, 1, k=k, G=G, prec=prec, lambda_p=0.05,
+                                         lambda_q=0.05, seed=6, batch_len=64, ps=ps0, qs=qs0)
+    if prec == 3:
+        Pw, psw = oracle.fp8_rescale(Pw, psw)
+        Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+    Pw, Qw, psw, qsw = oracle.sgd_serial(gu, gv, gr, Pw, Qw, lr, 1, 2, k=k, G=G, prec=prec, lambda_p=0.05,
+                                         lambda_q=0.05, seed=6, batch_len=64, ps=psw, qs=qsw)
+    if prec == 3:
+        Pw, psw = oracle.fp8_rescale(Pw, psw)
+        Qw, qsw = oracle.fp8_rescale(Qw, qsw)
+        assert np.array_equal(ps1, psw) and np.array_equal(qs1, qsw)
+    assert np.array_equal(P1.view(np.uint8), Pw.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8))

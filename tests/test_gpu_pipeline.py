@@ -69,7 +69,7 @@ def test_sgd_train_stratum_schedule(gpu, rounds):
     p = cs.make_param(k=64, num_workers=256, num_iters=8)
     with cs.Context(0) as c:
         c.set_ratings(u, v, r, m, n)
-        assert c.default_schedule(64, cs.FP16) == (cs.STRATUM, 4)
+        assert c.default_schedule(64, cs.FP16) == (cs.STRATUM, 4, 0)
     _, _, _, dflt = cs.sgd_train(p, _nodes(u, v, r), m, n, precision=cs.FP16, seed=3, test=_nodes(tu, tv, tr))
     o = cs.make_opts(k=64, schedule=cs.STRATUM, stratum_rounds=rounds)
     P, Q, b, st = cs.sgd_train(p, _nodes(u, v, r), m, n, precision=cs.FP16, seed=3, test=_nodes(tu, tv, tr),
