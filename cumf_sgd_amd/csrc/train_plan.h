@@ -49,6 +49,9 @@ constexpr int64_t kSkewedShareInv = 100;
 // Stratum passes per epoch that the default schedule gives factors small
 // enough for one XCD's launch placement.
 constexpr int kSmallFactorRounds = 4;
+// A P larger than this (four times MI355X's 256 MB Infinity Cache) is read
+// from HBM: the default schedule then groups each block's records by row.
+constexpr int64_t kHbmResidentP = 1ll << 30;
 
 // lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
 // (singleGPU/sgd_kernel.cu:137-154).

@@ -224,11 +224,14 @@ int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* 
  *    the diagonals per epoch;
  *  - a row and a column that each hold at least 1% of the ratings:
  *    CUMF_SCHED_HOGWILD (one diagonal at a time would pull both towards one
- *    range of the other axis per sub-epoch);
+ *    range of the other axis per sub-epoch), its one block's records ordered
+ *    by column for 16-bit formats;
  *  - otherwise CUMF_SCHED_STRATUM, one pass, each block's records ordered
- *    by column (cumf_order_blocks, CUMF_ORDER_COLUMNS) when there are at
+ *    by column (cumf_order_blocks, CUMF_ORDER_COLUMNS), or by row when P
+ *    takes more than 1 GB (its rows then come from HBM), when there are at
  *    most 2^32 - 1 ratings.
- * *order receives CUMF_ORDER_COLUMNS or 0 (keep the records' order).
+ * *order receives CUMF_ORDER_COLUMNS, CUMF_ORDER_ROWS or 0 (keep the
+ * records' order).
  * It replaces the reference's fixed choice of Hogwild for a 1x1 grid
  * (singleGPU/sgd_kernel.cu:212-248), made for one coherent L2; MI355X has
  * eight XCDs with an L2 each. */

@@ -36,8 +36,9 @@ def test_large_sparse_formats(gpu):
     """1,000,990 x 624,961, 252,800,275 ratings (Zipf(1.0) on both axes,
     bimodal 0..100 values, generated on the host), 6 epochs per run.  The
     library's default for it is Hogwild (the hottest user and the hottest
-    item each hold about 7% of the ratings): FP32 and BF16-SR under it end at
-    most 0.005 rating standard deviations above the serialized FP32 oracle
+    item each hold about 7% of the ratings; 16-bit rows train the records
+    grouped by item): FP32 and BF16-SR under it end at most 0.005 rating
+    standard deviations above the serialized FP32 oracle
     (tests/golden/large_oracle_rmse.json, produced on exactly this data and
     preprocessing).  FP32, BF16, BF16-SR and FP8 storage under the stratum
     schedule (not the default here: one diagonal at a time drifts the hottest
@@ -83,9 +84,14 @@ def test_large_sparse_formats(gpu):
             print(f"{name}: rmse={[round(x, 3) for x in curve]} {c.nnz * 6 / t / 1e9:.2f}e9 updates/s{gap}")
             assert np.all(np.isfinite(curve)) and curve[-1] < curve[0]
 
-        assert c.default_schedule(128, 0)[0] == cs.HOGWILD and c.default_schedule(128, 4)[0] == cs.HOGWILD
+        assert c.default_schedule(128, 0) == (cs.HOGWILD, 1, 0)
+        assert c.default_schedule(128, 4) == (cs.HOGWILD, 1, cs.ORDER_COLUMNS)
         run("hogwild fp32", 0, cs.HOGWILD)
-        run("hogwild bf16-sr", 4, cs.HOGWILD)
+        u0, v0, r0 = c.get_ratings()
+        c.partition(1, 1)
+        c.order_blocks(cs.ORDER_COLUMNS)
+        run("hogwild bf16-sr", 4, cs.HOGWILD)     # the default for 16-bit rows: records grouped by item
+        c.set_ratings(u0, v0, r0, 1_000_990, 624_961)
         off = c.partition_strata(16)
         assert off[-1] == c.nnz and np.all(np.diff(off) >= 0)
         for prec, name in ((0, "fp32"), (2, "bf16"), (4, "bf16-sr"), (3, "fp8")):
@@ -101,9 +107,10 @@ def test_large_sparse_formats(gpu):
 
 def test_tall_skinny_3_1_billion_ratings(gpu):
     """50,082,604 x 39,780 with 3,101,144,313 ratings (37 GB, generated straight
-    into the context's buffers), FP16, stratum schedule, 10 epochs: 64-bit
-    indexing, partition covers every rating, the test RMSE falls over the
-    first three epochs and ends below all of them.
+    into the context's buffers), FP16, the default schedule (16 x 16 strata,
+    each block's records grouped by user), 10 epochs: 64-bit indexing,
+    partition covers every rating, the test RMSE falls over the first three
+    epochs and ends below all of them.
 
     The ratings reach ~1300 (log-normal over 3.1e9 draws), and the
     reference's update has no clipping: alpha = 0.005 (and the reference's
@@ -133,8 +140,12 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
         _free()
         print(f"generated {nnz} ratings in {time.time() - t0:.1f}s")
         scale, b = _prep(c, 5, m, n, nnz)
+        # the default for it: one-pass strata, each block's records grouped by
+        # user (P is 12.8 GB, read from HBM)
+        assert c.default_schedule(128, cs.FP16) == (cs.STRATUM, 1, cs.ORDER_ROWS)
         off = c.partition_strata(16)
         assert off[-1] == nnz and off[0] == 0 and np.all(np.diff(off) >= 0)
+        c.order_blocks(cs.ORDER_ROWS)
         c.init_factors(128, cs.FP16, 5)
         o = cs.make_opts(k=128, schedule=cs.STRATUM, lambda_p=float(np.float32(0.05) / scale), seed=5, alpha=0.002,
                          beta=0.01)
