@@ -173,7 +173,8 @@ def profile_summary(prec: int, info, schedule: str, upl: float | None = None, or
     if not os.path.exists(path):
         return None
     d = json.load(open(path))
-    shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}>"
+    # the medium workload's factors are under 4 GB: rows addressed with 32-bit offsets
+    shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}, true>"
     if shape not in d["kernel"]:
         return None
     if d.get("batch_len", 128) != info.batch_len:   # summaries without it used the default 128

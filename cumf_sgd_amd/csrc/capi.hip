@@ -1307,6 +1307,14 @@ int queue_one(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const Laun
 {
     {
         SgdLaunch a{};
+        // factors under 4 GB each: rows addressed as launch-wide base + 32-bit offset
+        const uint64_t eb = (uint64_t)prec_bytes(c->prec);
+        if ((uint64_t)c->m * c->k * eb <= UINT32_MAX && (uint64_t)c->n * c->k * eb <= UINT32_MAX) {
+            a.P = c->P;
+            a.Q = c->Q;
+            a.ps = c->ps;
+            a.qs = c->qs;
+        }
         a.tasks = c->tasks + l.task_off;
         a.num_tasks = l.task_cnt;
         a.epoch_begin = ep;

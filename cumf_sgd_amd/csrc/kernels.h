@@ -40,6 +40,13 @@ struct SgdLaunch {
     int32_t num_xcd;          // > 1: workers numbered XCD-major over blockIdx % num_xcd
     int32_t pad;
     const int32_t* epoch_dev; // non-null: the launch's first epoch is *epoch_dev (graph replay)
+    // non-null: every task's factors, each matrix under 4 GB; the kernel then
+    // addresses rows as this launch-wide base plus a 32-bit offset (one
+    // instruction per address instead of three)
+    void* P;
+    void* Q;
+    float* ps;
+    float* qs;
 };
 
 constexpr int kSgdWavesPerBlock = 4;

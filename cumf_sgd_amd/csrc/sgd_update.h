@@ -71,8 +71,10 @@ struct Pending {
     float r, ps, qs;
 };
 
-// One worker (a slot of G lanes) running its windows of task T.
-template <int K, int G, int PREC, bool COH, int D>
+// One worker (a slot of G lanes) running its windows of task T.  OFF32: the
+// factors are the launch's (a.P, a.Q, a.ps, a.qs, under 4 GB each), addressed
+// as that wave-uniform base plus a 32-bit row offset.
+template <int K, int G, int PREC, bool COH, int D, bool OFF32 = false>
 __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int slot, int ls,
                                            int32_t* su, int32_t* sv, float* sr)
 {
@@ -94,13 +96,25 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
 
     gchar* Pb = to_global((char*)T.P) + (int64_t)ls * TB;
     gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * TB;
+    gchar* Pl = to_global((char*)a.P);
+    gchar* Ql = to_global((char*)a.Q);
     gcint* Ru = to_global(T.u);
     gcint* Rv = to_global(T.v);
     gcfloat* Rr = to_global(T.r);
-    gcfloat* Ps = to_global((const float*)T.ps);
-    gcfloat* Qs = to_global((const float*)T.qs);
+    gcfloat* Ps = to_global((const float*)(OFF32 ? a.ps : T.ps));
+    gcfloat* Qs = to_global((const float*)(OFF32 ? a.qs : T.qs));
     gcfloat* LR = to_global(T.lr);
     constexpr int64_t kRow = (int64_t)K * EB;
+    const uint32_t lane_off = (uint32_t)(ls * TB);
+    // the lane's part of row u of P / v of Q
+    auto prow = [&](int u) -> gchar* {
+        if constexpr (OFF32) return Pl + ((uint32_t)u * (uint32_t)kRow + lane_off);
+        else return Pb + (int64_t)u * kRow;
+    };
+    auto qrow = [&](int v) -> gchar* {
+        if constexpr (OFF32) return Ql + ((uint32_t)v * (uint32_t)kRow + lane_off);
+        else return Qb + (int64_t)v * kRow;
+    };
     const int64_t nnz = T.nnz;
 
     // The records of one window go from memory straight into LDS (global_load_lds,
@@ -139,8 +153,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         pd.u = su[at];
         pd.v = sv[at];
         pd.r = sr[at];
-        if (pd.u != prev_u) pd.p = load_raw<TB, COH>(Pb + (int64_t)pd.u * kRow);
-        if (pd.v != prev_v) pd.q = load_raw<TB, COH>(Qb + (int64_t)pd.v * kRow);
+        if (pd.u != prev_u) pd.p = load_raw<TB, COH>(prow(pd.u));
+        if (pd.v != prev_v) pd.q = load_raw<TB, COH>(qrow(pd.v));
         if constexpr (FP8) {
             pd.ps = Ps[pd.u];
             pd.qs = Qs[pd.v];
@@ -374,8 +388,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 // it from the ring and stores its own result
                 const bool has_next = decltype(more)::value || d + 1 < D;
                 const Pending<NW>& nx = pend[(d + 1) % D];
-                if (!(has_next && nx.u == uc)) store_raw<TB, COH>(Pb + (int64_t)uc * kRow, po);
-                if (!(has_next && nx.v == vc)) store_raw<TB, COH>(Qb + (int64_t)vc * kRow, qo);
+                if (!(has_next && nx.u == uc)) store_raw<TB, COH>(prow(uc), po);
+                if (!(has_next && nx.v == vc)) store_raw<TB, COH>(qrow(vc), qo);
                 wp[d] = po;
                 wq[d] = qo;
                 wu[d] = uc;
@@ -397,7 +411,7 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     }
 }
 
-template <int K, int G, int PREC, bool COH, int D>
+template <int K, int G, int PREC, bool COH, int D, bool OFF32>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaunch a)
 {
     constexpr int S = 64 / G;
@@ -418,8 +432,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
     while (t < a.num_tasks && worker >= a.tasks[t].worker_begin + a.tasks[t].worker_count) ++t;
     if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
     const SgdTask T = a.tasks[t];
-    sgd_worker<K, G, PREC, COH, D>(a, T, (uint32_t)(worker - T.worker_begin), slot, ls, s_u[wave], s_v[wave],
-                                   s_r[wave]);
+    sgd_worker<K, G, PREC, COH, D, OFF32>(a, T, (uint32_t)(worker - T.worker_begin), slot, ls, s_u[wave], s_v[wave],
+                                          s_r[wave]);
 }
 
 // Lane group of a factor length in batched launches (one per k).
@@ -494,7 +508,12 @@ inline hipError_t launch_batched_prec(const SgdLaunch& a, int blocks, int D, hip
 template <int K, int G, int PREC, bool COH, int D>
 inline hipError_t launch_sgd_t(const SgdLaunch& a, int blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH, D>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    if (a.P)
+        hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH, D, true>), dim3(blocks), dim3(kWavesPerBlock * 64), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL((sgd_update_kernel<K, G, PREC, COH, D, false>), dim3(blocks), dim3(kWavesPerBlock * 64), 0,
+                           s, a);
     return hipGetLastError();
 }
 
