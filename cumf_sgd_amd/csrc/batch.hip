@@ -4,6 +4,8 @@
 // reference trains one problem per process.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -343,6 +345,12 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     }
     BHIP_TRY(hipMemcpyAsync(b->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, b->stream));
     const int blocks = (int)((wave + kSgdWavesPerBlock - 1) / kSgdWavesPerBlock);
+    // every problem's factors under 4 GB: rows addressed as the (wave-uniform)
+    // task base plus a 32-bit offset
+    int32_t off32 = 1;
+    for (const Problem& p : b->probs)
+        if ((uint64_t)p.m * p.k * eb_of(b->prec) > UINT32_MAX || (uint64_t)p.n * p.k * eb_of(b->prec) > UINT32_MAX)
+            off32 = 0;
     BHIP_TRY(hipEventRecord(b->ev0, b->stream));
     for (int ep = e0; ep < e1; ++ep) {
         SgdLaunch a{};
@@ -353,6 +361,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
         a.batch_len = L;
         a.seed = o->seed;
         a.num_xcd = kBatchXcds;
+        a.task_off32 = off32;
         BHIP_TRY(launch_sgd_batched(a, blocks, b->prec, D, b->stream));
         if (b->prec == CUMF_FP8_E4M3)
             for (const Problem& p : b->probs) {

@@ -38,7 +38,8 @@ struct SgdLaunch {
     int32_t batch_len;        // <= 128
     uint64_t seed;
     int32_t num_xcd;          // > 1: workers numbered XCD-major over blockIdx % num_xcd
-    int32_t pad;
+    int32_t task_off32;       // batched launches: 1 when every task's factors are under 4 GB
+                              // (rows then addressed as the task's base + a 32-bit offset)
     const int32_t* epoch_dev; // non-null: the launch's first epoch is *epoch_dev (graph replay)
     // non-null: every task's factors, each matrix under 4 GB; the kernel then
     // addresses rows as this launch-wide base plus a 32-bit offset (one

@@ -72,11 +72,14 @@ struct Pending {
 };
 
 // One worker (a slot of G lanes) running its windows of task T.  OFF32: the
-// factors are the launch's (a.P, a.Q, a.ps, a.qs, under 4 GB each), addressed
-// as that wave-uniform base plus a 32-bit row offset.
+// factors are at the wave-uniform bases FP, FQ (row scales FPs, FQs), under
+// 4 GB each, and rows are addressed as base plus a 32-bit offset; otherwise
+// at T's, with 64-bit offsets.
 template <int K, int G, int PREC, bool COH, int D, bool OFF32 = false>
 __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int slot, int ls,
-                                           int32_t* su, int32_t* sv, float* sr)
+                                           int32_t* su, int32_t* sv, float* sr, void* FP = nullptr,
+                                           void* FQ = nullptr, const float* FPs = nullptr,
+                                           const float* FQs = nullptr)
 {
     constexpr int E = K / G;
     constexpr int EB = PrecBytes<PREC>::value;
@@ -96,13 +99,13 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
 
     gchar* Pb = to_global((char*)T.P) + (int64_t)ls * TB;
     gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * TB;
-    gchar* Pl = to_global((char*)a.P);
-    gchar* Ql = to_global((char*)a.Q);
+    gchar* Pl = to_global((char*)FP);
+    gchar* Ql = to_global((char*)FQ);
     gcint* Ru = to_global(T.u);
     gcint* Rv = to_global(T.v);
     gcfloat* Rr = to_global(T.r);
-    gcfloat* Ps = to_global((const float*)(OFF32 ? a.ps : T.ps));
-    gcfloat* Qs = to_global((const float*)(OFF32 ? a.qs : T.qs));
+    gcfloat* Ps = to_global((const float*)(OFF32 ? FPs : T.ps));
+    gcfloat* Qs = to_global((const float*)(OFF32 ? FQs : T.qs));
     gcfloat* LR = to_global(T.lr);
     constexpr int64_t kRow = (int64_t)K * EB;
     const uint32_t lane_off = (uint32_t)(ls * TB);
@@ -433,7 +436,19 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
     if (t >= a.num_tasks || worker < a.tasks[t].worker_begin) return;
     const SgdTask T = a.tasks[t];
     sgd_worker<K, G, PREC, COH, D, OFF32>(a, T, (uint32_t)(worker - T.worker_begin), slot, ls, s_u[wave], s_v[wave],
-                                          s_r[wave]);
+                                          s_r[wave], a.P, a.Q, a.ps, a.qs);
+}
+
+// A pointer every lane of the wave holds (tasks own whole waves in batched
+// launches), moved to scalar registers so that rows can be addressed as
+// scalar base plus vector offset.
+template <typename T>
+__device__ inline T* wave_uniform(T* p)
+{
+    const uint64_t x = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return (T*)(((uint64_t)hi << 32) | lo);
 }
 
 // Lane group of a factor length in batched launches (one per k).
@@ -448,7 +463,7 @@ __host__ __device__ constexpr int batch_group(int k)
 // wave_count counted in waves here, tasks sorted by worker_begin, gaps
 // allowed) and its first worker_count workers are active; each wave splits
 // into 64/G slots for the task's k.
-template <int PREC, bool COH, int D>
+template <int PREC, bool COH, int D, bool OFF32>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLaunch a)
 {
     constexpr int SMAX = 64 / batch_group<PREC>(32);
@@ -474,7 +489,9 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLau
         const int slot = lane / G, ls = lane % G;                                                        \
         const uint32_t lw = (uint32_t)(wl * (64 / G) + slot);                                            \
         if (lw >= (uint32_t)T.worker_count) return;                                                      \
-        sgd_worker<KK, G, PREC, COH, D>(a, T, lw, slot, ls, s_u[wave], s_v[wave], s_r[wave]);           \
+        sgd_worker<KK, G, PREC, COH, D, OFF32>(a, T, lw, slot, ls, s_u[wave], s_v[wave], s_r[wave],      \
+                                               wave_uniform(T.P), wave_uniform(T.Q),                     \
+                                               wave_uniform(T.ps), wave_uniform(T.qs));                  \
         return;                                                                                          \
     }
     switch (T.k) {
@@ -490,7 +507,10 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_batched_kernel(SgdLau
 template <int PREC, bool COH, int D>
 inline hipError_t launch_batched_t(const SgdLaunch& a, int blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((sgd_batched_kernel<PREC, COH, D>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    if (a.task_off32)
+        hipLaunchKernelGGL((sgd_batched_kernel<PREC, COH, D, true>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
+    else
+        hipLaunchKernelGGL((sgd_batched_kernel<PREC, COH, D, false>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, s, a);
     return hipGetLastError();
 }
 
