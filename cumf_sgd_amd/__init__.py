@@ -151,6 +151,7 @@ def _load():
         "cumf_batch_destroy": ([P], ctypes.c_int),
         "cumf_batch_add": ([P, P, P, P, I64, I32, I32, I32, F32, F32, P, P, P, I64, U64, ctypes.POINTER(I32)],
                            ctypes.c_int),
+        "cumf_batch_order": ([P, I32], ctypes.c_int),
         "cumf_batch_train": ([P, ctypes.POINTER(TrainOpts), I32, I32], ctypes.c_int),
         "cumf_batch_test_rmse": ([P, I32, F32, F32, D], ctypes.c_int),
         "cumf_batch_test_eval": ([P, I32, I32, F32, F32, D], ctypes.c_int),
@@ -452,6 +453,11 @@ class Batch:
                                   _ptr(tu), _ptr(tv), _ptr(tr), tn, seed, ctypes.byref(idx)), "cumf_batch_add")
         self.shapes.append((m, n, k))
         return idx.value
+
+    def order(self, by: int) -> None:
+        """Stable sort of every problem's training records by row
+        (ORDER_ROWS) or column id (ORDER_COLUMNS)."""
+        _check(lib.cumf_batch_order(self._h, by), "cumf_batch_order")
 
     def train(self, opts: "TrainOpts", epoch_begin: int, epoch_end: int) -> None:
         _check(lib.cumf_batch_train(self._h, ctypes.byref(opts), epoch_begin, epoch_end), "cumf_batch_train")

@@ -268,6 +268,68 @@ int cumf_batch_add(cumf_batch* b, const int32_t* u, const int32_t* v, const floa
     return CUMF_OK;
 }
 
+int cumf_batch_order(cumf_batch* b, int32_t by)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (by != CUMF_ORDER_ROWS && by != CUMF_ORDER_COLUMNS) return fail(CUMF_ERR_ARG, "bad order %d", by);
+    int64_t most = 0;
+    for (const Problem& p : b->probs) most = std::max(most, p.nnz);
+    if (most > (int64_t)UINT32_MAX)
+        return fail(CUMF_ERR_ARG, "%lld ratings in one problem: ordering supports up to 2^32 - 1", (long long)most);
+    if (most < 2) return CUMF_OK;
+    // one block per problem, [0, nnz): keys are the ids themselves (< 2^31)
+    struct Scratch {
+        void* p[5] = {};
+        ~Scratch()
+        {
+            for (void* q : p) dev_free(q);
+        }
+    } s;
+    void*& off = s.p[0];
+    void*& keys = s.p[1];
+    void*& keys_out = s.p[2];
+    void*& idx = s.p[3];
+    void*& idx_out = s.p[4];
+    struct Temp {
+        void* p = nullptr;
+        size_t bytes = 0;
+        ~Temp() { dev_free(p); }
+    } temp;
+    if ((rc = dev_alloc(&off, 2 * sizeof(int64_t))) || (rc = dev_alloc(&keys, 4 * (size_t)most)) ||
+        (rc = dev_alloc(&keys_out, 4 * (size_t)most)) || (rc = dev_alloc(&idx, 4 * (size_t)most)) ||
+        (rc = dev_alloc(&idx_out, 4 * (size_t)most)))
+        return rc;
+    for (Problem& p : b->probs) {
+        if (p.nnz < 2) continue;
+        const int64_t hoff[2] = {0, p.nnz};
+        const uint64_t stride = (uint64_t)(by == CUMF_ORDER_COLUMNS ? p.n : p.m);
+        int32_t *uo = nullptr, *vo = nullptr;
+        float* ro = nullptr;
+        if (dev_alloc((void**)&uo, 4 * (size_t)p.nnz) || dev_alloc((void**)&vo, 4 * (size_t)p.nnz) ||
+            dev_alloc((void**)&ro, 4 * (size_t)p.nnz)) {
+            dev_free(uo); dev_free(vo); dev_free(ro);
+            return CUMF_ERR_NOMEM;
+        }
+        hipError_t e = hipMemcpyAsync(off, hoff, sizeof(hoff), hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = launch_order_blocks(p.nnz, by == CUMF_ORDER_COLUMNS ? p.v : p.u, (const int64_t*)off, 1, stride, keys,
+                                    keys_out, (uint32_t*)idx, (uint32_t*)idx_out, p.u, p.v, p.r, uo, vo, ro, &temp.p,
+                                    &temp.bytes, b->stream);
+        // hoff and the old arrays stay live until the stream has drained
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            dev_free(uo); dev_free(vo); dev_free(ro);
+            return fail(CUMF_ERR_HIP, "order: %s", hipGetErrorString(e));
+        }
+        std::swap(p.u, uo);
+        std::swap(p.v, vo);
+        std::swap(p.r, ro);
+        dev_free(uo); dev_free(vo); dev_free(ro);
+    }
+    return CUMF_OK;
+}
+
 int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_t e1)
 {
     int rc = check_batch(b);

@@ -280,6 +280,12 @@ int cumf_batch_destroy(cumf_batch* batch);
 int cumf_batch_add(cumf_batch* batch, const int32_t* u, const int32_t* v, const float* r, int64_t nnz,
                    int32_t m, int32_t n, int32_t k, float lambda_p, float lambda_q, const int32_t* tu,
                    const int32_t* tv, const float* tr, int64_t test_nnz, uint64_t seed, int32_t* index);
+/* Stable sort of every added problem's training records by row id
+ * (CUMF_ORDER_ROWS) or column id (CUMF_ORDER_COLUMNS), as cumf_order_blocks
+ * does for one block: a worker's consecutive samples then mostly share a
+ * row, which the update kernel keeps in registers.  Up to 2^32 - 1 ratings
+ * per problem. */
+int cumf_batch_order(cumf_batch* batch, int32_t by);
 /* Epochs [epoch_begin, epoch_end) of every problem.  opts: schedule
  * (CUMF_SCHED_HOGWILD, or CUMF_SCHED_SERIAL = one worker per problem),
  * num_workers = total waves (0 = 64 per CU, shared in proportion to each
