@@ -1,7 +1,8 @@
 """Python binding of the cumf_sgd_amd C ABI (include/cumf_sgd_amd.h).
 
-The library is the gfx950 build in this directory (libcumf_sgd_amd.so).  There
-is no fallback: if the library is missing, importing the binding raises.
+The library is the gfx950 build in this directory (libcumf_sgd_amd.so).  When
+it is missing, the first import builds it (hipcc, build.py); there is no
+fallback: if that build fails, importing the binding raises.
 Arrays are numpy arrays (host) or objects exposing ``data_ptr()`` (torch
 device tensors); both are passed as raw pointers.
 """
@@ -92,9 +93,23 @@ class Node(ctypes.Structure):
     _fields_ = [("u", ctypes.c_int32), ("v", ctypes.c_int32), ("rate", ctypes.c_float)]
 
 
+def _build_library() -> None:
+    """Compile the library from its sources (build.py, loaded by path: going
+    through this package would recurse into this import)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("_cumf_build", os.path.join(_HERE, "build.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    b.build_library()
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is not built; run `python cumf_sgd_amd/build.py`")
+        try:
+            _build_library()
+        except Exception as e:
+            raise ImportError(f"{LIB_PATH} is not built and building it failed: {e}") from e
     lib = ctypes.CDLL(LIB_PATH)
     P, I32, I64, U64, F32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
     D = ctypes.POINTER(ctypes.c_double)

@@ -22,7 +22,11 @@ F32, F16, BF16, E4M3, BF16_SR = 0, 1, 2, 3, 4
 STORAGE = {F32: np.float32, F16: np.uint16, BF16: np.uint16, E4M3: np.uint8, BF16_SR: np.uint16}
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
+    # built on first use when missing; a failed build leaves the import failing
+    _mk = subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True)
+    if _mk.returncode != 0 or not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built and `make -C oracle` failed:\n{_mk.stdout}")
 _lib = ctypes.CDLL(LIB_PATH)
 
 P = ctypes.c_void_p
