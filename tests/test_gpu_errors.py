@@ -108,3 +108,27 @@ def test_bad_arguments_are_refused_and_the_context_stays_usable(gpu):
         assert np.array_equal(P, P1) and np.array_equal(S, S1)
         c.train(cs.make_opts(k=64, schedule=cs.GRID, num_workers=64), 0, 2)
         assert np.isfinite(c.test_rmse(1.0, 3.0))
+
+
+def test_batch_refusals_leave_the_batch_usable(gpu):
+    """Batched training refuses what it does not run (schedules other than
+    Hogwild and serial, coherent mode, an unknown record order, k outside
+    {32, 64, 128}) before any kernel, then still trains."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem()
+    with cs.Batch(cs.BF16) as b:
+        _err(lambda: b.add(u, v, r, m, n, 48, 0.05), ERR_ARG, "k=48")
+        b.add(u, v, r, m, n, 64, 0.05, tu=tu, tv=tv, tr=tr, seed=3)
+        b.add(u[:5000], v[:5000], r[:5000], m, n, 32, 0.05, seed=3)
+        _err(lambda: b.order(3), ERR_ARG, "bad order")
+        _err(lambda: b.train(cs.make_opts(schedule=cs.STRATUM), 0, 1), ERR_ARG, "hogwild and serial")
+        _err(lambda: b.train(cs.make_opts(schedule=cs.HOGWILD, coherent=1), 0, 1), ERR_ARG, "coherent")
+        _err(lambda: b.train(cs.make_opts(schedule=cs.HOGWILD, batch_len=3), 0, 1), ERR_ARG, "even")
+        P0, _ = b.get_factors(0, 0)
+        b.order(cs.ORDER_COLUMNS)
+        # the order moves records, never factors
+        P1, _ = b.get_factors(0, 0)
+        assert np.array_equal(P0, P1)
+        b.train(cs.make_opts(schedule=cs.HOGWILD, seed=3), 0, 2)
+        assert np.isfinite(b.test_rmse(0, 1.0, 3.0))
