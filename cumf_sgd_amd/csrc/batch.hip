@@ -13,6 +13,7 @@
 #include "../../include/cumf_sgd_amd.h"
 #include "errors.h"
 #include "kernels.h"
+#include "train_plan.h"
 
 using namespace cumf;
 
@@ -77,6 +78,11 @@ struct cumf_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double kernel_ms = 0;
     int64_t launches = 0, updates = 0;
+    // use_graph: the epoch (update launch, FP8 rescales, counter advance)
+    // captured once per plan and replayed; kernels read the epoch here
+    int32_t* epoch_dev = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
 };
 
 namespace {
@@ -199,6 +205,8 @@ int cumf_batch_destroy(cumf_batch* b)
     dev_free(b->lr);
     dev_free(b->tasks);
     dev_free(b->partials);
+    dev_free(b->epoch_dev);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -339,6 +347,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     if (o->schedule != CUMF_SCHED_HOGWILD && o->schedule != CUMF_SCHED_SERIAL)
         return fail(CUMF_ERR_ARG, "batched training supports the hogwild and serial schedules");
     if (o->coherent) return fail(CUMF_ERR_ARG, "batched training has no coherent mode");
+    if (o->use_graph != 0 && o->use_graph != 1) return fail(CUMF_ERR_ARG, "use_graph must be 0 or 1");
     const int L = o->batch_len ? o->batch_len : 128;
     if (L < 2 || L > kSgdMaxBatch || L % 2) return fail(CUMF_ERR_ARG, "batch_len must be even and <= %d", kSgdMaxBatch);
     int D = o->pipeline_depth ? o->pipeline_depth : 4;
@@ -413,8 +422,9 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     for (const Problem& p : b->probs)
         if ((uint64_t)p.m * p.k * eb_of(b->prec) > UINT32_MAX || (uint64_t)p.n * p.k * eb_of(b->prec) > UINT32_MAX)
             off32 = 0;
-    BHIP_TRY(hipEventRecord(b->ev0, b->stream));
-    for (int ep = e0; ep < e1; ++ep) {
+    // one epoch: the update launch, the FP8 rescales, and with epoch_dev the
+    // device epoch counter's advance (the form captured into a graph)
+    auto queue_epoch = [&](int ep, int32_t* epoch_dev) -> hipError_t {
         SgdLaunch a{};
         a.tasks = b->tasks;
         a.num_tasks = (int32_t)tasks.size();
@@ -424,12 +434,60 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
         a.seed = o->seed;
         a.num_xcd = kBatchXcds;
         a.task_off32 = off32;
-        BHIP_TRY(launch_sgd_batched(a, blocks, b->prec, D, b->stream));
+        a.epoch_dev = epoch_dev;
+        hipError_t e = launch_sgd_batched(a, blocks, b->prec, D, b->stream);
+        if (e != hipSuccess) return e;
         if (b->prec == CUMF_FP8_E4M3)
             for (const Problem& p : b->probs) {
-                BHIP_TRY(launch_fp8_rescale((uint8_t*)p.P, p.ps, p.m, p.k, b->stream));
-                BHIP_TRY(launch_fp8_rescale((uint8_t*)p.Q, p.qs, p.n, p.k, b->stream));
+                if ((e = launch_fp8_rescale((uint8_t*)p.P, p.ps, p.m, p.k, b->stream)) != hipSuccess) return e;
+                if ((e = launch_fp8_rescale((uint8_t*)p.Q, p.qs, p.n, p.k, b->stream)) != hipSuccess) return e;
             }
+        return epoch_dev ? launch_advance_int(epoch_dev, b->stream) : hipSuccess;
+    };
+    if (o->use_graph) {
+        // The key covers what the captured launches hold by value: the task
+        // table's contents and address, the counter's and the learning-rate
+        // table's addresses, the launch shape, and every problem's factors
+        // and row counts (the FP8 rescales walk them, empty problems too).
+        if (!b->epoch_dev && (rc = dev_alloc((void**)&b->epoch_dev, sizeof(int32_t)))) return rc;
+        uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
+        const int64_t shape[7] = {blocks, L, D, b->prec, off32, (int64_t)o->seed, (int64_t)b->probs.size()};
+        key = fnv1a(shape, sizeof(shape), key);
+        const void* addr[3] = {b->tasks, b->epoch_dev, b->lr};
+        key = fnv1a(addr, sizeof(addr), key);
+        for (const Problem& p : b->probs) {
+            const void* f[4] = {p.P, p.Q, p.ps, p.qs};
+            const int32_t d[3] = {p.m, p.n, p.k};
+            key = fnv1a(d, sizeof(d), fnv1a(f, sizeof(f), key));
+        }
+        if (!b->graph_exec || b->graph_key != key) {
+            if (b->graph_exec) {
+                (void)hipGraphExecDestroy(b->graph_exec);
+                b->graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t qe = queue_epoch(e0, b->epoch_dev);
+            const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (qe != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(CUMF_ERR_HIP, "graph capture failed: %s",
+                            hipGetErrorString(qe != hipSuccess ? qe : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                b->graph_exec = nullptr;
+                return fail(CUMF_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+            }
+            b->graph_key = key;
+        }
+        BHIP_TRY(launch_set_int(b->epoch_dev, e0, b->stream));
+    }
+    BHIP_TRY(hipEventRecord(b->ev0, b->stream));
+    for (int ep = e0; ep < e1; ++ep) {
+        if (o->use_graph) BHIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+        else BHIP_TRY(queue_epoch(ep, nullptr));
         b->launches += 1;
         b->updates += upd;
     }

@@ -1416,13 +1416,6 @@ int train_streamed(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const
     return CUMF_OK;
 }
 
-uint64_t fnv1a(const void* p, size_t n, uint64_t h)
-{
-    const unsigned char* b = (const unsigned char*)p;
-    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
-    return h;
-}
-
 }  // namespace
 
 extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int32_t e1, const int32_t* block_order,

@@ -1,8 +1,9 @@
-// Launch-shape resolution and the learning-rate table, shared by the
-// context's training entry points (capi.hip) and the allocation-free one
-// (noalloc.hip).
+// Launch-shape resolution, the learning-rate table and the graph-key hash,
+// shared by the context's training entry points (capi.hip), the
+// allocation-free one (noalloc.hip) and batched training (batch.hip).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/cumf_sgd_amd.h"
@@ -56,5 +57,14 @@ constexpr int64_t kHbmResidentP = 1ll << 30;
 // lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
 // (singleGPU/sgd_kernel.cu:137-154).
 void lr_table(const cumf_train_opts* o, int n, float* out);
+
+// FNV-1a over n bytes, continuing from h: the key of a captured epoch graph
+// (start from 1469598103934665603).
+inline uint64_t fnv1a(const void* p, size_t n, uint64_t h)
+{
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
 
 }  // namespace cumf

@@ -290,7 +290,9 @@ int cumf_batch_order(cumf_batch* batch, int32_t by);
  * (CUMF_SCHED_HOGWILD, or CUMF_SCHED_SERIAL = one worker per problem),
  * num_workers = total waves (0 = 64 per CU, shared in proportion to each
  * problem's bytes of work), alpha, beta, lrate, seed, batch_len,
- * pipeline_depth; k and the lambdas come from each problem. */
+ * pipeline_depth, use_graph (1: the epoch's launches are captured once as a
+ * HIP graph, re-captured when the problems or the shape change, and replayed
+ * per epoch); k and the lambdas come from each problem. */
 int cumf_batch_train(cumf_batch* batch, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end);
 /* The batch's launch shape: times one epoch for total waves in
  * {4, 8, 16, 32, 64} per CU and pipeline depth {2, 4, 8}, writes the fastest

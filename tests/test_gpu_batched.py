@@ -89,6 +89,42 @@ def test_batched_order_then_serial_matches_oracle(gpu, prec, by):
             assert np.array_equal(Q1.view(np.uint8), Qw.view(np.uint8)), (i, p.k)
 
 
+@pytest.mark.parametrize("prec", [1, 3, 4])
+def test_batched_graph_replay_equals_eager(gpu, prec):
+    """use_graph=1: an epoch's batched launch (with the FP8 rescales of every
+    problem) is captured once and replayed per epoch.  In serial mode the
+    factors equal the eager loop's bit for bit: across two train calls (the
+    device epoch counter restarts at each call's first epoch), with host
+    reads between the replays, and after cumf_batch_order moves the records
+    (the graph is captured again)."""
+    import cumf_sgd_amd as cs
+
+    probs = _problems(5, seed=14)
+    seed = 29
+
+    def run(graph):
+        with cs.Batch(prec) as b:
+            for p in probs:
+                u, v, r = (t.numpy() for t in (p.u, p.v, p.r))
+                b.add(u, v, r, p.m, p.n, p.k, 0.05, seed=seed)
+            o = cs.make_opts(schedule=cs.SERIAL, seed=seed, batch_len=32, use_graph=graph)
+            b.train(o, 0, 2)
+            mid = [b.get_factors(i, 0)[0].copy() for i in range(len(probs))]
+            b.train(o, 2, 3)
+            b.order(cs.ORDER_COLUMNS)
+            b.train(o, 3, 5)
+            return mid, [b.get_factors(i, w) for i in range(len(probs)) for w in (0, 1)]
+
+    mid_e, fin_e = run(0)
+    mid_g, fin_g = run(1)
+    for a, g in zip(mid_e, mid_g):
+        assert np.array_equal(a.view(np.uint8), g.view(np.uint8))
+    for (a, sa), (g, sg) in zip(fin_e, fin_g):
+        assert np.array_equal(a.view(np.uint8), g.view(np.uint8))
+        if sa is not None:
+            assert np.array_equal(sa, sg)
+
+
 def test_batched_hogwild_rmse_vs_serial_fp32(gpu):
     """24 problems of mixed k in BF16, 20 Hogwild epochs in one launch per
     epoch, against serial FP32 oracle solves of each problem.
