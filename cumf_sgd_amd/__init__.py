@@ -484,8 +484,9 @@ class Batch:
         return out.value
 
     def autotune(self, base: "TrainOpts"):
-        """Fastest total waves and pipeline depth for this batch (factors left
-        unchanged); returns (opts, epoch milliseconds)."""
+        """Fastest total waves, pipeline depth and (when base.batch_len is 0)
+        window length for this batch (factors left unchanged); returns
+        (opts, epoch milliseconds)."""
         best = TrainOpts()
         ms = ctypes.c_double()
         _check(lib.cumf_batch_autotune(self._h, ctypes.byref(base), ctypes.byref(best), ctypes.byref(ms)),

@@ -533,26 +533,31 @@ int cumf_batch_autotune(cumf_batch* b, const cumf_train_opts* base, cumf_train_o
     const int64_t keep_l = b->launches, keep_u = b->updates;
     const int waves_per_cu[5] = {4, 8, 16, 32, 64};
     const int depths[3] = {2, 4, 8};
+    // window lengths only when the caller leaves batch_len to the library
+    std::vector<int> lens{base->batch_len};
+    if (base->batch_len == 0) lens = {128, 64, 32};
     double best_t = 1e300;
     cumf_train_opts bo = *base;
     bool warm = false;
-    for (int wpc : waves_per_cu)
-        for (int D : depths) {
-            cumf_train_opts o = *base;
-            o.num_workers = b->cu_count * wpc;
-            o.pipeline_depth = D;
-            if (!warm) {
+    for (int len : lens)
+        for (int wpc : waves_per_cu)
+            for (int D : depths) {
+                cumf_train_opts o = *base;
+                o.batch_len = len;
+                o.num_workers = b->cu_count * wpc;
+                o.pipeline_depth = D;
+                if (!warm) {
+                    if ((rc = cumf_batch_train(b, &o, 0, 1))) return rc;
+                    warm = true;
+                }
+                const double before = b->kernel_ms;
                 if ((rc = cumf_batch_train(b, &o, 0, 1))) return rc;
-                warm = true;
+                const double t = b->kernel_ms - before;
+                if (t < best_t) {
+                    best_t = t;
+                    bo = o;
+                }
             }
-            const double before = b->kernel_ms;
-            if ((rc = cumf_batch_train(b, &o, 0, 1))) return rc;
-            const double t = b->kernel_ms - before;
-            if (t < best_t) {
-                best_t = t;
-                bo = o;
-            }
-        }
     size_t j = 0;
     for (const Problem& p : b->probs) {
         BHIP_TRY(hipMemcpyAsync(p.P, keep[j++], (size_t)p.m * p.k * eb, hipMemcpyDeviceToDevice, b->stream));

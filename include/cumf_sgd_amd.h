@@ -295,7 +295,8 @@ int cumf_batch_order(cumf_batch* batch, int32_t by);
  * per epoch); k and the lambdas come from each problem. */
 int cumf_batch_train(cumf_batch* batch, const cumf_train_opts* opts, int32_t epoch_begin, int32_t epoch_end);
 /* The batch's launch shape: times one epoch for total waves in
- * {4, 8, 16, 32, 64} per CU and pipeline depth {2, 4, 8}, writes the fastest
+ * {4, 8, 16, 32, 64} per CU, pipeline depth {2, 4, 8} and, when
+ * base->batch_len is 0, window length {128, 64, 32}, writes the fastest
  * into *best (other fields from base) and its epoch time (ms) into *best_ms,
  * and puts every problem's factors back.  A serial schedule returns base. */
 int cumf_batch_autotune(cumf_batch* batch, const cumf_train_opts* base, cumf_train_opts* best, double* best_ms);

@@ -179,7 +179,7 @@ def test_c4_full_config_256_problems(gpu):
     """The batched configuration at full size: 256 problems (users 1K-200K,
     items 500-50K, density 0.1-2%, k in {32, 64, 128}; 677M ratings), BF16
     storage, Hogwild, one launch per epoch with the wave count and pipeline
-    depth the batch autotuner picks, 20 epochs.
+    depth and window length the batch autotuner picks, 20 epochs.
 
     Checked against serial FP32 solves (batched serial mode, bit-exact with
     the oracle per problem) for the 187 problems of at most 2M ratings, and
@@ -228,7 +228,8 @@ def test_c4_full_config_256_problems(gpu):
             opts = cs.make_opts(schedule=sched, seed=seed)
             if tune:
                 opts, ms = b.autotune(opts)
-                print(f"autotuned: {opts.num_workers} waves, depth {opts.pipeline_depth} ({ms:.1f} ms/epoch)")
+                print(f"autotuned: {opts.num_workers} waves, depth {opts.pipeline_depth}, window {opts.batch_len} "
+                      f"({ms:.1f} ms/epoch)")
             t0 = time.perf_counter()
             b.train(opts, 0, epochs)
             t = time.perf_counter() - t0
