@@ -30,12 +30,12 @@ PREDICT_PATH = os.path.join(_HERE, "cumf_predict")
 
 FP32, FP16, BF16, FP8_E4M3, BF16_SR = 0, 1, 2, 3, 4
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16, "fp8": FP8_E4M3, "bf16sr": BF16_SR}
-AUTO, HOGWILD, SERIAL, GRID, STRATUM = -1, 0, 1, 2, 3
+AUTO, HOGWILD, SERIAL, GRID, STRATUM, TILE = -1, 0, 1, 2, 3, 4
 ORDER_ROWS, ORDER_COLUMNS = 1, 2
 # mf-predict -e criteria (test/mf.h:30-40)
 EVAL_RMSE, EVAL_MAE, EVAL_GKL, EVAL_LOGLOSS, EVAL_ACC = 0, 1, 2, 5, 6
 EVAL_ROW_MPR, EVAL_COL_MPR, EVAL_ROW_AUC, EVAL_COL_AUC = 10, 11, 12, 13
-SCHEDULES = {"hogwild": HOGWILD, "serial": SERIAL, "grid": GRID, "stratum": STRATUM}
+SCHEDULES = {"hogwild": HOGWILD, "serial": SERIAL, "grid": GRID, "stratum": STRATUM, "tile": TILE}
 ELEM_BYTES = {FP32: 4, FP16: 2, BF16: 2, FP8_E4M3: 1}
 STORAGE_DTYPE = {FP32: np.float32, FP16: np.uint16, BF16: np.uint16, FP8_E4M3: np.uint8, BF16_SR: np.uint16}
 
@@ -65,7 +65,7 @@ class TrainOpts(ctypes.Structure):
         ("lrate", ctypes.c_float),
         ("seed", ctypes.c_uint64),
         ("stratum_rounds", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("tiles_in_flight", ctypes.c_int32),
     ]
 
 
@@ -132,6 +132,9 @@ def _load():
         "cumf_scale_ratings": ([P, F32], ctypes.c_int),
         "cumf_partition": ([P, I32, I32, P], ctypes.c_int),
         "cumf_partition_strata": ([P, I32, P], ctypes.c_int),
+        "cumf_partition_tiles": ([P, I32, I32, P], ctypes.c_int),
+        "cumf_tile_rows": ([I32, I32], ctypes.c_int32),
+        "cumf_get_tiles": ([P, ctypes.POINTER(I64), P, P, P], ctypes.c_int),
         "cumf_set_blocks": ([P, I32, I32, P], ctypes.c_int),
         "cumf_init_factors": ([P, I32, I32, U64], ctypes.c_int),
         "cumf_set_factors": ([P, I32, P, P], ctypes.c_int),
@@ -316,6 +319,24 @@ class Context:
         off = np.zeros(s * s + 1, np.int64)
         _check(lib.cumf_partition_strata(self._h, s, _ptr(off)), "cumf_partition_strata")
         return off
+
+    def partition_tiles(self, s: int, tile_rows: int = 0) -> np.ndarray:
+        """cumf_partition_tiles: s x s strata, each block's records grouped by
+        row tile and ordered by column inside a tile; returns the block
+        offsets."""
+        off = np.zeros(s * s + 1, np.int64)
+        _check(lib.cumf_partition_tiles(self._h, s, tile_rows, _ptr(off)), "cumf_partition_tiles")
+        return off
+
+    def get_tiles(self):
+        """(rec_off[count + 1], row_begin[count], row_count[count]) of the tile partition."""
+        n = ctypes.c_int64()
+        _check(lib.cumf_get_tiles(self._h, ctypes.byref(n), None, None, None), "cumf_get_tiles")
+        off = np.zeros(n.value + 1, np.int64)
+        r0 = np.zeros(n.value, np.int32)
+        rc = np.zeros(n.value, np.int32)
+        _check(lib.cumf_get_tiles(self._h, ctypes.byref(n), _ptr(off), _ptr(r0), _ptr(rc)), "cumf_get_tiles")
+        return off, r0, rc
 
     def order_blocks(self, by: int) -> None:
         """cumf_order_blocks: stable sort of each block's records by column
@@ -532,7 +553,8 @@ def train_noalloc(opts: TrainOpts, precision: int, u, v, r, m: int, n: int, P, Q
 def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambda_p: float = 0.05,
               lambda_q: float | None = None, alpha: float = 0.08, beta: float = 0.3, lrate: float = 0.0,
               seed: int = 1, batch_len: int = 0, lanes_per_sample: int = 0, coherent: int = 0,
-              pipeline_depth: int = 0, use_graph: int = 0, stratum_rounds: int = 0) -> TrainOpts:
+              pipeline_depth: int = 0, use_graph: int = 0, stratum_rounds: int = 0,
+              tiles_in_flight: int = 0) -> TrainOpts:
     o = TrainOpts()
     o.k, o.schedule, o.num_workers = k, schedule, num_workers
     o.lambda_p = lambda_p
@@ -542,6 +564,7 @@ def make_opts(k: int = 128, schedule: int = HOGWILD, num_workers: int = 0, lambd
     o.pipeline_depth = pipeline_depth
     o.use_graph = use_graph
     o.stratum_rounds = stratum_rounds
+    o.tiles_in_flight = tiles_in_flight
     return o
 
 

@@ -420,7 +420,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     // task base plus a 32-bit offset
     int32_t off32 = 1;
     for (const Problem& p : b->probs)
-        if ((uint64_t)p.m * p.k * eb_of(b->prec) > UINT32_MAX || (uint64_t)p.n * p.k * eb_of(b->prec) > UINT32_MAX)
+        if ((uint64_t)p.m * p.k * eb_of(b->prec) > kBufRecords || (uint64_t)p.n * p.k * eb_of(b->prec) > kBufRecords)
             off32 = 0;
     // one epoch: the update launch, the FP8 rescales, and with epoch_dev the
     // device epoch counter's advance (the form captured into a graph)

@@ -103,6 +103,13 @@ struct cumf_ctx {
 
     int32_t ux = 1, vy = 1;
     std::vector<int64_t> offsets{0, 0};
+    // tiles of cumf_partition_tiles, block-major: tile t holds records
+    // [tile_rec[t], tile_rec[t+1]) and P rows [tile_row0[t], + tile_rows[t]);
+    // block b's tiles are [block_tile[b], block_tile[b+1]).  Empty when the
+    // current partition is not a tile partition.
+    std::vector<int64_t> tile_rec;
+    std::vector<int32_t> tile_row0, tile_rows, block_tile;
+    int32_t tile_rows_max = 0;
 
     int32_t k = 0, prec = -1;
     void* P = nullptr; size_t P_cap = 0;
@@ -228,9 +235,21 @@ void drop_streamed(cumf_ctx* c)
     c->streamed = false;
 }
 
+// The tile list describes the records' order and block bounds: any change
+// of either drops it.
+void drop_tiles(cumf_ctx* c)
+{
+    c->tile_rec.clear();
+    c->tile_row0.clear();
+    c->tile_rows.clear();
+    c->block_tile.clear();
+    c->tile_rows_max = 0;
+}
+
 int after_ratings(cumf_ctx* c, int64_t nnz, int32_t m, int32_t n)
 {
     drop_streamed(c);
+    drop_tiles(c);
     c->nnz = nnz;
     c->m = m;
     c->n = n;
@@ -519,6 +538,7 @@ int cumf_remap(cumf_ctx* c, const int32_t* p_map, const int32_t* q_map)
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->ux = c->vy = 1;
     c->offsets.assign({0, c->nnz});
+    drop_tiles(c);
     return CUMF_OK;
 }
 
@@ -543,6 +563,7 @@ int partition_with(cumf_ctx* c, const std::vector<int64_t>& bu, const std::vecto
     const int ux = (int)bu.size() - 1, vy = (int)bv.size() - 1;
     const int nb = ux * vy;
     if (nb < 1 || nb > 4096) return fail(CUMF_ERR_ARG, "grid of %d blocks unsupported (1..4096)", nb);
+    drop_tiles(c);
     const int64_t nnz = c->nnz;
     DevTmp useg, vseg, bnd, hist, tot, base, off, uo, vo, ro;
     HIP_TRY(hipMalloc(&useg.p, sizeof(int32_t) * (size_t)c->m));
@@ -692,6 +713,134 @@ int cumf_order_blocks(cumf_ctx* c, int32_t by)
     std::swap(*(void**)&c->v, vo.p);
     std::swap(*(void**)&c->r, ro.p);
     c->u_cap = c->v_cap = c->r_cap = (size_t)nnz;
+    drop_tiles(c);
+    return CUMF_OK;
+}
+
+int32_t cumf_tile_rows(int32_t k, int32_t precision)
+{
+    if (k < 1 || precision < CUMF_FP32 || precision > CUMF_BF16_SR) return 0;
+    return (int32_t)(kSgdTileDefaultBytes / ((int64_t)k * prec_bytes(precision)));
+}
+
+int cumf_partition_tiles(cumf_ctx* c, int32_t s, int32_t tile_rows, int64_t* offsets)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
+    if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (s < 1 || s > 64) return fail(CUMF_ERR_ARG, "strata %d out of range 1..64", s);
+    if (tile_rows < 0) return fail(CUMF_ERR_ARG, "tile_rows=%d must be >= 0", tile_rows);
+    if (c->nnz > (int64_t)UINT32_MAX)
+        return fail(CUMF_ERR_ARG, "%lld ratings: tiles support up to 2^32 - 1", (long long)c->nnz);
+    if (tile_rows == 0) tile_rows = c->have_factors ? cumf_tile_rows(c->k, c->prec) : cumf_tile_rows(128, CUMF_FP16);
+    std::vector<int32_t> hu, hv;
+    if ((rc = rating_degrees(c, hu, hv))) return rc;
+    const std::vector<int64_t> bu = balanced_bounds(hu, s, c->nnz);
+    if ((rc = partition_with(c, bu, balanced_bounds(hv, s, c->nnz), offsets))) return rc;
+
+    // Tiles of each row band: consecutive rows, a tile closed when it holds
+    // tile_rows rows or its share of the band's ratings (the band's count
+    // over ceil(rows / tile_rows)), so that a band's tiles carry about equal
+    // work where the row cap allows.
+    std::vector<int32_t> tile_of((size_t)std::max(1, c->m), 0);
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> band_tiles((size_t)s);   // (first row, rows)
+    int T = 1;
+    for (int i = 0; i < s; ++i) {
+        const int64_t r0 = bu[(size_t)i], r1 = bu[(size_t)i + 1];
+        if (r1 <= r0) continue;
+        int64_t band_nnz = 0;
+        for (int64_t x = r0; x < r1; ++x) band_nnz += hu[(size_t)x];
+        const int64_t nt = (r1 - r0 + tile_rows - 1) / tile_rows;
+        const int64_t share = std::max<int64_t>(1, (band_nnz + nt - 1) / nt);
+        int64_t first = r0, z = 0;
+        for (int64_t x = r0; x < r1; ++x) {
+            tile_of[(size_t)x] = (int32_t)band_tiles[(size_t)i].size();
+            z += hu[(size_t)x];
+            if (x + 1 - first == tile_rows || z >= share || x + 1 == r1) {
+                band_tiles[(size_t)i].emplace_back((int32_t)first, (int32_t)(x + 1 - first));
+                first = x + 1;
+                z = 0;
+            }
+        }
+        T = std::max(T, (int)band_tiles[(size_t)i].size());
+    }
+    const int nb = s * s;
+    const int64_t nnz = c->nnz;
+    if (nnz > 0) {
+        const uint64_t stride = (uint64_t)std::max(1, c->n);
+        const size_t kb = order_key_bytes(nb * T, stride);
+        DevTmp dof, dtile, cnt, keys, keys_out, idx, idx_out, uo, vo, ro, temp;
+        size_t temp_bytes = 0;
+        HIP_TRY(hipMalloc(&dof.p, sizeof(int64_t) * (size_t)(nb + 1)));
+        HIP_TRY(hipMalloc(&dtile.p, sizeof(int32_t) * tile_of.size()));
+        HIP_TRY(hipMalloc(&cnt.p, sizeof(int32_t) * (size_t)nb * T));
+        HIP_TRY(hipMalloc(&keys.p, kb * (size_t)nnz));
+        HIP_TRY(hipMalloc(&keys_out.p, kb * (size_t)nnz));
+        HIP_TRY(hipMalloc(&idx.p, sizeof(uint32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&idx_out.p, sizeof(uint32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&uo.p, sizeof(int32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&vo.p, sizeof(int32_t) * (size_t)nnz));
+        HIP_TRY(hipMalloc(&ro.p, sizeof(float) * (size_t)nnz));
+        HIP_TRY(hipMemcpyAsync(dof.p, c->offsets.data(), sizeof(int64_t) * (size_t)(nb + 1), hipMemcpyHostToDevice,
+                               c->stream));
+        HIP_TRY(hipMemcpyAsync(dtile.p, tile_of.data(), sizeof(int32_t) * tile_of.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (size_t)nb * T, c->stream));
+        HIP_TRY(launch_order_tiles(nnz, c->u, c->v, c->r, (const int64_t*)dof.p, nb, (const int32_t*)dtile.p, T, stride,
+                                   keys.p, keys_out.p, (uint32_t*)idx.p, (uint32_t*)idx_out.p, (int32_t*)cnt.p,
+                                   (int32_t*)uo.p, (int32_t*)vo.p, (float*)ro.p, &temp.p, &temp_bytes, c->stream));
+        std::vector<int32_t> counts((size_t)nb * T);
+        HIP_TRY(hipMemcpyAsync(counts.data(), cnt.p, sizeof(int32_t) * counts.size(), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        std::swap(*(void**)&c->u, uo.p);
+        std::swap(*(void**)&c->v, vo.p);
+        std::swap(*(void**)&c->r, ro.p);
+        c->u_cap = c->v_cap = c->r_cap = (size_t)nnz;
+        // the non-empty tiles, block by block (window draws are tagged with
+        // a tile's index in this list)
+        c->block_tile.assign((size_t)nb + 1, 0);
+        c->tile_rec.assign(1, 0);
+        for (int b = 0; b < nb; ++b) {
+            const int i = b / s;
+            int64_t at = c->offsets[(size_t)b];
+            for (size_t t = 0; t < band_tiles[(size_t)i].size(); ++t) {
+                const int32_t cn = counts[(size_t)b * T + t];
+                if (cn == 0) continue;
+                at += cn;
+                c->tile_rec.push_back(at);
+                c->tile_row0.push_back(band_tiles[(size_t)i][t].first);
+                c->tile_rows.push_back(band_tiles[(size_t)i][t].second);
+            }
+            if (at != c->offsets[(size_t)b + 1]) {
+                drop_tiles(c);
+                return fail(CUMF_ERR_STATE, "tile counts of block %d do not add up to its ratings", b);
+            }
+            c->block_tile[(size_t)b + 1] = (int32_t)c->tile_row0.size();
+        }
+        if (c->tile_row0.size() >= (1u << 24)) {
+            drop_tiles(c);
+            return fail(CUMF_ERR_ARG, "%zu tiles: at most 2^24 - 1 (raise tile_rows)", c->tile_row0.size());
+        }
+    } else {
+        c->block_tile.assign((size_t)nb + 1, 0);
+        c->tile_rec.assign(1, 0);
+    }
+    c->tile_rows_max = tile_rows;
+    return CUMF_OK;
+}
+
+int cumf_get_tiles(cumf_ctx* c, int64_t* count, int64_t* rec_off, int32_t* row_begin, int32_t* row_count)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!count) return fail(CUMF_ERR_ARG, "null count");
+    if (c->block_tile.empty()) return fail(CUMF_ERR_STATE, "no tile partition (cumf_partition_tiles)");
+    const size_t nt = c->tile_row0.size();
+    *count = (int64_t)nt;
+    if (rec_off) std::copy(c->tile_rec.begin(), c->tile_rec.end(), rec_off);
+    if (row_begin) std::copy(c->tile_row0.begin(), c->tile_row0.end(), row_begin);
+    if (row_count) std::copy(c->tile_rows.begin(), c->tile_rows.end(), row_count);
     return CUMF_OK;
 }
 
@@ -826,6 +975,7 @@ int cumf_stream_ratings(cumf_ctx* c, const cumf_node* R, int64_t nnz, int32_t m,
     c->n = n;
     c->ux = c->vy = s;
     c->offsets = off;
+    drop_tiles(c);
     c->streamed = true;
     c->have_ratings = true;
     if (offsets) std::copy(off.begin(), off.end(), offsets);
@@ -846,6 +996,7 @@ int cumf_set_blocks(cumf_ctx* c, int32_t ux, int32_t vy, const int64_t* offsets)
     c->ux = ux;
     c->vy = vy;
     c->offsets.assign(offsets, offsets + nb + 1);
+    drop_tiles(c);
     return CUMF_OK;
 }
 
@@ -1152,6 +1303,14 @@ SgdTask base_task(cumf_ctx* c, const cumf_train_opts* o, int64_t beg, int64_t en
     return t;
 }
 
+// Waves of a tile workgroup holding `workers` workers of S per wave: 4, 8
+// or 16.
+int tile_waves(int workers, int S)
+{
+    const int w = (workers + S - 1) / S;
+    return w <= 4 ? 4 : (w <= 8 ? 8 : 16);
+}
+
 // One launch of the update kernel: a slice of the uploaded task table.
 struct LaunchRec {
     int64_t task_off;
@@ -1191,6 +1350,87 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
             std::vector<SgdTask> t{base_task(c, o, beg, end, sh.W, sh.L, (uint32_t)blk)};
             push(t, spread, spread_blocks);
         }
+    } else if (o->schedule == CUMF_SCHED_TILE) {
+        // tiles: sub-epoch d trains the tiles of blocks (i, (i + d) mod s).
+        // A block's tiles stay on one XCD (they share the block's Q rows
+        // through its L2); the blocks of a diagonal go to XCDs largest first
+        // onto the least-loaded one, and each XCD's tiles run largest first
+        // (workgroups are dispatched to an XCD in order).  A tile gets one
+        // workgroup of per_tile workers, fewer when it has fewer windows.
+        const int s = c->ux;
+        const int R = std::max(1, o->stratum_rounds);
+        const int per_tile = sh.W;   // workers per tile (cumf_train resolved it)
+        auto block_size = [&](int blk) { return c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk]; };
+        // diag[d][x]: the tiles XCD x trains in sub-epoch d, in launch order
+        std::vector<std::vector<std::vector<SgdTask>>> diag((size_t)s, std::vector<std::vector<SgdTask>>((size_t)num_xcd));
+        for (int d = 0; d < s; ++d) {
+            std::vector<int> order((size_t)s);
+            for (int i = 0; i < s; ++i) order[(size_t)i] = i * s + (i + d) % s;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return block_size(a) > block_size(b); });
+            std::vector<int64_t> load((size_t)num_xcd, 0);
+            for (int blk : order) {
+                if (block_size(blk) == 0) continue;
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[(size_t)x] += block_size(blk);
+                for (int32_t t = c->block_tile[(size_t)blk]; t < c->block_tile[(size_t)blk + 1]; ++t) {
+                    // the fewest windows per worker the tile's workers allow,
+                    // then the fewest workers that keep them (least over-sampling)
+                    const int64_t beg = c->tile_rec[(size_t)t], end = c->tile_rec[(size_t)t + 1];
+                    const int64_t uc = (end - beg + (int64_t)per_tile * sh.L - 1) / ((int64_t)per_tile * sh.L);
+                    const int w = (int)((end - beg + uc * sh.L - 1) / (uc * sh.L));
+                    SgdTask tk = base_task(c, o, beg, end, w, sh.L, (uint32_t)t);
+                    tk.row_begin = c->tile_row0[(size_t)t];
+                    tk.row_count = c->tile_rows[(size_t)t];
+                    diag[(size_t)d][(size_t)x].push_back(tk);
+                }
+            }
+            for (auto& l : diag[(size_t)d])
+                std::stable_sort(l.begin(), l.end(), [](const SgdTask& a, const SgdTask& b) { return a.nnz > b.nnz; });
+        }
+        // A launch holds up to F tiles per XCD (F = tiles_in_flight, 0: all):
+        // task x * per + j is XCD x's j-th tile, the rest padding (nnz 0).
+        auto emit = [&](const std::vector<std::vector<SgdTask>>& lists) {
+            const int F = o->tiles_in_flight;
+            size_t most = 0;
+            for (const auto& l : lists) most = std::max(most, l.size());
+            for (size_t c0 = 0; c0 < most; c0 += F > 0 ? (size_t)F : most) {
+                const size_t c1 = F > 0 ? std::min(most, c0 + (size_t)F) : most;
+                size_t per = 0;
+                for (const auto& l : lists) per = std::max(per, std::min(l.size(), c1) - std::min(l.size(), c0));
+                if (per == 0) continue;
+                std::vector<SgdTask> t((size_t)num_xcd * per, SgdTask{});
+                for (int x = 0; x < num_xcd; ++x)
+                    for (size_t j = c0; j < std::min(lists[(size_t)x].size(), c1); ++j)
+                        t[(size_t)x * per + (j - c0)] = lists[(size_t)x][j];
+                push(t, num_xcd, (int32_t)(num_xcd * per));
+            }
+        };
+        // pass rr runs N*(rr+1)/R - N*rr/R of a tile's N windows, as for strata
+        std::vector<std::vector<std::vector<int32_t>>> next_window((size_t)s);
+        for (int d = 0; d < s; ++d) {
+            next_window[(size_t)d].resize((size_t)num_xcd);
+            for (int x = 0; x < num_xcd; ++x) next_window[(size_t)d][(size_t)x].assign(diag[(size_t)d][(size_t)x].size(), 0);
+        }
+        for (int rr = 0; rr < R; ++rr)
+            for (int d = 0; d < s; ++d) {
+                std::vector<std::vector<SgdTask>> lists((size_t)num_xcd);
+                for (int x = 0; x < num_xcd; ++x)
+                    for (size_t j = 0; j < diag[(size_t)d][(size_t)x].size(); ++j) {
+                        SgdTask q = diag[(size_t)d][(size_t)x][j];
+                        if (R > 1) {
+                            const int64_t n_all = (int64_t)q.worker_count * q.update_count;
+                            const int64_t n_pass = n_all * (rr + 1) / R - n_all * rr / R;
+                            if (n_pass == 0) continue;
+                            const int32_t wc = q.worker_count;
+                            q.update_count = (int32_t)((n_pass + wc - 1) / wc);
+                            q.worker_count = (int32_t)((n_pass + q.update_count - 1) / q.update_count);
+                            q.window_begin = next_window[(size_t)d][(size_t)x][j];
+                            next_window[(size_t)d][(size_t)x][j] += q.update_count;
+                        }
+                        lists[(size_t)x].push_back(q);
+                    }
+                emit(lists);
+            }
     } else {
         // stratum: sub-epoch d trains blocks (i, (i + d) mod s), i = 0..s-1,
         // which share no P and no Q rows.  Each block runs on one XCD; the
@@ -1309,7 +1549,7 @@ int queue_one(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const Laun
         SgdLaunch a{};
         // factors under 4 GB each: rows addressed as launch-wide base + 32-bit offset
         const uint64_t eb = (uint64_t)prec_bytes(c->prec);
-        if ((uint64_t)c->m * c->k * eb <= UINT32_MAX && (uint64_t)c->n * c->k * eb <= UINT32_MAX) {
+        if ((uint64_t)c->m * c->k * eb <= kBufRecords && (uint64_t)c->n * c->k * eb <= kBufRecords) {
             a.P = c->P;
             a.Q = c->Q;
             a.ps = c->ps;
@@ -1330,8 +1570,12 @@ int queue_one(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const Laun
             if (!ev0 || !ev1) return fail(CUMF_ERR_HIP, "hipEventCreate failed");
             HIP_TRY(hipEventRecord(ev0, c->stream));
         }
-        HIP_TRY(launch_sgd(a, l.blocks ? l.blocks : sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D,
-                           c->stream));
+        if (o->schedule == CUMF_SCHED_TILE)
+            HIP_TRY(launch_sgd_tile(a, l.blocks, c->k, c->prec, sh.G, sh.D, c->tile_rows_max, tile_waves(sh.W, sh.S),
+                                    c->stream));
+        else
+            HIP_TRY(launch_sgd(a, l.blocks ? l.blocks : sh.blocks, c->k, c->prec, sh.G, o->coherent != 0, sh.D,
+                               c->stream));
         if (timed) HIP_TRY(hipEventRecord(ev1, c->stream));
     }
     return CUMF_OK;
@@ -1428,7 +1672,8 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     if (!c->have_factors) return fail(CUMF_ERR_STATE, "factors not initialised");
     if (o->k != c->k) return fail(CUMF_ERR_ARG, "opts k=%d but factors have k=%d", o->k, c->k);
     if (e0 < 0 || e1 < e0 || e1 > 1 << 20) return fail(CUMF_ERR_ARG, "bad epoch range [%d,%d)", e0, e1);
-    if (o->schedule < CUMF_SCHED_HOGWILD || o->schedule > CUMF_SCHED_STRATUM) return fail(CUMF_ERR_ARG, "bad schedule");
+    if (o->schedule < CUMF_SCHED_HOGWILD || o->schedule > CUMF_SCHED_TILE) return fail(CUMF_ERR_ARG, "bad schedule");
+    if (o->tiles_in_flight < 0) return fail(CUMF_ERR_ARG, "tiles_in_flight=%d must be >= 0", o->tiles_in_flight);
     if (o->use_graph != 0 && o->use_graph != 1) return fail(CUMF_ERR_ARG, "use_graph must be 0 or 1");
     if (o->stratum_rounds < 0 || o->stratum_rounds > 64)
         return fail(CUMF_ERR_ARG, "stratum_rounds=%d out of range 0..64", o->stratum_rounds);
@@ -1436,12 +1681,33 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
 
     const int nb = c->ux * c->vy;
     const bool stratum = o->schedule == CUMF_SCHED_STRATUM;
+    const bool tiled = o->schedule == CUMF_SCHED_TILE;
     if (stratum && (c->ux != c->vy || c->ux < 2))
         return fail(CUMF_ERR_STATE, "stratum schedule needs an s x s partition (have %d x %d)", c->ux, c->vy);
-    const int num_xcd = stratum ? 8 : 1;
+    if (tiled && c->block_tile.empty())
+        return fail(CUMF_ERR_STATE, "tile schedule needs the partition of cumf_partition_tiles");
+    const int num_xcd = stratum || tiled ? 8 : 1;
     Shape sh;
     if ((rc = resolve_shape(shape_input(c, o), o, &sh, num_xcd))) return rc;
     if (stratum) sh.W = sh.blocks * kSgdWavesPerBlock * sh.S;   // every wave of every XCD works
+    if (tiled) {
+        // one workgroup per tile: the default lane group, no coherent mode,
+        // the tile and the staged records in LDS
+        if (sh.G != sgd_default_group(c->k, c->prec))
+            return fail(CUMF_ERR_ARG, "tile schedule runs lanes_per_sample=%d only (k=%d precision=%d)",
+                        sgd_default_group(c->k, c->prec), c->k, c->prec);
+        if (o->coherent) return fail(CUMF_ERR_ARG, "tile schedule has no coherent mode");
+        if (sh.D != 2) return fail(CUMF_ERR_ARG, "tile schedule runs pipeline_depth 2");
+        const int most = kSgdTileMaxWaves * sh.S;
+        if (o->num_workers > most)
+            return fail(CUMF_ERR_ARG, "tile schedule: num_workers=%d per tile exceeds %d", o->num_workers, most);
+        sh.W = o->num_workers > 0 ? o->num_workers : kSgdTileWaves * sh.S;   // workers per tile
+        const size_t lds = sgd_tile_lds_bytes(c->k, c->prec, sh.G, sh.L, c->tile_rows_max, tile_waves(sh.W, sh.S));
+        if (lds > sgd_tile_lds_limit())
+            return fail(CUMF_ERR_ARG, "tiles of %d rows at k=%d precision=%d with %d-record windows need %zu bytes of LDS "
+                        "(at most %zu): partition with fewer tile rows", c->tile_rows_max, c->k, c->prec, sh.L, lds,
+                        sgd_tile_lds_limit());
+    }
     if ((rc = upload_lr(c, o, e1 + 4))) return rc;
 
     c->last.num_workers = sh.W;
@@ -1516,7 +1782,8 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         // walks.
         if ((rc = ensure(&c->epoch_dev, &c->epoch_dev_cap, 1))) return rc;
         uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
-        const int32_t shape[11] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec, c->k, c->m, c->n};
+        const int32_t shape[13] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec, c->k, c->m, c->n,
+                                   o->schedule, c->tile_rows_max};
         key = fnv1a(shape, sizeof(shape), key);
         key = fnv1a(&o->seed, sizeof(o->seed), key);
         const void* addr[7] = {c->tasks, c->epoch_dev, c->P, c->Q, c->ps, c->qs, c->lr};
@@ -1609,6 +1876,16 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
 
     // a field the caller set (non-zero) stays fixed; the zero ones are searched
     std::vector<int> groups{64, 32, 16, 8}, depths{2, 4, 8}, waves_per_cu{32, 64, 128, 256, 512}, lens{128, 64, 32};
+    const bool tiled = base->schedule == CUMF_SCHED_TILE;
+    std::vector<int> tile_workers{0};
+    if (tiled) {
+        // tile launches: the default lane group, depth 2, workgroups of 4, 8
+        // or 16 waves (workers per tile) unless the caller fixed them
+        groups = {sgd_default_group(c->k, c->prec)};
+        depths = {2};
+        const int S = 64 / groups[0];
+        if (!base->num_workers) tile_workers = {4 * S, 8 * S, 16 * S};
+    }
     if (base->lanes_per_sample) groups = {base->lanes_per_sample};
     if (base->pipeline_depth) depths = {base->pipeline_depth};
     if (base->batch_len) lens = {base->batch_len};
@@ -1621,13 +1898,18 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
             if (base->coherent && (c->k / G) * (int)eb < 4) continue;
             for (int D : depths) {
                 if (L % D) continue;
-                for (size_t wi = 0; wi < (base->num_workers ? 1 : waves_per_cu.size()); ++wi) {
+                const size_t nw = base->num_workers ? 1 : (tiled ? tile_workers.size() : waves_per_cu.size());
+                for (size_t wi = 0; wi < nw; ++wi) {
                     cumf_train_opts o = *base;
                     o.lanes_per_sample = G;
                     o.pipeline_depth = D;
                     o.batch_len = L;
-                    o.num_workers = base->num_workers ? base->num_workers
-                                                      : default_workers(shape_input(c, base), 64 / G, L, waves_per_cu[wi]);
+                    if (base->num_workers) o.num_workers = base->num_workers;
+                    else if (tiled) o.num_workers = tile_workers[wi];
+                    else o.num_workers = default_workers(shape_input(c, base), 64 / G, L, waves_per_cu[wi]);
+                    if (tiled && sgd_tile_lds_bytes(c->k, c->prec, G, L, c->tile_rows_max,
+                                                    tile_waves(o.num_workers, 64 / G)) > sgd_tile_lds_limit())
+                        continue;
                     o.use_graph = 0;
                     if (!warm) {   // first launch of the code object is not timed
                         if ((rc = cumf_train(c, &o, 0, 1, nullptr, 0))) return rc;

@@ -312,6 +312,94 @@ __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
     }
 }
 
+// The same row chunks through a buffer resource (raw, stride 0) over a
+// factor matrix of at most kBufRecords bytes.  A lane whose offset is
+// kBufSkip lies out of range: its load returns 0 and its store is dropped,
+// with no memory access.  Every lane's instruction is issued either way, so
+// the number of loads and stores in flight stays the same on every path and
+// the waits for a row can stay partial.
+constexpr uint32_t kBufRecords = 0xFFFFFF00u;
+constexpr uint32_t kBufSkip = 0xFFFFFFF0u;
+constexpr int kBufFlags = 0x00020000;   // gfx9 raw buffer: 32-bit data format, no swizzle
+
+__device__ inline __amdgpu_buffer_rsrc_t factor_rsrc(void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)kBufRecords, kBufFlags);
+}
+
+template <int TB>
+__device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    Words<(TB + 3) / 4> w;
+    if constexpr (TB == 2) {
+        w.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    } else if constexpr (TB == 4) {
+        w.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    } else if constexpr (TB == 8) {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        w.w[0] = x.x;
+        w.w[1] = x.y;
+    } else {
+        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        w.w[0] = x.x; w.w[1] = x.y; w.w[2] = x.z; w.w[3] = x.w;
+    }
+    return w;
+}
+
+template <int TB>
+__device__ inline void store_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, const Words<(TB + 3) / 4>& w)
+{
+    if constexpr (TB == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)w.w[0], r, off, 0, 0);
+    } else if constexpr (TB == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(w.w[0], r, off, 0, 0);
+    } else if constexpr (TB == 8) {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(u32x2{w.w[0], w.w[1]}, r, off, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[0], w.w[1], w.w[2], w.w[3]}, r, off, 0, 0);
+    }
+}
+
+// The same row chunks in LDS (a tile of P rows held by one workgroup).
+typedef __attribute__((address_space(3))) char lchar;
+
+template <int TB>
+__device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
+{
+    Words<(TB + 3) / 4> r;
+    if constexpr (TB == 2) {
+        r.w[0] = *(const __attribute__((address_space(3))) uint16_t*)p;
+    } else if constexpr (TB == 4) {
+        r.w[0] = *(const __attribute__((address_space(3))) uint32_t*)p;
+    } else if constexpr (TB == 8) {
+        const uint64_t x = *(const __attribute__((address_space(3))) uint64_t*)p;
+        r.w[0] = (uint32_t)x;
+        r.w[1] = (uint32_t)(x >> 32);
+    } else {
+        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        const u32x4 x = *(const __attribute__((address_space(3))) u32x4*)p;
+        r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+    }
+    return r;
+}
+
+template <int TB>
+__device__ inline void store_lds(lchar* p, const Words<(TB + 3) / 4>& r)
+{
+    if constexpr (TB == 2) {
+        *(__attribute__((address_space(3))) uint16_t*)p = (uint16_t)r.w[0];
+    } else if constexpr (TB == 4) {
+        *(__attribute__((address_space(3))) uint32_t*)p = r.w[0];
+    } else if constexpr (TB == 8) {
+        *(__attribute__((address_space(3))) uint64_t*)p = ((uint64_t)r.w[1] << 32) | r.w[0];
+    } else {
+        *(__attribute__((address_space(3))) u32x4*)p = u32x4{r.w[0], r.w[1], r.w[2], r.w[3]};
+    }
+}
+
 // Element i of a packed chunk, widened to FP32 (row scale applied by caller).
 // OCP E4M3 byte `sel` of a word, widened exactly (v_cvt_f32_fp8 on gfx950).
 template <int SEL>

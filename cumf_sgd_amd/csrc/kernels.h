@@ -29,6 +29,8 @@ struct SgdTask {
     int32_t wave_count;       // batched launches: waves owned by the task
     int32_t window_begin;     // the task's first window index in the epoch (its windows
                               // are window_begin + [0, update_count))
+    int32_t row_begin;        // tile launches: the task's P rows [row_begin, row_begin + row_count),
+    int32_t row_count;        // held in the workgroup's LDS for the launch
 };
 
 struct SgdLaunch {
@@ -57,6 +59,21 @@ bool sgd_shape_supported(int k, int prec, int G);
 int sgd_default_group(int k, int prec);
 // D: software-pipeline depth (2, 4 or 8; must divide the batch length)
 hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bool coh, int D, hipStream_t s);
+// Tile launch: one workgroup of `waves` waves (4, 8 or 16) per task (a tile:
+// the ratings of row_count consecutive P rows within one column range), the
+// tile's P rows held in LDS for the launch.  Lane group: the default one.
+// Workgroups go XCD-major: task (x * (blocks / num_xcd) + j) runs on XCD x.
+constexpr int kSgdTileWaves = 8;       // default workgroup
+constexpr int kSgdTileMaxWaves = 16;
+// Largest P tile a workgroup holds (bytes of LDS), and the default one: two
+// workgroups (16 waves) per CU with the record staging of 32-record windows.
+constexpr int kSgdTileMaxBytes = 128 * 1024;
+constexpr int kSgdTileDefaultBytes = 64 * 1024;
+// LDS bytes of a tile launch: the tile, then each wave's staged records.
+size_t sgd_tile_lds_bytes(int k, int prec, int G, int L, int rows, int waves);
+size_t sgd_tile_lds_limit();
+hipError_t launch_sgd_tile(const SgdLaunch& a, int blocks, int k, int prec, int G, int D, int rows, int waves,
+                           hipStream_t s);
 // Batched launch: tasks of mixed k (32, 64, 128), lane group k/2 (k/4 for FP8).
 hipError_t launch_sgd_batched(const SgdLaunch& a, int blocks, int prec, int D, hipStream_t s);
 int sgd_batch_group(int k, int prec);
@@ -109,5 +126,13 @@ hipError_t launch_order_blocks(int64_t nnz, const int32_t* ids, const int64_t* o
                                void* keys_out, uint32_t* idx, uint32_t* idx_out, const int32_t* u, const int32_t* v,
                                const float* r, int32_t* uo, int32_t* vo, float* ro, void** temp, size_t* temp_bytes,
                                hipStream_t st);
+// Stable sort of each block's records by (tile of the row, column), the
+// tile of row u being tile_of[u] (< T) within its row band; counts (nb * T
+// entries, zeroed by the caller) receives the records per (block, tile).
+// keys/keys_out hold order_key_bytes(nb * T, stride) bytes per record.
+hipError_t launch_order_tiles(int64_t nnz, const int32_t* u, const int32_t* v, const float* r, const int64_t* off,
+                              int nb, const int32_t* tile_of, int T, uint64_t stride, void* keys, void* keys_out,
+                              uint32_t* idx, uint32_t* idx_out, int32_t* counts, int32_t* uo, int32_t* vo, float* ro,
+                              void** temp, size_t* temp_bytes, hipStream_t st);
 
 }  // namespace cumf

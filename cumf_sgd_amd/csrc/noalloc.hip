@@ -125,7 +125,7 @@ int train_noalloc(int device, hipStream_t st, const cumf_train_opts* o, int32_t 
     SgdLaunch a{};
     // factors under 4 GB each: rows addressed as launch-wide base + 32-bit offset
     const uint64_t eb = prec == CUMF_FP32 ? 4 : (prec == CUMF_FP8_E4M3 ? 1 : 2);
-    if ((uint64_t)m * o->k * eb <= UINT32_MAX && (uint64_t)n * o->k * eb <= UINT32_MAX) {
+    if ((uint64_t)m * o->k * eb <= kBufRecords && (uint64_t)n * o->k * eb <= kBufRecords) {
         a.P = P;
         a.Q = Q;
         a.ps = ps;

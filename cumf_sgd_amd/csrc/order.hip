@@ -50,17 +50,34 @@ unsigned bits_for(uint64_t x)
     return b;
 }
 
+// Tile keys: record i of block b (prefix offsets off) whose row u is in tile
+// t = tile_of[u] of its row band gets the key (b * T + t) * stride + v, and
+// counts[b * T + t] counts the records of each (block, tile).
 template <typename K>
-hipError_t sort_keys(int64_t nnz, const int32_t* ids, const int64_t* off, int nb, uint64_t stride, void* keys,
-                     void* keys_out, uint32_t* idx, uint32_t* idx_out, void** temp, size_t* temp_bytes, hipStream_t st)
+__global__ void tile_key_kernel(const int32_t* __restrict__ u, const int32_t* __restrict__ v, int64_t nnz,
+                                const int64_t* __restrict__ off, int nb, const int32_t* __restrict__ tile_of, int T,
+                                uint64_t stride, K* __restrict__ key, uint32_t* __restrict__ idx,
+                                int32_t* __restrict__ counts)
 {
-    const unsigned end_bit = bits_for((uint64_t)nb * stride);
-    const int threads = 256;
-    const int64_t grid = (nnz + threads - 1) / threads;
-    hipLaunchKernelGGL((block_key_kernel<K>), dim3((unsigned)grid), dim3(threads), 0, st, ids, nnz, off, nb, stride,
-                       (K*)keys, idx);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint64_t bt = (uint64_t)lo * (uint64_t)T + (uint64_t)tile_of[u[i]];
+    key[i] = (K)(bt * stride + (uint64_t)(uint32_t)v[i]);
+    idx[i] = (uint32_t)i;
+    atomicAdd(counts + bt, 1);
+}
+
+template <typename K>
+hipError_t sort_pairs(int64_t nnz, unsigned end_bit, void* keys, void* keys_out, uint32_t* idx, uint32_t* idx_out,
+                      void** temp, size_t* temp_bytes, hipStream_t st)
+{
+    hipError_t e;
     size_t need = 0;
     e = rocprim::radix_sort_pairs(nullptr, need, (K*)keys, (K*)keys_out, idx, idx_out, (size_t)nnz, 0u, end_bit, st);
     if (e != hipSuccess) return e;
@@ -72,6 +89,35 @@ hipError_t sort_keys(int64_t nnz, const int32_t* ids, const int64_t* off, int nb
         *temp_bytes = need;
     }
     return rocprim::radix_sort_pairs(*temp, need, (K*)keys, (K*)keys_out, idx, idx_out, (size_t)nnz, 0u, end_bit, st);
+}
+
+template <typename K>
+hipError_t sort_keys(int64_t nnz, const int32_t* ids, const int64_t* off, int nb, uint64_t stride, void* keys,
+                     void* keys_out, uint32_t* idx, uint32_t* idx_out, void** temp, size_t* temp_bytes, hipStream_t st)
+{
+    const unsigned end_bit = bits_for((uint64_t)nb * stride);
+    const int threads = 256;
+    const int64_t grid = (nnz + threads - 1) / threads;
+    hipLaunchKernelGGL((block_key_kernel<K>), dim3((unsigned)grid), dim3(threads), 0, st, ids, nnz, off, nb, stride,
+                       (K*)keys, idx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sort_pairs<K>(nnz, end_bit, keys, keys_out, idx, idx_out, temp, temp_bytes, st);
+}
+
+template <typename K>
+hipError_t sort_tile_keys(int64_t nnz, const int32_t* u, const int32_t* v, const int64_t* off, int nb,
+                          const int32_t* tile_of, int T, uint64_t stride, void* keys, void* keys_out, uint32_t* idx,
+                          uint32_t* idx_out, int32_t* counts, void** temp, size_t* temp_bytes, hipStream_t st)
+{
+    const unsigned end_bit = bits_for((uint64_t)nb * (uint64_t)T * stride);
+    const int threads = 256;
+    const int64_t grid = (nnz + threads - 1) / threads;
+    hipLaunchKernelGGL((tile_key_kernel<K>), dim3((unsigned)grid), dim3(threads), 0, st, u, v, nnz, off, nb, tile_of, T,
+                       stride, (K*)keys, idx, counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sort_pairs<K>(nnz, end_bit, keys, keys_out, idx, idx_out, temp, temp_bytes, st);
 }
 
 }  // namespace
@@ -87,6 +133,29 @@ hipError_t launch_order_blocks(int64_t nnz, const int32_t* ids, const int64_t* o
     hipError_t e = order_key_bytes(nb, stride) == 8
                        ? sort_keys<uint64_t>(nnz, ids, off, nb, stride, keys, keys_out, idx, idx_out, temp, temp_bytes, st)
                        : sort_keys<uint32_t>(nnz, ids, off, nb, stride, keys, keys_out, idx, idx_out, temp, temp_bytes, st);
+    if (e != hipSuccess) return e;
+    const int threads = 256;
+    const int64_t grid = (nnz + threads - 1) / threads;
+    hipLaunchKernelGGL(gather_records_kernel, dim3((unsigned)grid), dim3(threads), 0, st, idx_out, nnz, u, v, r, uo, vo,
+                       ro);
+    return hipGetLastError();
+}
+
+}  // namespace cumf
+
+namespace cumf {
+
+hipError_t launch_order_tiles(int64_t nnz, const int32_t* u, const int32_t* v, const float* r, const int64_t* off,
+                              int nb, const int32_t* tile_of, int T, uint64_t stride, void* keys, void* keys_out,
+                              uint32_t* idx, uint32_t* idx_out, int32_t* counts, int32_t* uo, int32_t* vo, float* ro,
+                              void** temp, size_t* temp_bytes, hipStream_t st)
+{
+    if (nnz <= 0) return hipSuccess;
+    hipError_t e = order_key_bytes(nb * T, stride) == 8
+                       ? sort_tile_keys<uint64_t>(nnz, u, v, off, nb, tile_of, T, stride, keys, keys_out, idx, idx_out,
+                                                  counts, temp, temp_bytes, st)
+                       : sort_tile_keys<uint32_t>(nnz, u, v, off, nb, tile_of, T, stride, keys, keys_out, idx, idx_out,
+                                                  counts, temp, temp_bytes, st);
     if (e != hipSuccess) return e;
     const int threads = 256;
     const int64_t grid = (nnz + threads - 1) / threads;

@@ -74,12 +74,15 @@ struct Pending {
 // One worker (a slot of G lanes) running its windows of task T.  OFF32: the
 // factors are at the wave-uniform bases FP, FQ (row scales FPs, FQs), under
 // 4 GB each, and rows are addressed as base plus a 32-bit offset; otherwise
-// at T's, with 64-bit offsets.
-template <int K, int G, int PREC, bool COH, int D, bool OFF32 = false>
+// at T's, with 64-bit offsets.  PT: the task's P rows [T.row_begin,
+// T.row_begin + T.row_count) are in LDS at Pt (row-major, the workgroup's
+// tile), and P rows are read and written there; every record of the task
+// has its row in that range.
+template <int K, int G, int PREC, bool COH, int D, bool OFF32 = false, bool PT = false>
 __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T, uint32_t lw, int slot, int ls,
                                            int32_t* su, int32_t* sv, float* sr, void* FP = nullptr,
                                            void* FQ = nullptr, const float* FPs = nullptr,
-                                           const float* FQs = nullptr)
+                                           const float* FQs = nullptr, lchar* Pt = nullptr)
 {
     constexpr int E = K / G;
     constexpr int EB = PrecBytes<PREC>::value;
@@ -118,6 +121,17 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         if constexpr (OFF32) return Ql + ((uint32_t)v * (uint32_t)kRow + lane_off);
         else return Qb + (int64_t)v * kRow;
     };
+    const int row0 = T.row_begin;
+    auto prow_tile = [&](int u) -> lchar* { return Pt + ((uint32_t)(u - row0) * (uint32_t)kRow + lane_off); };
+    // Factors under 4 GB (OFF32): rows go through buffer resources, and a
+    // load or store that is skipped (a row kept in registers) is issued with
+    // an out-of-range offset instead of being branched around, so that the
+    // wait for a sample's rows counts the same loads and stores in flight on
+    // every path and stays partial.
+    constexpr bool BUF = OFF32 && !COH;
+    const auto Pr = factor_rsrc(FP);
+    const auto Qr = factor_rsrc(FQ);
+    auto boff = [&](int id) -> uint32_t { return (uint32_t)id * (uint32_t)kRow + lane_off; };
     const int64_t nnz = T.nnz;
 
     // The records of one window go from memory straight into LDS (global_load_lds,
@@ -150,14 +164,18 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     const int rbase = slot * G;
     // Request sample j's rows.  A row of the sample before it (prev_u,
     // prev_v: -1 at a window's start) is not loaded: that sample's written
-    // row is forwarded from the register ring instead (below).
+    // row is forwarded from the register ring instead (below).  P rows in
+    // LDS (PT) are read every sample; the ring covers one written since.
     auto request = [&](Pending<NW>& pd, int j, int prev_u, int prev_v) {
         const int at = (j / G) * 64 + rbase + j % G;
         pd.u = su[at];
         pd.v = sv[at];
         pd.r = sr[at];
-        if (pd.u != prev_u) pd.p = load_raw<TB, COH>(prow(pd.u));
-        if (pd.v != prev_v) pd.q = load_raw<TB, COH>(qrow(pd.v));
+        if constexpr (PT) pd.p = load_lds<TB>(prow_tile(pd.u));
+        else if constexpr (BUF) pd.p = load_buf<TB>(Pr, pd.u != prev_u ? boff(pd.u) : kBufSkip);
+        else if (pd.u != prev_u) pd.p = load_raw<TB, COH>(prow(pd.u));
+        if constexpr (BUF) pd.q = load_buf<TB>(Qr, pd.v != prev_v ? boff(pd.v) : kBufSkip);
+        else if (pd.v != prev_v) pd.q = load_raw<TB, COH>(qrow(pd.v));
         if constexpr (FP8) {
             pd.ps = Ps[pd.u];
             pd.qs = Qs[pd.v];
@@ -388,11 +406,16 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 }
                 // a row that the next sample of this worker (in pend[(d + 1) % D],
                 // requested already) rewrites is not stored: that sample takes
-                // it from the ring and stores its own result
+                // it from the ring and stores its own result (P rows in LDS
+                // are stored every sample)
                 const bool has_next = decltype(more)::value || d + 1 < D;
                 const Pending<NW>& nx = pend[(d + 1) % D];
-                if (!(has_next && nx.u == uc)) store_raw<TB, COH>(prow(uc), po);
-                if (!(has_next && nx.v == vc)) store_raw<TB, COH>(qrow(vc), qo);
+                const bool keep_p = has_next && nx.u == uc, keep_q = has_next && nx.v == vc;
+                if constexpr (PT) store_lds<TB>(prow_tile(uc), po);
+                else if constexpr (BUF) store_buf<TB>(Pr, keep_p ? kBufSkip : boff(uc), po);
+                else if (!keep_p) store_raw<TB, COH>(prow(uc), po);
+                if constexpr (BUF) store_buf<TB>(Qr, keep_q ? kBufSkip : boff(vc), qo);
+                else if (!keep_q) store_raw<TB, COH>(qrow(vc), qo);
                 wp[d] = po;
                 wq[d] = qo;
                 wu[d] = uc;
@@ -437,6 +460,96 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
     const SgdTask T = a.tasks[t];
     sgd_worker<K, G, PREC, COH, D, OFF32>(a, T, (uint32_t)(worker - T.worker_begin), slot, ls, s_u[wave], s_v[wave],
                                           s_r[wave], a.P, a.Q, a.ps, a.qs);
+}
+
+// Tile launch: one workgroup per task, the task a tile (the ratings of the
+// P rows [row_begin, row_begin + row_count) within one column range, no P
+// row shared with another task of the launch).  The workgroup copies its P
+// rows into LDS, its workers train the tile there (Q rows in memory, shared
+// through the XCD's L2 with the other tiles of the same column range), and
+// the rows go back to memory when every worker is done.  Per update, the L2
+// then sees the Q row (when the previous sample's differs) and the record,
+// instead of a P row read and written as well.
+// Dynamic LDS: the tile (row_count * K * bytes), then per wave the staged
+// records of a window (stage words per array: 64 * ceil(L / G)).
+template <int K, int G, int PREC, int D, bool OFF32, int NWAVES>
+__global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a, int stage_words)
+{
+    extern __shared__ __attribute__((aligned(16))) char lds_dyn[];
+    constexpr int S = 64 / G;
+    constexpr int64_t kRow = (int64_t)K * PrecBytes<PREC>::value;
+    static_assert(kRow % 16 == 0, "rows are copied in 16-byte words");
+    constexpr int kThreads = NWAVES * 64;
+
+    // workgroups go round-robin over the XCDs: task x * per + j is the j-th
+    // workgroup dispatched to XCD x
+    const int per = (int)gridDim.x / a.num_xcd;
+    const int ti = (int)(blockIdx.x % a.num_xcd) * per + (int)(blockIdx.x / a.num_xcd);
+    if (ti >= a.num_tasks) return;
+    const SgdTask T = a.tasks[ti];
+    if (T.nnz <= 0 || T.row_count <= 0) return;
+
+    lchar* tile = (lchar*)lds_dyn;
+    const int64_t bytes = (int64_t)T.row_count * kRow;
+    const gchar* src = to_global((const char*)T.P) + (int64_t)T.row_begin * kRow;
+    for (int64_t i = (int64_t)threadIdx.x * 16; i < bytes; i += kThreads * 16)
+        *(__attribute__((address_space(3))) u32x4*)(tile + i) = *(const __attribute__((address_space(1))) u32x4*)(src + i);
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / G;
+    const int ls = lane % G;
+    const int worker = wave * S + slot;
+    int32_t* stage = (int32_t*)(lds_dyn + bytes) + (int64_t)wave * 3 * stage_words;
+    if (worker < T.worker_count)
+        sgd_worker<K, G, PREC, false, D, OFF32, true>(a, T, (uint32_t)worker, slot, ls, stage, stage + stage_words,
+                                                      (float*)(stage + 2 * stage_words), a.P, a.Q, a.ps, a.qs, tile);
+    __syncthreads();
+
+    gchar* dst = to_global((char*)T.P) + (int64_t)T.row_begin * kRow;
+    for (int64_t i = (int64_t)threadIdx.x * 16; i < bytes; i += kThreads * 16)
+        *(__attribute__((address_space(1))) u32x4*)(dst + i) = *(const __attribute__((address_space(3))) u32x4*)(tile + i);
+}
+
+template <int K, int G, int PREC, int D, int NWAVES>
+inline hipError_t launch_tile_t(const SgdLaunch& a, int blocks, size_t lds, int stage_words, hipStream_t s)
+{
+    // dynamic LDS above the default limit must be allowed per kernel
+    if (a.P) {
+        hipError_t e = hipFuncSetAttribute((const void*)sgd_tile_kernel<K, G, PREC, D, true, NWAVES>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((sgd_tile_kernel<K, G, PREC, D, true, NWAVES>), dim3(blocks), dim3(NWAVES * 64), lds, s, a,
+                           stage_words);
+    } else {
+        hipError_t e = hipFuncSetAttribute((const void*)sgd_tile_kernel<K, G, PREC, D, false, NWAVES>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((sgd_tile_kernel<K, G, PREC, D, false, NWAVES>), dim3(blocks), dim3(NWAVES * 64), lds, s,
+                           a, stage_words);
+    }
+    return hipGetLastError();
+}
+
+// Tile launches use the default lane group (16 bytes of a row per lane, at
+// least 8 lanes), depth 2 and workgroups of 4, 8 or 16 waves.
+template <int K, int PREC>
+inline hipError_t launch_tile_k(const SgdLaunch& a, int blocks, int G, int D, int waves, size_t lds, int stage_words,
+                                hipStream_t s)
+{
+    constexpr int EB = PrecBytes<PREC>::value;
+    constexpr int G0 = (K * EB / 16) > 64 ? 64 : ((K * EB / 16) < 8 ? 8 : K * EB / 16);
+    if (G != G0 || D != 2) return hipErrorInvalidValue;
+    if constexpr (K % G0 == 0 && (K / G0) * EB <= 16 && (K / G0) * EB >= 2 && (PREC != kE4M3 || (K / G0) % 4 == 0)) {
+        switch (waves) {
+        case 4: return launch_tile_t<K, G0, PREC, 2, 4>(a, blocks, lds, stage_words, s);
+        case 8: return launch_tile_t<K, G0, PREC, 2, 8>(a, blocks, lds, stage_words, s);
+        case 16: return launch_tile_t<K, G0, PREC, 2, 16>(a, blocks, lds, stage_words, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 // A pointer every lane of the wave holds (tasks own whole waves in batched
@@ -589,5 +702,8 @@ template <int PREC>
 hipError_t launch_sgd_prec(const SgdLaunch& a, int blocks, int k, int G, bool coh, int D, hipStream_t s);
 template <int PREC>
 hipError_t launch_batched_for(const SgdLaunch& a, int blocks, int D, hipStream_t s);
+template <int PREC>
+hipError_t launch_tile_prec(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds,
+                            int stage_words, hipStream_t s);
 
 }  // namespace cumf

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CUMF_ABI_VERSION 2
+#define CUMF_ABI_VERSION 3
 
 enum cumf_status {
     CUMF_OK = 0,
@@ -54,7 +54,11 @@ enum cumf_schedule {
                                 the only L2 copy of every row */
     CUMF_SCHED_SERIAL = 1,   /* one worker, deterministic window order */
     CUMF_SCHED_GRID = 2,     /* blocks one after another, all workers each (u x v / x x y) */
-    CUMF_SCHED_STRATUM = 3   /* s x s blocks, a conflict-free diagonal per launch */
+    CUMF_SCHED_STRATUM = 3,  /* s x s blocks, a conflict-free diagonal per launch */
+    CUMF_SCHED_TILE = 4      /* the stratum diagonals of cumf_partition_tiles' blocks; each block's
+                                tiles (a few hundred consecutive rows of P) are trained by one
+                                workgroup each, the tile's P rows held in that workgroup's LDS for
+                                the launch */
 };
 
 /* One rating: mf_node, singleGPU/sgd.h:79-83 (12 bytes). */
@@ -80,9 +84,12 @@ typedef struct cumf_train_opts {
     float beta;
     float lrate;
     uint64_t seed;             /* Philox key for windows and rounding */
-    int32_t stratum_rounds;    /* stratum schedule: passes over the s diagonals per epoch, each running
-                                  1/rounds of every block's windows; 0 = 1 */
-    int32_t reserved;          /* 0 */
+    int32_t stratum_rounds;    /* stratum and tile schedules: passes over the s diagonals per epoch, each
+                                  running 1/rounds of every block's windows; 0 = 1 */
+    int32_t tiles_in_flight;   /* tile schedule: tiles one launch trains per XCD (a diagonal's other tiles
+                                  in later launches); 0 = all of a diagonal's tiles in one launch.  For
+                                  the tile schedule num_workers counts the workers of one tile, at most 16
+                                  waves' worth (0 = 8 waves'); the tile's workgroup has 4, 8 or 16 waves */
 } cumf_train_opts;
 
 typedef struct cumf_ctx cumf_ctx;
@@ -167,6 +174,27 @@ int cumf_partition_strata(cumf_ctx* ctx, int32_t s, int64_t* offsets);
  * streamed ratings. */
 enum cumf_order { CUMF_ORDER_ROWS = 1, CUMF_ORDER_COLUMNS = 2 };
 int cumf_order_blocks(cumf_ctx* ctx, int32_t by);
+/* Partition for CUMF_SCHED_TILE: the s x s blocks of cumf_partition_strata,
+ * each block's records then grouped by tile and, inside a tile, ordered by
+ * column (stable).  A tile is a range of consecutive rows of one row band
+ * (at most tile_rows rows, cut where the band's ratings would otherwise be
+ * unevenly spread over its tiles); tile_rows = 0 takes cumf_tile_rows(k,
+ * precision) for the factors the context holds, or FP16 at k = 128 before
+ * they are initialised.  No counterpart in the reference (its blocks are
+ * trained by all workers from memory, singleGPU/sgd_kernel.cu:249-372).  A
+ * tile's P rows fit one workgroup's LDS, so the tile trains without sending
+ * its P rows through the L2.  Up to 2^32 - 1 ratings; not for streamed
+ * ratings.  offsets (s*s + 1 entries, may be NULL) receives the block
+ * bounds; cumf_get_tiles returns the tiles. */
+int cumf_partition_tiles(cumf_ctx* ctx, int32_t s, int32_t tile_rows, int64_t* offsets);
+/* Rows per tile for factors of length k in a format: the most that fit the
+ * default 64 KB tile (two tile workgroups per CU). */
+int32_t cumf_tile_rows(int32_t k, int32_t precision);
+/* The tiles of cumf_partition_tiles, block by block (block-major, then tile
+ * order within the block): *count receives their number; when the arrays
+ * are non-NULL, tile t holds records [rec_off[t], rec_off[t+1]) (count + 1
+ * entries) and rows [row_begin[t], row_begin[t] + row_count[t]). */
+int cumf_get_tiles(cumf_ctx* ctx, int64_t* count, int64_t* rec_off, int32_t* row_begin, int32_t* row_count);
 /* Declare that the uploaded ratings are already grouped into ux x vy blocks
  * with the given prefix offsets (as the host cycle-swap leaves them). */
 int cumf_set_blocks(cumf_ctx* ctx, int32_t ux, int32_t vy, const int64_t* offsets);

@@ -1,0 +1,220 @@
+"""The tile schedule (CUMF_SCHED_TILE) through the C ABI: the tile partition
+against a numpy restatement, tile epochs bit for bit against the CPU oracle
+wherever they are deterministic, graph replay, refusals and convergence.
+
+A tile is a range of consecutive rows of one row band of the s x s strata;
+its records (within one block) are trained by one workgroup that holds the
+tile's P rows in LDS for the launch.  With one worker per tile and one tile
+per XCD in flight (tiles_in_flight = 1) no two concurrent workers share a
+row, so an epoch is deterministic: the diagonals in order, and within a
+block its tiles largest first, each tile's windows drawn with the tile's
+index in the tile list as tag.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(seed, m, n, nnz, n_test=0):
+    from cumf_sgd_amd import datagen
+
+    R = datagen.uniform(seed, m, n, nnz, n_test)
+    return R.numpy() + (m, n)
+
+
+def _ctx(u, v, r, m, n):
+    import cumf_sgd_amd as cs
+
+    c = cs.Context(0)
+    c.set_ratings(u, v, r, m, n)
+    return c
+
+
+def _row_key(row0, rows, m):
+    """Tile of every row: the first row of the tile interval holding it."""
+    key = np.full(m, -1, np.int64)
+    for a, b in zip(row0, rows):
+        key[a:a + b] = a
+    return key
+
+
+@pytest.mark.parametrize("s,tile_rows", [(8, 16), (8, 0), (4, 7), (16, 64)])
+def test_tile_partition_is_strata_then_stable_tile_column_sort(gpu, s, tile_rows):
+    """The tile partition has the blocks of cumf_partition_strata; inside a
+    block the records are the stratum block's, stably sorted by (tile of
+    the row, column).  Tiles cover each block exactly, hold at most
+    tile_rows rows of one row band, and every record's row lies in its
+    tile."""
+    u, v, r, _, _, _, m, n = _problem(seed=21, m=900, n=300, nnz=40_000)
+    with _ctx(u, v, r, m, n) as c:
+        off_s = c.partition_strata(s)
+        su, sv, sr = c.get_ratings()
+    with _ctx(u, v, r, m, n) as c:
+        off = c.partition_tiles(s, tile_rows)
+        gu, gv, gr = c.get_ratings()
+        toff, row0, rows = c.get_tiles()
+    assert np.array_equal(off, off_s)
+    assert toff[0] == 0 and toff[-1] == len(u) and np.all(np.diff(toff) > 0)
+    cap = tile_rows or 256   # cumf_tile_rows(128, FP16): 64 KB / 256 B
+    assert rows.max() <= cap and rows.min() >= 1
+    # every block is a whole number of tiles
+    assert set(off.tolist()) <= set(toff.tolist())
+    key = _row_key(row0, rows, m)
+    tile_of_rec = np.repeat(np.arange(len(row0)), np.diff(toff))
+    assert np.all(gu >= row0[tile_of_rec]) and np.all(gu < row0[tile_of_rec] + rows[tile_of_rec])
+    # row ranges of distinct tiles never overlap
+    iv = sorted(set(zip(row0.tolist(), rows.tolist())))
+    assert all(a0 + a1 <= b0 for (a0, a1), (b0, _) in zip(iv, iv[1:]))
+    # expected order: each stratum block stably sorted by (tile, column)
+    blk = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    order = np.lexsort((sv, key[su], blk))
+    assert np.array_equal(gu, su[order]) and np.array_equal(gv, sv[order]) and np.array_equal(gr, sr[order])
+
+
+def _oracle_tiles(gu, gv, gr, off, toff, s, P, Q, ps, qs, epochs, rounds, prec, k, G, L, lp, lq, seed):
+    """The deterministic tile epoch restated: passes, diagonals, each block's
+    tiles largest first (stable), a tile's windows of the pass in order."""
+    import oracle
+
+    nt = len(toff) - 1
+    lr = oracle.lr_table(0.08, 0.3, 0.0, epochs + 4)
+    tile_blk = np.searchsorted(off, toff[:-1], side="right") - 1
+    for ep in range(epochs):
+        nxt = np.zeros(nt, np.int64)
+        for rr in range(rounds):
+            for d in range(s):
+                for i in range(s):
+                    b = i * s + (i + d) % s
+                    ts = np.flatnonzero(tile_blk == b)
+                    sizes = toff[ts + 1] - toff[ts]
+                    for t in ts[np.argsort(-sizes, kind="stable")]:
+                        lo, hi = int(toff[t]), int(toff[t + 1])
+                        n_all = (hi - lo + L - 1) // L   # one worker: one window per L records
+                        n_pass = n_all * (rr + 1) // rounds - n_all * rr // rounds
+                        if n_pass == 0:
+                            continue
+                        oracle.sgd_block_windows(gu[lo:hi], gv[lo:hi], gr[lo:hi], P, Q, lr, ep, int(t), int(nxt[t]),
+                                                 int(n_pass), k=k, G=G, prec=prec, lambda_p=lp, lambda_q=lq,
+                                                 seed=seed, batch_len=L, ps=ps, qs=qs)
+                        nxt[t] += n_pass
+        if prec == 3:
+            P, ps = oracle.fp8_rescale(P, ps)
+            Q, qs = oracle.fp8_rescale(Q, qs)
+    return P, Q, ps, qs
+
+
+@pytest.mark.parametrize("prec,tile_rows,rounds,k", [
+    (0, 0, 1, 64), (1, 0, 1, 64), (1, 8, 1, 64), (1, 8, 2, 64), (0, 8, 1, 128), (3, 8, 1, 64), (4, 8, 2, 64),
+    (2, 8, 1, 32), (1, 5, 1, 128)])
+def test_tile_epochs_match_oracle_when_deterministic(gpu, prec, tile_rows, rounds, k):
+    """One worker per tile and one tile per XCD in flight: GPU factors after
+    three epochs equal the oracle's bit for bit (P rows trained in LDS, Q
+    rows in memory), with one tile per block (tile_rows 0: the default,
+    larger than a band) and with several tiles per block, one and two passes
+    over the diagonals, every storage format."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, _, _, _, m, n = _problem(seed=22, m=320, n=200, nnz=12_000)
+    s, seed, lp, lq, L = 8, 5, 0.05, 0.04, 32
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(k, prec, seed=seed)
+        off = c.partition_tiles(s, tile_rows)
+        gu, gv, gr = c.get_ratings()
+        toff, row0, rows = c.get_tiles()
+        if tile_rows == 0:
+            assert len(toff) - 1 <= s * s   # one tile per non-empty block
+        P, ps = c.get_factors(0)
+        Q, qs = c.get_factors(1)
+        c.train(cs.make_opts(k=k, schedule=cs.TILE, num_workers=1, tiles_in_flight=1, seed=seed, lambda_p=lp,
+                             lambda_q=lq, stratum_rounds=rounds, batch_len=L), 0, 3)
+        G = c.last_launch().lanes_per_sample
+        P1, ps1 = c.get_factors(0)
+        Q1, qs1 = c.get_factors(1)
+    P, Q = np.ascontiguousarray(P).copy(), np.ascontiguousarray(Q).copy()
+    P, Q, ps, qs = _oracle_tiles(gu, gv, gr, off, toff, s, P, Q, ps, qs, 3, rounds, prec, k, G, L, lp, lq, seed)
+    assert np.array_equal(P1.view(np.uint8), P.view(np.uint8))
+    assert np.array_equal(Q1.view(np.uint8), Q.view(np.uint8))
+    if prec == 3:
+        assert np.array_equal(ps1, ps) and np.array_equal(qs1, qs)
+
+
+@pytest.mark.parametrize("prec", [1, 3])
+def test_tile_graph_replay_equals_eager(gpu, prec):
+    """Three tile epochs replayed from a captured graph equal the eager
+    launches bit for bit (deterministic configuration, eager work between
+    replays)."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, _, _, _, m, n = _problem(seed=23, m=400, n=160, nnz=10_000)
+    out = []
+    for graph in (0, 1):
+        with _ctx(u, v, r, m, n) as c:
+            c.partition_tiles(8, 12)
+            c.init_factors(64, prec, seed=2)
+            opts = cs.make_opts(k=64, schedule=cs.TILE, num_workers=1, tiles_in_flight=1, seed=2, batch_len=32,
+                                use_graph=graph)
+            for ep in range(3):
+                c.train(opts, ep, ep + 1)
+                c.get_factors(0)   # eager work between replays
+            out.append((c.get_factors(0), c.get_factors(1)))
+    (pe, qe), (pg, qg) = out
+    assert np.array_equal(pe[0].view(np.uint8), pg[0].view(np.uint8))
+    assert np.array_equal(qe[0].view(np.uint8), qg[0].view(np.uint8))
+
+
+def test_tile_full_workgroups_train_every_rating_window(gpu):
+    """The default tile launch (a workgroup of workers per tile, all tiles of
+    a diagonal at once) trains the planned updates, leaves the factors
+    finite and lowers the training error of a small problem."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, tu, tv, tr, m, n = _problem(seed=24, m=3000, n=800, nnz=200_000, n_test=20_000)
+    with _ctx(u, v, r, m, n) as c:
+        c.set_test(tu, tv, tr)
+        c.partition_tiles(8)
+        c.init_factors(128, cs.FP16, seed=3)
+        r0 = c.test_rmse(1.0, 0.0)
+        opts = cs.make_opts(k=128, schedule=cs.TILE, seed=3, batch_len=32)
+        c.train(opts, 0, 5)
+        info = c.last_launch()
+        toff, _, _ = c.get_tiles()
+        L, W = info.batch_len, info.num_workers
+        sizes = np.diff(toff)
+        uc = (sizes + W * L - 1) // (W * L)     # fewest windows per worker
+        w = (sizes + uc * L - 1) // (uc * L)    # then fewest workers keeping them
+        assert W == 32 and info.updates_per_epoch == int(np.sum(w * uc * L))
+        r1 = c.test_rmse(1.0, 0.0)
+        P, _ = c.get_factors(0)
+        assert np.all(np.isfinite(P.view(np.float16).astype(np.float32)))
+        assert r1 < r0
+
+
+def test_tile_refusals(gpu):
+    """The tile schedule needs a tile partition, the default lane group,
+    depth 2, no coherent mode, at most 16 waves of workers per tile and tiles
+    that fit in LDS; a later partition drops the tiles."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, _, _, _, m, n = _problem(seed=25, m=500, n=200, nnz=5_000)
+    with _ctx(u, v, r, m, n) as c:
+        c.init_factors(128, cs.FP16, seed=1)
+        base = dict(k=128, schedule=cs.TILE, seed=1, batch_len=32)
+        with pytest.raises(cs.CumfError) as e:
+            c.train(cs.make_opts(**base), 0, 1)
+        assert e.value.code == 3
+        c.partition_tiles(8)
+        for bad in (dict(lanes_per_sample=64), dict(coherent=1), dict(pipeline_depth=4), dict(num_workers=65),
+                    dict(tiles_in_flight=-1)):
+            with pytest.raises(cs.CumfError) as e:
+                c.train(cs.make_opts(**{**base, **bad}), 0, 1)
+            assert e.value.code == 1, bad
+        c.train(cs.make_opts(**base), 0, 1)   # still usable after the refusals
+        c.partition_tiles(8, 700)   # 700 rows of 256 B exceed the LDS with the staged records
+        with pytest.raises(cs.CumfError) as e:
+            c.train(cs.make_opts(**base), 0, 1)
+        assert e.value.code == 1
+        c.partition_strata(8)
+        with pytest.raises(cs.CumfError) as e:
+            c.train(cs.make_opts(**base), 0, 1)
+        assert e.value.code == 3

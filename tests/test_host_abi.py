@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     exported = set(re.findall(r" T (cumf_[a-z0-9_]+)$", out, flags=re.M))
     missing = [n for n in names if n not in exported]
     assert not missing, missing
-    assert cs.lib.cumf_abi_version() == 2
+    assert cs.lib.cumf_abi_version() == 3
 
 
 def test_library_has_gfx950_code():
