@@ -229,10 +229,12 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "bf16sr", "fp8"])
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum"],
+    ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum", "tile"],
                     help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
-                         "(default); hogwild: the reference's 1x1 mode")
-    ap.add_argument("--strata", type=int, default=24, help="s for the stratum schedule (s x s blocks)")
+                         "(default); tile: the blocks of 8 x 8 strata split into row tiles, each trained by one "
+                         "workgroup with its P rows in LDS; hogwild: the reference's 1x1 mode")
+    ap.add_argument("--strata", type=int, default=0,
+                    help="s for the stratum and tile schedules (s x s blocks; 0 = 24 for strata, 8 for tiles)")
     ap.add_argument("--order", default="columns", choices=["columns", "rows", "input"],
                     help="record order inside each stratum block (cumf_order_blocks): by column (default, the "
                          "library's default for this workload), by row, or as generated")
@@ -273,7 +275,9 @@ def main() -> int:
         return 2
     prec = cs.PRECISIONS[args.precision]
     sched = cs.SCHEDULES[args.schedule]
-    sched_tag = f"stratum{args.strata}" if sched == cs.STRATUM else "hogwild"
+    if not args.strata:
+        args.strata = 8 if sched == cs.TILE else 24
+    sched_tag = {cs.STRATUM: f"stratum{args.strata}", cs.TILE: f"tile{args.strata}"}.get(sched, "hogwild")
     t0 = time.perf_counter()
     # generated on the host (deterministic across machines), then made resident in HBM
     torch.set_num_threads(min(16, os.cpu_count() or 1))
@@ -295,6 +299,9 @@ def main() -> int:
         ctx.partition_strata(args.strata)
         if args.order != "input":
             ctx.order_blocks(cs.ORDER_COLUMNS if args.order == "columns" else cs.ORDER_ROWS)
+    elif sched == cs.TILE:
+        ctx.init_factors(K, prec, args.seed)   # the tile rows follow the factor format
+        ctx.partition_tiles(args.strata)
     torch.cuda.synchronize()
     log(f"preprocessed: scale={scale:.5f} b={b:.5f}")
 
@@ -308,7 +315,6 @@ def main() -> int:
         opts, tuned_ms = ctx.autotune(opts)
         log(f"autotuned in {time.perf_counter() - t1:.1f}s: lanes={opts.lanes_per_sample} depth={opts.pipeline_depth} "
             f"batch_len={opts.batch_len} workers={opts.num_workers} ({tuned_ms:.2f} ms/epoch)")
-    tuned = (opts.num_workers, opts.lanes_per_sample, opts.pipeline_depth, opts.batch_len)
     # the epochs of each phase in one call, as the reference's training loop
     # runs all its iterations inside sgd_update_k128 (singleGPU/sgd_kernel.cu:212-248)
     if args.warmup:
@@ -331,27 +337,31 @@ def main() -> int:
         f"per launch ({kt.launches} launches), {achieved:.0f} GB/s algorithmic; workers={info.num_workers} "
         f"lanes={info.lanes_per_sample}")
 
-    # ---- time to target RMSE (evaluation excluded from the time)
+    # ---- time to target RMSE (evaluation excluded from the time): every
+    # format trains the library's default schedule for this problem and that
+    # format (cumf_default_schedule: FP32 row tiles in two passes, the 16-bit
+    # and FP8 formats 24 x 24 strata with item-ordered blocks)
     ttt = None
     if not args.no_rmse:
         ttt = {}
         curves = {}
         shapes = {}
         for p in (0, 1, 2, 4, 3):
-            # the headline format trains with the measured shape; each other
-            # format with its own autotuned shape (with --autotune), else with
-            # the library's defaults for that format and the fixed window length
-            w, l, d, bl = tuned if p == prec else (args.workers, args.lanes, args.depth, args.batch_len)
-            if p != prec and args.autotune:
-                ctx.init_factors(K, p, args.seed)
-                o_p, _ = ctx.autotune(cs.make_opts(k=K, schedule=sched, num_workers=w, lambda_p=float(lam),
+            ctx.init_factors(K, p, args.seed)
+            sched_p, rounds_p = ctx.prepare_default(K, p)
+            w, l, d, bl = (args.workers, args.lanes, args.depth, args.batch_len)
+            if args.autotune:
+                o_p, _ = ctx.autotune(cs.make_opts(k=K, schedule=sched_p, num_workers=w, lambda_p=float(lam),
                                                    seed=args.seed, lanes_per_sample=l, pipeline_depth=d,
-                                                   stratum_rounds=args.rounds, batch_len=bl,
+                                                   stratum_rounds=rounds_p, batch_len=bl,
                                                    use_graph=int(not args.no_graph)))
                 w, l, d, bl = o_p.num_workers, o_p.lanes_per_sample, o_p.pipeline_depth, o_p.batch_len
-            shapes[p] = {"lanes_per_sample": l, "pipeline_depth": d, "batch_len": bl or 128}   # 0 = default
-            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched, d, bl,
-                                      rounds=args.rounds, graph=int(not args.no_graph))
+            times, rmses = run_epochs(ctx, p, args.rmse_epochs, scale, b, lam, args.seed, w, l, sched_p, d, bl,
+                                      rounds=rounds_p, graph=int(not args.no_graph))
+            info_p = ctx.last_launch()
+            shapes[p] = {"schedule": {cs.STRATUM: "stratum24", cs.TILE: "tile8", cs.HOGWILD: "hogwild"}[sched_p],
+                         "passes": rounds_p, "lanes_per_sample": info_p.lanes_per_sample,
+                         "pipeline_depth": info_p.pipeline_depth, "batch_len": info_p.batch_len}
             curves[p] = (times, rmses)
             log(f"{PREC_NAME[p]}: rmse by epoch {[round(x, 4) for x in rmses]}")
         target = curves[0][1][-1] + 0.005
@@ -418,7 +428,8 @@ def main() -> int:
                    "factor_storage": {0: "f32", 1: "f16", 2: "bf16", 3: "fp8e4m3 + f32 row scale",
                                       4: "bf16 (stochastic rounding)"}[prec],
                    "schedule": sched_tag, "graph_replay": not args.no_graph,
-                   "block_order": args.order if sched == cs.STRATUM else "input",
+                   "block_order": {cs.STRATUM: args.order, cs.TILE: "row tiles, columns within a tile"}.get(
+                       sched, "input"),
                    "shape_source": "cumf_autotune" if args.autotune else
                    "fixed: library defaults with the given window length (cumf_autotune's pick on this workload)",
                    "workers": info.num_workers,

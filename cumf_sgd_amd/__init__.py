@@ -425,6 +425,21 @@ class Context:
         _check(lib.cumf_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(up)), "cumf_kernel_time")
         return KernelTime(ms.value, la.value, up.value)
 
+    def prepare_default(self, k: int, precision: int, strata: int = 24) -> tuple[int, int]:
+        """cumf_default_schedule, then the partition and record order it asks
+        for (strata x strata blocks for STRATUM, the tile partition for
+        TILE); returns (schedule, stratum_rounds) for the training options."""
+        sched, rounds, order = self.default_schedule(k, precision)
+        if sched == STRATUM:
+            self.partition_strata(strata)
+        elif sched == TILE:
+            self.partition_tiles(8)
+        elif order:
+            self.partition(1, 1)
+        if order:
+            self.order_blocks(order)
+        return sched, rounds
+
     def default_schedule(self, k: int, precision: int) -> tuple[int, int, int]:
         """cumf_default_schedule: (schedule, stratum_rounds, order) that a
         1x1 grid trains by default for these ratings (order 0: the records

@@ -83,6 +83,14 @@ int prec_bytes(int prec) { return prec == CUMF_FP32 ? 4 : (prec == CUMF_FP8_E4M3
 
 }  // namespace
 
+// One launch of the update kernel: a slice of the uploaded task table.
+struct LaunchRec {
+    int64_t task_off;
+    int32_t task_cnt;
+    int32_t num_xcd;
+    int32_t blocks;   // workgroups of the launch; 0 = the shape's
+};
+
 struct cumf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -110,6 +118,16 @@ struct cumf_ctx {
     std::vector<int64_t> tile_rec;
     std::vector<int32_t> tile_row0, tile_rows, block_tile;
     int32_t tile_rows_max = 0;
+    // bumped whenever the ratings, their order, the blocks or the tiles change
+    uint64_t data_gen = 0;
+    // the task table on the device (tasks) was planned under plan_key: its
+    // host copy, one epoch's launches and updates, and the streamed offsets
+    bool plan_valid = false;
+    uint64_t plan_key = 0;
+    std::vector<SgdTask> plan_tasks;
+    std::vector<LaunchRec> plan_launches;
+    int64_t plan_upd = 0;
+    std::vector<int64_t> plan_soff;
 
     int32_t k = 0, prec = -1;
     void* P = nullptr; size_t P_cap = 0;
@@ -239,6 +257,7 @@ void drop_streamed(cumf_ctx* c)
 // of either drops it.
 void drop_tiles(cumf_ctx* c)
 {
+    ++c->data_gen;
     c->tile_rec.clear();
     c->tile_row0.clear();
     c->tile_rows.clear();
@@ -720,7 +739,11 @@ int cumf_order_blocks(cumf_ctx* c, int32_t by)
 int32_t cumf_tile_rows(int32_t k, int32_t precision)
 {
     if (k < 1 || precision < CUMF_FP32 || precision > CUMF_BF16_SR) return 0;
-    return (int32_t)(kSgdTileDefaultBytes / ((int64_t)k * prec_bytes(precision)));
+    // 128 rows at k = 128 (more rows per tile at smaller k), at most 64 KB:
+    // on the 99M workload 128-row tiles trained fastest for FP32 (64 KB) and
+    // FP16 (32 KB) alike; larger ones leave fewer workgroups per CU
+    const int64_t rows = 128 * 128 / k, cap = kSgdTileDefaultBytes / ((int64_t)k * prec_bytes(precision));
+    return (int32_t)std::max<int64_t>(1, std::min(rows, cap));
 }
 
 int cumf_partition_tiles(cumf_ctx* c, int32_t s, int32_t tile_rows, int64_t* offsets)
@@ -729,6 +752,7 @@ int cumf_partition_tiles(cumf_ctx* c, int32_t s, int32_t tile_rows, int64_t* off
     if (rc) return rc;
     if (c->streamed) return fail(CUMF_ERR_STATE, "streamed ratings stay in host memory in their stratum blocks");
     if (!c->have_ratings) return fail(CUMF_ERR_STATE, "no ratings");
+    if (s == 0) s = kTileStrata;
     if (s < 1 || s > 64) return fail(CUMF_ERR_ARG, "strata %d out of range 1..64", s);
     if (tile_rows < 0) return fail(CUMF_ERR_ARG, "tile_rows=%d must be >= 0", tile_rows);
     if (c->nnz > (int64_t)UINT32_MAX)
@@ -827,6 +851,7 @@ int cumf_partition_tiles(cumf_ctx* c, int32_t s, int32_t tile_rows, int64_t* off
         c->tile_rec.assign(1, 0);
     }
     c->tile_rows_max = tile_rows;
+    ++c->data_gen;
     return CUMF_OK;
 }
 
@@ -882,16 +907,27 @@ int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* sc
     *schedule = both_skewed ? CUMF_SCHED_HOGWILD : CUMF_SCHED_STRATUM;
     // records grouped by column, so that consecutive samples of a worker
     // share the column's row (kept in registers, not stored and reloaded):
-    // always for the one-pass strata; for Hogwild on doubly skewed data only
-    // with 16-bit rows, whose test RMSE the grouping leaves within a third of
-    // a 0.005-standard-deviation bar there, where it moves FP32's close to it
-    const bool sixteen = precision == CUMF_FP16 || precision == CUMF_BF16 || precision == CUMF_BF16_SR;
-    if (c->nnz <= (int64_t)UINT32_MAX && (*schedule == CUMF_SCHED_STRATUM || sixteen)) *order = CUMF_ORDER_COLUMNS;
+    // always for the one-pass strata; for Hogwild on doubly skewed data with
+    // round-to-nearest rows (FP32, FP16, BF16: within half of a
+    // 0.005-standard-deviation bar there, at twice the speed), not with
+    // stochastically rounded ones, whose write-back noise on the hot rows
+    // takes the grouped run to the bar
+    const bool stochastic = precision == CUMF_BF16_SR || precision == CUMF_FP8_E4M3;
+    if (c->nnz <= (int64_t)UINT32_MAX && (*schedule == CUMF_SCHED_STRATUM || !stochastic)) *order = CUMF_ORDER_COLUMNS;
     // strata over a P too large for the caches (its rows then come from HBM
     // at random): records grouped by row instead, so that P's rows are the
     // ones kept in registers
     if (*order && *schedule == CUMF_SCHED_STRATUM && (int64_t)c->m * k * prec_bytes(precision) > kHbmResidentP)
         *order = CUMF_ORDER_ROWS;
+    // FP32 rows make the strata L2-bound (a 512-byte P row read and written
+    // per update); a tile keeps its P rows in LDS instead, with two passes
+    // over the diagonals per epoch (one pass ends at the 0.005 bar on the
+    // Netflix shape, two 0.002 above the serialized oracle)
+    if (*schedule == CUMF_SCHED_STRATUM && *order == CUMF_ORDER_COLUMNS && precision == CUMF_FP32) {
+        *schedule = CUMF_SCHED_TILE;
+        *stratum_rounds = kTileFp32Rounds;
+        *order = 0;
+    }
     return CUMF_OK;
 }
 
@@ -1209,7 +1245,9 @@ int cumf::default_workers(const ShapeInput& in, int S, int L, int waves_per_cu)
 
 int cumf::resolve_shape(const ShapeInput& in, const cumf_train_opts* o, Shape* sh, int xcd_multiple)
 {
-    sh->G = o->lanes_per_sample ? o->lanes_per_sample : sgd_default_group(in.k, in.prec);
+    sh->G = o->lanes_per_sample ? o->lanes_per_sample
+                                : (o->schedule == CUMF_SCHED_TILE ? tile_default_group(in.k, in.prec)
+                                                                  : sgd_default_group(in.k, in.prec));
     if (!sgd_shape_supported(in.k, in.prec, sh->G))
         return fail(CUMF_ERR_ARG, "lanes_per_sample=%d unsupported for k=%d precision=%d", sh->G, in.k, in.prec);
     if (o->coherent && (in.k / sh->G) * prec_bytes(in.prec) < 4)
@@ -1241,6 +1279,17 @@ int cumf::resolve_shape(const ShapeInput& in, const cumf_train_opts* o, Shape* s
     sh->W = W;
     sh->blocks = blocks;
     return CUMF_OK;
+}
+
+int cumf::tile_default_group(int k, int prec)
+{
+    // 32 bytes of a row per lane where that is a supported shape: a tile
+    // launch is bound by its VALU issue, and the per-sample work outside the
+    // arithmetic (records, compares, addresses) then serves twice the
+    // elements (99M workload: FP32 16.1e9 -> 19.0e9 updates/s, FP16
+    // 20.8e9 -> 21.4e9)
+    const int g0 = sgd_default_group(k, prec);
+    return sgd_shape_supported(k, prec, g0 / 2) ? g0 / 2 : g0;
 }
 
 bool cumf::confine_to_one_xcd(int schedule, int coherent, int64_t m, int64_t n, int k, int prec)
@@ -1310,14 +1359,6 @@ int tile_waves(int workers, int S)
     const int w = (workers + S - 1) / S;
     return w <= 4 ? 4 : (w <= 8 ? 8 : 16);
 }
-
-// One launch of the update kernel: a slice of the uploaded task table.
-struct LaunchRec {
-    int64_t task_off;
-    int32_t task_cnt;
-    int32_t num_xcd;
-    int32_t blocks;   // workgroups of the launch; 0 = the shape's
-};
 
 // The launches of one epoch, appended to `tasks`.
 int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_xcd, int ep, int e0,
@@ -1691,17 +1732,18 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     if ((rc = resolve_shape(shape_input(c, o), o, &sh, num_xcd))) return rc;
     if (stratum) sh.W = sh.blocks * kSgdWavesPerBlock * sh.S;   // every wave of every XCD works
     if (tiled) {
-        // one workgroup per tile: the default lane group, no coherent mode,
-        // the tile and the staged records in LDS
-        if (sh.G != sgd_default_group(c->k, c->prec))
-            return fail(CUMF_ERR_ARG, "tile schedule runs lanes_per_sample=%d only (k=%d precision=%d)",
-                        sgd_default_group(c->k, c->prec), c->k, c->prec);
+        // one workgroup per tile: 16 or 32 bytes of a row per lane, no
+        // coherent mode, the tile and the staged records in LDS
+        const int g0 = sgd_default_group(c->k, c->prec);
+        if (sh.G != g0 && !(sh.G == g0 / 2 && sgd_shape_supported(c->k, c->prec, g0 / 2)))
+            return fail(CUMF_ERR_ARG, "tile schedule runs lanes_per_sample=%d or %d (k=%d precision=%d)", g0, g0 / 2,
+                        c->k, c->prec);
         if (o->coherent) return fail(CUMF_ERR_ARG, "tile schedule has no coherent mode");
         if (sh.D != 2) return fail(CUMF_ERR_ARG, "tile schedule runs pipeline_depth 2");
         const int most = kSgdTileMaxWaves * sh.S;
         if (o->num_workers > most)
             return fail(CUMF_ERR_ARG, "tile schedule: num_workers=%d per tile exceeds %d", o->num_workers, most);
-        sh.W = o->num_workers > 0 ? o->num_workers : kSgdTileWaves * sh.S;   // workers per tile
+        sh.W = o->num_workers > 0 ? o->num_workers : std::min(kTileWorkers, most);   // workers per tile
         const size_t lds = sgd_tile_lds_bytes(c->k, c->prec, sh.G, sh.L, c->tile_rows_max, tile_waves(sh.W, sh.S));
         if (lds > sgd_tile_lds_limit())
             return fail(CUMF_ERR_ARG, "tiles of %d rows at k=%d precision=%d with %d-record windows need %zu bytes of LDS "
@@ -1727,47 +1769,87 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
             if (block_order[i] < 0 || block_order[i] >= nb) return fail(CUMF_ERR_ARG, "block_order entry out of range");
     }
 
-    // plan every epoch's launches and upload the task table once
-    std::vector<SgdTask> tasks;
-    std::vector<std::vector<LaunchRec>> plan;
-    const int plans = per_epoch_order ? e1 - e0 : 1;
-    int64_t upd = 0;
-    for (int i = 0; i < plans; ++i) {
-        plan.emplace_back();
-        if ((rc = plan_epoch(c, o, sh, num_xcd, e0 + i, e0, block_order, order_len, tasks, plan.back(), &upd))) return rc;
+    // Plan every epoch's launches and upload the task table once.  A call
+    // whose plan key (the ratings' generation and addresses, the factor
+    // storage, the learning-rate table's address, the options and the
+    // resolved shape) equals the last upload's reuses that table as it is:
+    // an epoch loop driven one epoch per call neither re-plans nor uploads.
+    const bool cacheable = !per_epoch_order;
+    uint64_t pkey = 1469598103934665603ull;
+    {
+        const uint64_t gen[2] = {c->data_gen, (uint64_t)c->nnz};
+        pkey = fnv1a(gen, sizeof(gen), pkey);
+        const int32_t f[20] = {o->schedule, o->num_workers, o->batch_len, o->lanes_per_sample, o->coherent,
+                               o->pipeline_depth, o->stratum_rounds, o->tiles_in_flight, sh.G, sh.S, sh.L, sh.W,
+                               sh.blocks, sh.D, num_xcd, c->k, c->prec, c->m, c->n, c->tile_rows_max};
+        pkey = fnv1a(f, sizeof(f), pkey);
+        const float lam[2] = {o->lambda_p, o->lambda_q};
+        pkey = fnv1a(lam, sizeof(lam), pkey);
+        const void* addr[14] = {c->u,     c->v,     c->r,     c->P,     c->Q,     c->ps,    c->qs,
+                                c->lr,    c->su[0], c->su[1], c->sv[0], c->sv[1], c->sr[0], c->sr[1]};
+        pkey = fnv1a(addr, sizeof(addr), pkey);
     }
+    std::vector<SgdTask> fresh;
+    std::vector<std::vector<LaunchRec>> plan;
+    std::vector<int64_t> soff;
+    int64_t upd = 0;
+    const bool reuse = cacheable && c->plan_valid && c->plan_key == pkey;
+    if (reuse) {
+        plan.push_back(c->plan_launches);
+        upd = c->plan_upd;
+        soff = c->plan_soff;
+    } else {
+        c->plan_valid = false;
+        const int plans = per_epoch_order ? e1 - e0 : 1;
+        for (int i = 0; i < plans; ++i) {
+            plan.emplace_back();
+            if ((rc = plan_epoch(c, o, sh, num_xcd, e0 + i, e0, block_order, order_len, fresh, plan.back(), &upd)))
+                return rc;
+        }
+        if (c->streamed && !fresh.empty()) {
+            if (!stratum) return fail(CUMF_ERR_STATE, "streamed ratings train with the stratum schedule");
+            if (o->stratum_rounds > 1)
+                return fail(CUMF_ERR_ARG, "streamed ratings make one pass over the diagonals per epoch (stratum_rounds 1)");
+            // point every task at its block's place in its diagonal's staging buffer
+            const int s = c->ux;
+            soff.assign((size_t)s * s, 0);
+            for (int d = 0; d < s; ++d) {
+                int64_t at = 0;
+                for (int i = 0; i < s; ++i) {
+                    const int blk = i * s + (i + d) % s;
+                    soff[(size_t)blk] = at;
+                    at += c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk];
+                }
+            }
+            for (SgdTask& t : fresh) {
+                const int blk = (int)t.tag;
+                const int b = ((blk % s - blk / s + s) % s) & 1;
+                t.u = c->su[b] + soff[(size_t)blk];
+                t.v = c->sv[b] + soff[(size_t)blk];
+                t.r = c->sr[b] + soff[(size_t)blk];
+            }
+        }
+        if (!fresh.empty()) {
+            if ((rc = ensure(&c->tasks, &c->tasks_cap, fresh.size()))) return rc;
+            HIP_TRY(hipMemcpyAsync(c->tasks, fresh.data(), sizeof(SgdTask) * fresh.size(), hipMemcpyHostToDevice,
+                                   c->stream));
+        }
+        if (cacheable) {
+            c->plan_tasks = std::move(fresh);
+            c->plan_launches = plan[0];
+            c->plan_upd = upd;
+            c->plan_soff = soff;
+            c->plan_key = pkey;
+            c->plan_valid = true;
+        }
+    }
+    const std::vector<SgdTask>& tasks = cacheable ? c->plan_tasks : fresh;
     c->last.updates_per_epoch = upd;
-    // stratum launches carry their own, smaller grids: report the largest
+    // stratum and tile launches carry their own grids: report the largest
     int32_t most_blocks = 0;
     for (const LaunchRec& l : plan[0]) most_blocks = std::max(most_blocks, l.blocks);
     if (most_blocks > 0) c->last.blocks = most_blocks;
     if (tasks.empty()) return CUMF_OK;
-    std::vector<int64_t> soff;
-    if (c->streamed) {
-        if (!stratum) return fail(CUMF_ERR_STATE, "streamed ratings train with the stratum schedule");
-        if (o->stratum_rounds > 1)
-            return fail(CUMF_ERR_ARG, "streamed ratings make one pass over the diagonals per epoch (stratum_rounds 1)");
-        // point every task at its block's place in its diagonal's staging buffer
-        const int s = c->ux;
-        soff.assign((size_t)s * s, 0);
-        for (int d = 0; d < s; ++d) {
-            int64_t at = 0;
-            for (int i = 0; i < s; ++i) {
-                const int blk = i * s + (i + d) % s;
-                soff[(size_t)blk] = at;
-                at += c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk];
-            }
-        }
-        for (SgdTask& t : tasks) {
-            const int blk = (int)t.tag;
-            const int b = ((blk % s - blk / s + s) % s) & 1;
-            t.u = c->su[b] + soff[(size_t)blk];
-            t.v = c->sv[b] + soff[(size_t)blk];
-            t.r = c->sr[b] + soff[(size_t)blk];
-        }
-    }
-    if ((rc = ensure(&c->tasks, &c->tasks_cap, tasks.size()))) return rc;
-    HIP_TRY(hipMemcpyAsync(c->tasks, tasks.data(), sizeof(SgdTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
     int64_t launches_per_epoch = 0;
     for (const LaunchRec& l : plan[0]) (void)l, ++launches_per_epoch;
 
@@ -1776,22 +1858,14 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
     } else if (o->use_graph && !per_epoch_order) {
         // the epoch loop as a graph: captured once per plan, replayed per epoch.
         // The key covers everything the captured launches hold by value: the
-        // task table's contents and its device address (a larger plan
-        // reallocates it), the epoch counter's address, the launch shapes,
-        // the factor format and length and the row counts the FP8 rescale
-        // walks.
+        // plan key (which fixes the task table's contents, the launch shapes,
+        // the factor format, length and addresses and the row counts the FP8
+        // rescale walks), the seed, and the addresses of the task table (a
+        // larger plan reallocates it) and of the epoch counter.
         if ((rc = ensure(&c->epoch_dev, &c->epoch_dev_cap, 1))) return rc;
-        uint64_t key = fnv1a(tasks.data(), sizeof(SgdTask) * tasks.size(), 1469598103934665603ull);
-        const int32_t shape[13] = {sh.G, sh.S, sh.L, sh.W, sh.blocks, sh.D, o->coherent, c->prec, c->k, c->m, c->n,
-                                   o->schedule, c->tile_rows_max};
-        key = fnv1a(shape, sizeof(shape), key);
-        key = fnv1a(&o->seed, sizeof(o->seed), key);
-        const void* addr[7] = {c->tasks, c->epoch_dev, c->P, c->Q, c->ps, c->qs, c->lr};
+        uint64_t key = fnv1a(&o->seed, sizeof(o->seed), pkey);
+        const void* addr[2] = {c->tasks, c->epoch_dev};
         key = fnv1a(addr, sizeof(addr), key);
-        for (const LaunchRec& l : plan[0]) {
-            const int64_t f[4] = {l.task_off, l.task_cnt, l.num_xcd, l.blocks};
-            key = fnv1a(f, sizeof(f), key);
-        }
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) {
                 (void)hipGraphExecDestroy(c->graph_exec);
@@ -1877,14 +1951,15 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
     // a field the caller set (non-zero) stays fixed; the zero ones are searched
     std::vector<int> groups{64, 32, 16, 8}, depths{2, 4, 8}, waves_per_cu{32, 64, 128, 256, 512}, lens{128, 64, 32};
     const bool tiled = base->schedule == CUMF_SCHED_TILE;
-    std::vector<int> tile_workers{0};
+    std::vector<int> tile_workers{0};   // tile schedule: waves of workers per tile
     if (tiled) {
         // tile launches: the default lane group, depth 2, workgroups of 4, 8
         // or 16 waves (workers per tile) unless the caller fixed them
-        groups = {sgd_default_group(c->k, c->prec)};
+        const int g0 = sgd_default_group(c->k, c->prec);
+        groups = {g0};
+        if (sgd_shape_supported(c->k, c->prec, g0 / 2)) groups.push_back(g0 / 2);
         depths = {2};
-        const int S = 64 / groups[0];
-        if (!base->num_workers) tile_workers = {4 * S, 8 * S, 16 * S};
+        if (!base->num_workers) tile_workers = {4, 8, 16};
     }
     if (base->lanes_per_sample) groups = {base->lanes_per_sample};
     if (base->pipeline_depth) depths = {base->pipeline_depth};
@@ -1905,7 +1980,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
                     o.pipeline_depth = D;
                     o.batch_len = L;
                     if (base->num_workers) o.num_workers = base->num_workers;
-                    else if (tiled) o.num_workers = tile_workers[wi];
+                    else if (tiled) o.num_workers = tile_workers[wi] * (64 / G);
                     else o.num_workers = default_workers(shape_input(c, base), 64 / G, L, waves_per_cu[wi]);
                     if (tiled && sgd_tile_lds_bytes(c->k, c->prec, G, L, c->tile_rows_max,
                                                     tile_waves(o.num_workers, 64 / G)) > sgd_tile_lds_limit())

@@ -252,7 +252,14 @@ template <int TB, bool COH>
 __device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
 {
     Words<(TB + 3) / 4> r;
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        const Words<4> a = load_raw<16, COH>(p), b = load_raw<16, COH>(p + 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            r.w[i] = a.w[i];
+            r.w[4 + i] = b.w[i];
+        }
+    } else if constexpr (TB == 2) {
         static_assert(!COH, "2-byte coherent loads are not supported");
         r.w[0] = *(const __attribute__((address_space(1))) uint16_t*)p;
     } else if constexpr (TB == 4) {
@@ -269,7 +276,7 @@ __device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
         r.w[0] = (uint32_t)x;
         r.w[1] = (uint32_t)(x >> 32);
     } else {
-        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        static_assert(TB == 16, "row chunk must be 2, 4, 8, 16 or 32 bytes");
         if constexpr (COH) {
             uint64_t a = __hip_atomic_load((__attribute__((address_space(1))) uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint64_t b = __hip_atomic_load((__attribute__((address_space(1))) uint64_t*)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -286,7 +293,16 @@ __device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
 template <int TB, bool COH>
 __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
 {
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        Words<4> a, b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a.w[i] = r.w[i];
+            b.w[i] = r.w[4 + i];
+        }
+        store_raw<16, COH>(p, a);
+        store_raw<16, COH>(p + 16, b);
+    } else if constexpr (TB == 2) {
         static_assert(!COH, "2-byte coherent stores are not supported");
         *(__attribute__((address_space(1))) uint16_t*)p = (uint16_t)r.w[0];
     } else if constexpr (TB == 4) {
@@ -319,7 +335,7 @@ __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
 // the number of loads and stores in flight stays the same on every path and
 // the waits for a row can stay partial.
 constexpr uint32_t kBufRecords = 0xFFFFFF00u;
-constexpr uint32_t kBufSkip = 0xFFFFFFF0u;
+constexpr uint32_t kBufSkip = 0xFFFFFF80u;   // out of range for every chunk of a 32-byte access
 constexpr int kBufFlags = 0x00020000;   // gfx9 raw buffer: 32-bit data format, no swizzle
 
 __device__ inline __amdgpu_buffer_rsrc_t factor_rsrc(void* base)
@@ -331,7 +347,14 @@ template <int TB>
 __device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
     Words<(TB + 3) / 4> w;
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        // two 16-byte halves; the second at the scalar offset 16 (a skipped
+        // lane's offset stays out of range with it)
+        const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(r, off, 16, 0);
+        w.w[0] = x.x; w.w[1] = x.y; w.w[2] = x.z; w.w[3] = x.w;
+        w.w[4] = y.x; w.w[5] = y.y; w.w[6] = y.z; w.w[7] = y.w;
+    } else if constexpr (TB == 2) {
         w.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
     } else if constexpr (TB == 4) {
         w.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
@@ -341,7 +364,7 @@ __device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_
         w.w[0] = x.x;
         w.w[1] = x.y;
     } else {
-        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        static_assert(TB == 16, "row chunk must be 2, 4, 8, 16 or 32 bytes");
         const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
         w.w[0] = x.x; w.w[1] = x.y; w.w[2] = x.z; w.w[3] = x.w;
     }
@@ -351,7 +374,10 @@ __device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_
 template <int TB>
 __device__ inline void store_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, const Words<(TB + 3) / 4>& w)
 {
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[0], w.w[1], w.w[2], w.w[3]}, r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[4], w.w[5], w.w[6], w.w[7]}, r, off, 16, 0);
+    } else if constexpr (TB == 2) {
         __builtin_amdgcn_raw_buffer_store_b16((unsigned short)w.w[0], r, off, 0, 0);
     } else if constexpr (TB == 4) {
         __builtin_amdgcn_raw_buffer_store_b32(w.w[0], r, off, 0, 0);
@@ -370,7 +396,12 @@ template <int TB>
 __device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
 {
     Words<(TB + 3) / 4> r;
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        const u32x4 x = *(const __attribute__((address_space(3))) u32x4*)p;
+        const u32x4 y = *(const __attribute__((address_space(3))) u32x4*)(p + 16);
+        r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+        r.w[4] = y.x; r.w[5] = y.y; r.w[6] = y.z; r.w[7] = y.w;
+    } else if constexpr (TB == 2) {
         r.w[0] = *(const __attribute__((address_space(3))) uint16_t*)p;
     } else if constexpr (TB == 4) {
         r.w[0] = *(const __attribute__((address_space(3))) uint32_t*)p;
@@ -379,7 +410,7 @@ __device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
         r.w[0] = (uint32_t)x;
         r.w[1] = (uint32_t)(x >> 32);
     } else {
-        static_assert(TB == 16, "row chunk must be 2, 4, 8 or 16 bytes");
+        static_assert(TB == 16, "row chunk must be 2, 4, 8, 16 or 32 bytes");
         const u32x4 x = *(const __attribute__((address_space(3))) u32x4*)p;
         r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
     }
@@ -389,7 +420,10 @@ __device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
 template <int TB>
 __device__ inline void store_lds(lchar* p, const Words<(TB + 3) / 4>& r)
 {
-    if constexpr (TB == 2) {
+    if constexpr (TB == 32) {
+        *(__attribute__((address_space(3))) u32x4*)p = u32x4{r.w[0], r.w[1], r.w[2], r.w[3]};
+        *(__attribute__((address_space(3))) u32x4*)(p + 16) = u32x4{r.w[4], r.w[5], r.w[6], r.w[7]};
+    } else if constexpr (TB == 2) {
         *(__attribute__((address_space(3))) uint16_t*)p = (uint16_t)r.w[0];
     } else if constexpr (TB == 4) {
         *(__attribute__((address_space(3))) uint32_t*)p = r.w[0];

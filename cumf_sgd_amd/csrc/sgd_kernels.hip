@@ -24,7 +24,7 @@ bool sgd_shape_supported(int k, int prec, int G)
     if (k % G) return false;
     int E = k / G;
     int eb = prec == kF32 ? 4 : (prec == kE4M3 ? 1 : 2);
-    if (E * eb < 2 || E * eb > 16) return false;
+    if (E * eb < 2 || E * eb > 32) return false;
     if (prec == kE4M3 && E % 4) return false;
     return true;
 }

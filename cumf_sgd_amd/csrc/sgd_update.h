@@ -533,22 +533,33 @@ inline hipError_t launch_tile_t(const SgdLaunch& a, int blocks, size_t lds, int 
 }
 
 // Tile launches use the default lane group (16 bytes of a row per lane, at
-// least 8 lanes), depth 2 and workgroups of 4, 8 or 16 waves.
+// least 8 lanes) or half of it (32 bytes per lane), depth 2 and workgroups
+// of 4, 8 or 16 waves.
+template <int K, int PREC, int G>
+inline hipError_t launch_tile_w(const SgdLaunch& a, int blocks, int waves, size_t lds, int stage_words, hipStream_t s)
+{
+    constexpr int EB = PrecBytes<PREC>::value;
+    if constexpr (G >= 8 && K % G == 0 && (K / G) * EB <= 32 && (K / G) * EB >= 2 &&
+                  (PREC != kE4M3 || (K / G) % 4 == 0)) {
+        switch (waves) {
+        case 4: return launch_tile_t<K, G, PREC, 2, 4>(a, blocks, lds, stage_words, s);
+        case 8: return launch_tile_t<K, G, PREC, 2, 8>(a, blocks, lds, stage_words, s);
+        case 16: return launch_tile_t<K, G, PREC, 2, 16>(a, blocks, lds, stage_words, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int K, int PREC>
 inline hipError_t launch_tile_k(const SgdLaunch& a, int blocks, int G, int D, int waves, size_t lds, int stage_words,
                                 hipStream_t s)
 {
     constexpr int EB = PrecBytes<PREC>::value;
     constexpr int G0 = (K * EB / 16) > 64 ? 64 : ((K * EB / 16) < 8 ? 8 : K * EB / 16);
-    if (G != G0 || D != 2) return hipErrorInvalidValue;
-    if constexpr (K % G0 == 0 && (K / G0) * EB <= 16 && (K / G0) * EB >= 2 && (PREC != kE4M3 || (K / G0) % 4 == 0)) {
-        switch (waves) {
-        case 4: return launch_tile_t<K, G0, PREC, 2, 4>(a, blocks, lds, stage_words, s);
-        case 8: return launch_tile_t<K, G0, PREC, 2, 8>(a, blocks, lds, stage_words, s);
-        case 16: return launch_tile_t<K, G0, PREC, 2, 16>(a, blocks, lds, stage_words, s);
-        default: return hipErrorInvalidValue;
-        }
-    }
+    if (D != 2) return hipErrorInvalidValue;
+    if (G == G0) return launch_tile_w<K, PREC, G0>(a, blocks, waves, lds, stage_words, s);
+    if (G == G0 / 2) return launch_tile_w<K, PREC, G0 / 2>(a, blocks, waves, lds, stage_words, s);
     return hipErrorInvalidValue;
 }
 
@@ -677,11 +688,11 @@ template <int K, int PREC>
 inline hipError_t launch_sgd_g(const SgdLaunch& a, int blocks, int G, bool coh, int D, hipStream_t s)
 {
     constexpr int EB = PrecBytes<PREC>::value;
-    // lane groups whose per-lane chunk is 2..16 bytes (4..16 for FP8)
+    // lane groups whose per-lane chunk is 2..32 bytes (4..32 for FP8)
     switch (G) {
 #define CUMF_G(GG)                                                                                     \
     case GG:                                                                                           \
-        if constexpr (K % GG == 0 && (K / GG) * EB <= 16 && (K / GG) * EB >= 2 &&                      \
+        if constexpr (K % GG == 0 && (K / GG) * EB <= 32 && (K / GG) * EB >= 2 &&                      \
                       (PREC != kE4M3 || (K / GG) % 4 == 0))                                            \
             return launch_sgd_c<K, GG, PREC>(a, blocks, coh, D, s);                                    \
         break;

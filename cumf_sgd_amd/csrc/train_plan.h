@@ -54,6 +54,13 @@ constexpr int kSmallFactorRounds = 4;
 // from HBM: the default schedule then groups each block's records by row.
 constexpr int64_t kHbmResidentP = 1ll << 30;
 
+// Tile launches: the default lane group (32 bytes of a row per lane where
+// supported), workers per tile, strata and passes of the default schedule.
+int tile_default_group(int k, int prec);
+constexpr int kTileWorkers = 32;
+constexpr int kTileStrata = 8;
+constexpr int kTileFp32Rounds = 2;
+
 // lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
 // (singleGPU/sgd_kernel.cu:137-154).
 void lr_table(const cumf_train_opts* o, int n, float* out);

@@ -345,6 +345,7 @@ int cumf_sgd_update_k128(const cumf_param* para, int32_t m, int32_t n, int64_t u
         int32_t order = 0;
         CUMF_TRY(cumf_default_schedule(g.c, p.k, CUMF_FP16, &o.schedule, &o.stratum_rounds, &order));
         if (o.schedule == CUMF_SCHED_STRATUM) CUMF_TRY(cumf_partition_strata(g.c, 24, nullptr));
+        else if (o.schedule == CUMF_SCHED_TILE) CUMF_TRY(cumf_partition_tiles(g.c, 0, 0, nullptr));
         else if (order) CUMF_TRY(cumf_partition(g.c, 1, 1, nullptr));
         if (order) CUMF_TRY(cumf_order_blocks(g.c, order));
         CUMF_TRY(cumf_train(g.c, &o, 0, p.num_iters, nullptr, 0));
@@ -506,6 +507,9 @@ int train_loaded(cumf_ctx* ctx, const cumf_param& p, const cumf_train_opts* extr
         const int s = ux == vy && ux > 1 ? ux : 24;
         CUMF_TRY(cumf_partition_strata(ctx, s, nullptr));
         if (block_order) CUMF_TRY(cumf_order_blocks(ctx, block_order));
+    } else if (o.schedule == CUMF_SCHED_TILE) {
+        // row tiles of 8 x 8 strata (a square grid gives s)
+        CUMF_TRY(cumf_partition_tiles(ctx, ux == vy && ux > 1 ? ux : 0, 0, nullptr));
     } else if (block_order && o.schedule == CUMF_SCHED_HOGWILD) {
         // Hogwild's one block: all records grouped by column
         CUMF_TRY(cumf_partition(ctx, 1, 1, nullptr));

@@ -8,8 +8,8 @@
 //   --seed N        Philox key and srand seed (reference: time(NULL))
 //   --precision P   fp32 | fp16 (default, the reference's) | bf16 | bf16sr | fp8
 //   --serial        one worker, deterministic window order
-//   --schedule S    hogwild | grid | stratum (default: a 1x1 grid trains the
-//                   library's choice for the ratings, cumf_default_schedule;
+//   --schedule S    hogwild | grid | stratum | tile (default: a 1x1 grid trains
+//                   the library's choice for the ratings, cumf_default_schedule;
 //                   a larger grid trains its blocks in the reference's order)
 //   --lanes G       lanes per update (8..64)
 //   --rounds R      stratum schedule (implied on a 1x1 grid): passes over the
@@ -109,11 +109,12 @@ Args parse(int argc, char** argv)
         }
         else if (s == "--serial") a.schedule = CUMF_SCHED_SERIAL;
         else if (s == "--schedule") {
-            need("hogwild|grid|stratum after --schedule");
+            need("hogwild|grid|stratum|tile after --schedule");
             std::string v = argv[i];
             if (v == "hogwild") a.schedule = CUMF_SCHED_HOGWILD;
             else if (v == "grid") a.schedule = CUMF_SCHED_GRID;
             else if (v == "stratum") a.schedule = CUMF_SCHED_STRATUM;
+            else if (v == "tile") a.schedule = CUMF_SCHED_TILE;
             else throw std::invalid_argument("unknown schedule " + v);
         }
         else if (s == "--lanes") { need("a number after --lanes"); num("--lanes"); a.lanes = std::atoi(argv[i]); }

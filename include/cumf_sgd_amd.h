@@ -74,7 +74,8 @@ typedef struct cumf_train_opts {
     int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU (FP8 or stratum: 512), rounded for least over-sampling */
     int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128 */
     int32_t schedule;          /* enum cumf_schedule */
-    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane */
+    int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane (tile
+                                  schedule: 32 B where k and the format allow it) */
     int32_t coherent;          /* 1: factor loads/stores coherent across XCDs (device scope) */
     int32_t pipeline_depth;    /* row pairs in flight per worker: 2, 4 or 8; 0 = 2 */
     int32_t use_graph;         /* 1: run the epoch loop as a captured HIP graph, replayed per epoch */
@@ -89,7 +90,7 @@ typedef struct cumf_train_opts {
     int32_t tiles_in_flight;   /* tile schedule: tiles one launch trains per XCD (a diagonal's other tiles
                                   in later launches); 0 = all of a diagonal's tiles in one launch.  For
                                   the tile schedule num_workers counts the workers of one tile, at most 16
-                                  waves' worth (0 = 8 waves'); the tile's workgroup has 4, 8 or 16 waves */
+                                  waves' worth (0 = 32); the tile's workgroup has 4, 8 or 16 waves */
 } cumf_train_opts;
 
 typedef struct cumf_ctx cumf_ctx;
@@ -174,21 +175,23 @@ int cumf_partition_strata(cumf_ctx* ctx, int32_t s, int64_t* offsets);
  * streamed ratings. */
 enum cumf_order { CUMF_ORDER_ROWS = 1, CUMF_ORDER_COLUMNS = 2 };
 int cumf_order_blocks(cumf_ctx* ctx, int32_t by);
-/* Partition for CUMF_SCHED_TILE: the s x s blocks of cumf_partition_strata,
+/* Partition for CUMF_SCHED_TILE: the s x s blocks of cumf_partition_strata
+ * (s = 0: 8, one row band per XCD and diagonal),
  * each block's records then grouped by tile and, inside a tile, ordered by
  * column (stable).  A tile is a range of consecutive rows of one row band
  * (at most tile_rows rows, cut where the band's ratings would otherwise be
  * unevenly spread over its tiles); tile_rows = 0 takes cumf_tile_rows(k,
- * precision) for the factors the context holds, or FP16 at k = 128 before
- * they are initialised.  No counterpart in the reference (its blocks are
+ * precision) for the factors the context holds, or k = 128 before they are
+ * initialised.  No counterpart in the reference (its blocks are
  * trained by all workers from memory, singleGPU/sgd_kernel.cu:249-372).  A
  * tile's P rows fit one workgroup's LDS, so the tile trains without sending
  * its P rows through the L2.  Up to 2^32 - 1 ratings; not for streamed
  * ratings.  offsets (s*s + 1 entries, may be NULL) receives the block
  * bounds; cumf_get_tiles returns the tiles. */
 int cumf_partition_tiles(cumf_ctx* ctx, int32_t s, int32_t tile_rows, int64_t* offsets);
-/* Rows per tile for factors of length k in a format: the most that fit the
- * default 64 KB tile (two tile workgroups per CU). */
+/* Default rows per tile for factors of length k in a format: 128 * 128 / k,
+ * at most 64 KB of rows (measured fastest on a Netflix-shaped problem for
+ * FP32 and FP16). */
 int32_t cumf_tile_rows(int32_t k, int32_t precision);
 /* The tiles of cumf_partition_tiles, block by block (block-major, then tile
  * order within the block): *count receives their number; when the arrays
@@ -253,11 +256,14 @@ int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* 
  *  - a row and a column that each hold at least 1% of the ratings:
  *    CUMF_SCHED_HOGWILD (one diagonal at a time would pull both towards one
  *    range of the other axis per sub-epoch), its one block's records ordered
- *    by column for 16-bit formats;
+ *    by column for the round-to-nearest formats (FP32, FP16, BF16);
  *  - otherwise CUMF_SCHED_STRATUM, one pass, each block's records ordered
  *    by column (cumf_order_blocks, CUMF_ORDER_COLUMNS), or by row when P
  *    takes more than 1 GB (its rows then come from HBM), when there are at
- *    most 2^32 - 1 ratings.
+ *    most 2^32 - 1 ratings;
+ *  - except, for FP32 factors where that would be strata ordered by column:
+ *    CUMF_SCHED_TILE with two passes over the diagonals per epoch, on the
+ *    partition of cumf_partition_tiles(ctx, 0, 0, ...) (*order 0).
  * *order receives CUMF_ORDER_COLUMNS, CUMF_ORDER_ROWS or 0 (keep the
  * records' order).
  * It replaces the reference's fixed choice of Hogwild for a 1x1 grid

@@ -36,8 +36,8 @@ def test_large_sparse_formats(gpu):
     """1,000,990 x 624,961, 252,800,275 ratings (Zipf(1.0) on both axes,
     bimodal 0..100 values, generated on the host), 6 epochs per run.  The
     library's default for it is Hogwild (the hottest user and the hottest
-    item each hold about 7% of the ratings; 16-bit rows train the records
-    grouped by item): FP32 and BF16-SR under it end at most 0.005 rating
+    item each hold about 7% of the ratings; FP32 and FP16 train the records
+    grouped by item, BF16-SR in input order): FP32 and BF16-SR under it end at most 0.005 rating
     standard deviations above the serialized FP32 oracle
     (tests/golden/large_oracle_rmse.json, produced on exactly this data and
     preprocessing).  FP32, BF16, BF16-SR and FP8 storage under the stratum
@@ -84,13 +84,14 @@ def test_large_sparse_formats(gpu):
             print(f"{name}: rmse={[round(x, 3) for x in curve]} {c.nnz * 6 / t / 1e9:.2f}e9 updates/s{gap}")
             assert np.all(np.isfinite(curve)) and curve[-1] < curve[0]
 
-        assert c.default_schedule(128, 0) == (cs.HOGWILD, 1, 0)
-        assert c.default_schedule(128, 4) == (cs.HOGWILD, 1, cs.ORDER_COLUMNS)
-        run("hogwild fp32", 0, cs.HOGWILD)
+        assert c.default_schedule(128, 4) == (cs.HOGWILD, 1, 0)
+        assert c.default_schedule(128, 0) == (cs.HOGWILD, 1, cs.ORDER_COLUMNS)
+        assert c.default_schedule(128, 1) == (cs.HOGWILD, 1, cs.ORDER_COLUMNS)
+        run("hogwild bf16-sr", 4, cs.HOGWILD)     # stochastic rounding: records in input order
         u0, v0, r0 = c.get_ratings()
         c.partition(1, 1)
         c.order_blocks(cs.ORDER_COLUMNS)
-        run("hogwild bf16-sr", 4, cs.HOGWILD)     # the default for 16-bit rows: records grouped by item
+        run("hogwild fp32", 0, cs.HOGWILD)        # round to nearest: records grouped by item
         c.set_ratings(u0, v0, r0, 1_000_990, 624_961)
         off = c.partition_strata(16)
         assert off[-1] == c.nnz and np.all(np.diff(off) >= 0)
@@ -167,14 +168,16 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
         assert info.updates_per_epoch >= nnz
 
 
-@pytest.mark.parametrize("prec,schedule", [(0, "hogwild"), (1, "hogwild"), (0, "default"), (1, "default")])
+@pytest.mark.parametrize("prec,schedule", [(0, "hogwild"), (1, "hogwild"), (0, "default"), (1, "default"),
+                                           (1, "tile")])
 def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
     """Netflix-shaped 99M-rating workload, 20 epochs, under Hogwild (the
-    reference's 1x1 mode, default launch shape) and under the library's
-    default for it (cumf_default_schedule: 24 x 24 strata, one pass, each
-    block's records ordered by item; with the benchmark's 32-record windows
-    and graph replay): the test RMSE is
-    at most 0.005 above the serialized FP32 CPU oracle's at 20 epochs
+    reference's 1x1 mode, default launch shape), under the library's default
+    for it (cumf_default_schedule: for FP16 24 x 24 strata, one pass, each
+    block's records ordered by item; for FP32 row tiles of 8 x 8 strata with
+    the P rows in LDS, two passes) and under FP16 row tiles in one pass, with
+    the benchmark's 32-record windows and graph replay: the test RMSE is at
+    most 0.005 above the serialized FP32 CPU oracle's at 20 epochs
     (tests/golden/medium_oracle_rmse.json, produced on exactly this data and
     preprocessing)."""
     import json
@@ -194,11 +197,15 @@ def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
         scale, b, lam = bench.prepare(c, R, 2)
         if schedule == "hogwild":
             times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.HOGWILD)
+        elif schedule == "tile":
+            c.init_factors(128, prec, 2)
+            c.partition_tiles(8)
+            times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.TILE, batch_len=32, graph=1)
         else:
-            sched, rounds, order = c.default_schedule(128, prec)
-            assert (sched, rounds, order) == (cs.STRATUM, 1, cs.ORDER_COLUMNS)
-            c.partition_strata(24)
-            c.order_blocks(order)
+            want = (cs.TILE, 2, 0) if prec == cs.FP32 else (cs.STRATUM, 1, cs.ORDER_COLUMNS)
+            assert c.default_schedule(128, prec) == want
+            c.init_factors(128, prec, 2)
+            sched, rounds = c.prepare_default(128, prec)
             times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, sched, batch_len=32, rounds=rounds,
                                             graph=1)
     del R

@@ -54,6 +54,8 @@ SERIAL_CASES = [
     (32, 16, 0, 8), (32, 8, 2, 4), (32, 8, 3, 8),
     # BF16 with stochastic rounding, including two elements per lane (G=64)
     (128, 16, 4, 4), (128, 64, 4, 4), (64, 8, 4, 2), (32, 8, 4, 8),
+    # 32 bytes of a row per lane (two 16-byte accesses per row)
+    (128, 8, 1, 2), (128, 16, 0, 2), (128, 8, 2, 4), (128, 8, 4, 2), (64, 8, 0, 2),
 ]
 
 

@@ -56,7 +56,7 @@ def test_tile_partition_is_strata_then_stable_tile_column_sort(gpu, s, tile_rows
         toff, row0, rows = c.get_tiles()
     assert np.array_equal(off, off_s)
     assert toff[0] == 0 and toff[-1] == len(u) and np.all(np.diff(toff) > 0)
-    cap = tile_rows or 256   # cumf_tile_rows(128, FP16): 64 KB / 256 B
+    cap = tile_rows or 128   # cumf_tile_rows(128, ...) before factors exist
     assert rows.max() <= cap and rows.min() >= 1
     # every block is a whole number of tiles
     assert set(off.tolist()) <= set(toff.tolist())
@@ -183,7 +183,8 @@ def test_tile_full_workgroups_train_every_rating_window(gpu):
         sizes = np.diff(toff)
         uc = (sizes + W * L - 1) // (W * L)     # fewest windows per worker
         w = (sizes + uc * L - 1) // (uc * L)    # then fewest workers keeping them
-        assert W == 32 and info.updates_per_epoch == int(np.sum(w * uc * L))
+        assert W == 32 and info.lanes_per_sample == 8   # 32 bytes of an FP16 row per lane
+        assert info.updates_per_epoch == int(np.sum(w * uc * L))
         r1 = c.test_rmse(1.0, 0.0)
         P, _ = c.get_factors(0)
         assert np.all(np.isfinite(P.view(np.float16).astype(np.float32)))
@@ -204,7 +205,7 @@ def test_tile_refusals(gpu):
             c.train(cs.make_opts(**base), 0, 1)
         assert e.value.code == 3
         c.partition_tiles(8)
-        for bad in (dict(lanes_per_sample=64), dict(coherent=1), dict(pipeline_depth=4), dict(num_workers=65),
+        for bad in (dict(lanes_per_sample=64), dict(coherent=1), dict(pipeline_depth=4), dict(num_workers=1000),
                     dict(tiles_in_flight=-1)):
             with pytest.raises(cs.CumfError) as e:
                 c.train(cs.make_opts(**{**base, **bad}), 0, 1)
@@ -218,3 +219,37 @@ def test_tile_refusals(gpu):
         with pytest.raises(cs.CumfError) as e:
             c.train(cs.make_opts(**base), 0, 1)
         assert e.value.code == 3
+
+
+def test_epoch_per_call_reuses_the_plan_bit_exactly(gpu):
+    """One epoch per cumf_train call reuses the task table planned by the
+    call before (the plan key matches), also after a call that planned
+    another schedule in between (a Hogwild epoch with a zero learning rate,
+    which leaves the factors as they are): three such calls equal one call
+    of three epochs bit for bit, for the tile and the stratum schedule."""
+    import cumf_sgd_amd as cs
+
+    u, v, r, _, _, _, m, n = _problem(seed=26, m=320, n=200, nnz=12_000)
+    for sched in (cs.TILE, cs.STRATUM):
+        out = []
+        for per_call in (False, True):
+            with _ctx(u, v, r, m, n) as c:
+                c.init_factors(64, cs.FP16, seed=4)
+                if sched == cs.TILE:
+                    c.partition_tiles(8, 8)
+                else:
+                    c.partition_strata(8)
+                opts = cs.make_opts(k=64, schedule=sched, num_workers=1, tiles_in_flight=1, seed=4, batch_len=32,
+                                    use_graph=1)
+                idle = cs.make_opts(k=64, schedule=cs.HOGWILD, seed=4, alpha=0.0, beta=0.0)
+                if per_call:
+                    for ep in range(3):
+                        c.train(opts, ep, ep + 1)
+                        if ep == 1:
+                            c.train(idle, 0, 1)
+                else:
+                    c.train(opts, 0, 3)
+                out.append((c.get_factors(0)[0], c.get_factors(1)[0]))
+        (p3, q3), (p1, q1) = out
+        assert np.array_equal(p3.view(np.uint8), p1.view(np.uint8)), sched
+        assert np.array_equal(q3.view(np.uint8), q1.view(np.uint8)), sched
