@@ -3,26 +3,29 @@
 Workload (one GPU): the medium Netflix-shaped problem, 480,189 users x
 17,770 items, 99,072,112 training + 1,408,395 test ratings (integers 1..5),
 k = 128, lambda 0.05, alpha 0.08, beta 0.3, generated (seeded) on the host
-and copied to HBM.  One
-step = one epoch of batch-Hogwild SGD over all training ratings (the
-reference's update_per_sec = nnz * iters / time, singleGPU/sgd_kernel.cu:244),
-by default under the stratum schedule (24 x 24 nnz-balanced blocks, the 24
-conflict-free blocks of a diagonal spread over the 8 XCDs largest first onto
-the least-loaded one, one launch per diagonal, each block's records ordered
-by item; --schedule hogwild runs the reference's 1x1 mode).  Each epoch's launches are captured once as a HIP
-graph and replayed (--no-graph launches them one by one).  The launch shape
-is fixed (the library's defaults for the format with 32-record windows, which
-is what cumf_autotune picks on this workload), so that the profiled kernel is
-the timed one; --autotune runs the autotuner first instead.
+and copied to HBM.  One step = one epoch of batch-Hogwild SGD over all
+training ratings (the reference's update_per_sec = nnz * iters / time,
+singleGPU/sgd_kernel.cu:244), by default under the library's schedule for
+this problem and format (cumf_default_schedule; for the FP16 headline: the
+blocks of 8 x 8 nnz-balanced strata split into tiles of 96 consecutive users,
+each tile trained by one workgroup with its users' P rows in LDS, records
+ordered by item inside a tile, one launch per diagonal with each diagonal's
+blocks on distinct XCDs; --schedule stratum runs 24 x 24 strata with
+item-ordered blocks, --schedule hogwild the reference's 1x1 mode).  Each
+epoch's launches are captured once as a HIP graph and replayed (--no-graph
+launches them one by one).  The launch shape is fixed (the library's
+defaults for the format with 32-record windows, which is what cumf_autotune
+picks on this workload), so that the profiled kernel is the timed one;
+--autotune runs the autotuner first instead.
 Inputs are resident in HBM before the timed region starts.
 
 Printed: one JSON line with value (rating updates/s of the timed epochs),
-roofline of the update kernel (algorithmic bytes per launch over its
-HIP-event duration against the L2 bandwidth that serves them, and the
-measured HBM-side traffic of the same launch shape against the HBM peak;
-with graph replay the events bracket each epoch's graph, so the average
-launch includes the short gaps between launches), time-to-target RMSE for
-each factor format against the FP32 GPU run, and the reference's CPU trainer
+roofline of the update kernel per level (L2 bytes, HBM bytes and VALU issue
+over its HIP-event duration against each level's peak, from the committed
+profile of the same kernel and launch shape; with graph replay the events
+bracket each epoch's graph, so the average launch includes the short gaps
+between launches), time-to-target RMSE for each factor format against the
+FP32 GPU run (each format on its library default), and the reference's CPU trainer
 (test/mf-train, built from the reference sources into oracle/_ref) timed on
 the host's cores, or the CPU oracle when that build is absent.
 """
@@ -161,6 +164,19 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
+# VALU issue peak: one wave64 vector instruction per CU per clock (four
+# SIMDs, each a wave64 instruction per 4 clocks), 256 CUs at 2.4 GHz, in
+# lane operations (64 per instruction): 39.3e12, the FP32 FMA peak of 78.6
+# TFLOP/s counted once per FMA.
+VALU_PEAK_TLANE = 256 * 2.4e9 * 64 / 1e12
+
+
+def tile_waves(workers: int, lanes: int) -> int:
+    """Waves of a tile workgroup (the library's rule: 4, 8 or 16)."""
+    w = -(-workers // (64 // lanes))
+    return 4 if w <= 4 else (8 if w <= 8 else 16)
+
+
 def profile_summary(prec: int, info, schedule: str, upl: float | None = None, order: str | None = None) -> dict | None:
     """The committed rocprofv3 counter summary (profiles/) of the update
     kernel on this workload and schedule, when it was measured on the same
@@ -169,12 +185,17 @@ def profile_summary(prec: int, info, schedule: str, upl: float | None = None, or
     launch once it exceeds the windows of a diagonal) and, when given, the
     same record order inside the blocks; None otherwise."""
     name = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp8", 4: "bf16sr"}[prec]
-    path = os.path.join(ROOT, "profiles", f"sgd_update_{name}_medium_{schedule}.json")
+    tiled = schedule.startswith("tile")
+    path = os.path.join(ROOT, "profiles", f"{'sgd_tile' if tiled else 'sgd_update'}_{name}_medium_{schedule}.json")
     if not os.path.exists(path):
         return None
     d = json.load(open(path))
     # the medium workload's factors are under 4 GB: rows addressed with 32-bit offsets
-    shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}, true>"
+    if tiled:
+        shape = (f"sgd_tile_kernel<128, {info.lanes_per_sample}, {prec}, {info.pipeline_depth}, true, "
+                 f"{tile_waves(info.num_workers, info.lanes_per_sample)}>")
+    else:
+        shape = f"sgd_update_kernel<128, {info.lanes_per_sample}, {prec}, false, {info.pipeline_depth}, true>"
     if shape not in d["kernel"]:
         return None
     if d.get("batch_len", 128) != info.batch_len:   # summaries without it used the default 128
@@ -194,29 +215,47 @@ def measured_traffic(prec: int, info, schedule: str, upl: float | None = None, o
 
 
 def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, order: str) -> dict:
-    """Roofline of the update kernel.  `achieved` is the algorithmic bytes
-    (every row read and written, the rating record) per launch over the
-    launch's HIP-event duration.  Those bytes are L2 requests: at the L2 the
-    kernel's fraction is achieved / L2 peak.  The HBM side moves only what
-    the L2s miss or write back: its fraction is the measured traffic (the
-    profile of the same launch shape) over the 8 TB/s HBM peak.  `bound` is
-    the level with the larger fraction."""
+    """Roofline of the update kernel, per level, over the launch's HIP-event
+    duration:
+      - l2: bytes the L2 serves.  In a stratum launch every row load and store
+        of an update is an L2 request, so these are the algorithmic bytes (the
+        12-byte record + P and Q rows read and written); a tile launch keeps
+        its P rows in LDS, and its level is the measured request bytes of the
+        matching profile (TCP_TCC_READ_REQ x 128 + TCP_TCC_WRITE_REQ x 64);
+      - hbm: the profile's calibrated FETCH_SIZE + WRITE_SIZE against 8 TB/s;
+      - valu: the profile's VALU instructions per launch (SQ_INSTS_VALU) as
+        lane operations against the VALU issue peak.
+    `bound` is the level with the largest fraction."""
     algo = BYTES_PER_UPDATE[prec] * upl / avg_launch_s / 1e9
     prof = profile_summary(prec, info, schedule, upl, order)
+    tiled = schedule.startswith("tile")
     traffic = None if prof is None else prof["traffic_bytes_per_launch"]
-    levels = {"l2": {"achieved": algo, "peak": L2_PEAK_GBS, "frac": algo / L2_PEAK_GBS,
-                     "bytes": "algorithmic (12-byte record + P and Q rows read and written)"}}
-    if traffic is not None:
-        hbm = traffic / avg_launch_s / 1e9
-        levels["hbm"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS,
-                         "bytes": "measured: calibrated FETCH_SIZE + WRITE_SIZE per launch"}
-    if prof is not None and "l2_request_bytes_per_launch" in prof:
+    levels = {}
+    if not tiled:
+        levels["l2"] = {"achieved": algo, "peak": L2_PEAK_GBS, "frac": algo / L2_PEAK_GBS, "unit": "GB/s",
+                        "bytes": "algorithmic (12-byte record + P and Q rows read and written)"}
+    if prof is not None:
         req = prof["l2_request_bytes_per_launch"] / avg_launch_s / 1e9
-        levels["l2"]["measured_requests_gbs"] = req
-    bound = max(levels, key=lambda k: levels[k]["frac"])
-    return {"bound": bound, "achieved": levels[bound]["achieved"], "peak": levels[bound]["peak"], "unit": "GB/s",
-            "frac": levels[bound]["frac"], "traffic": traffic, "levels": levels, "effective_algorithmic_gbs": algo,
-            "kernel": "sgd_update_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+        if tiled:
+            levels["l2"] = {"achieved": req, "peak": L2_PEAK_GBS, "frac": req / L2_PEAK_GBS, "unit": "GB/s",
+                            "bytes": "measured L2 requests (128-byte reads, 64-byte writes)"}
+        else:
+            levels["l2"]["measured_requests_gbs"] = req
+        hbm = traffic / avg_launch_s / 1e9
+        levels["hbm"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS, "unit": "GB/s",
+                         "bytes": "measured: calibrated FETCH_SIZE + WRITE_SIZE per launch"}
+        if "valu_insts_per_launch" in prof:
+            v = prof["valu_insts_per_launch"] * 64 / avg_launch_s / 1e12
+            levels["valu"] = {"achieved": v, "peak": VALU_PEAK_TLANE, "frac": v / VALU_PEAK_TLANE,
+                              "unit": "Tlane-op/s",
+                              "ops": f"measured SQ_INSTS_VALU x 64 lanes "
+                                     f"({prof['valu_insts_per_launch'] / upl:.1f} wave-instructions per update)"}
+    bound = max(levels, key=lambda k: levels[k]["frac"]) if levels else None
+    kernel = "sgd_tile_kernel" if tiled else "sgd_update_kernel"
+    return {"bound": bound, "achieved": levels[bound]["achieved"] if bound else None,
+            "peak": levels[bound]["peak"] if bound else None, "unit": levels[bound]["unit"] if bound else None,
+            "frac": levels[bound]["frac"] if bound else None, "traffic": traffic, "levels": levels,
+            "effective_algorithmic_gbs": algo, "kernel": kernel, "avg_launch_ms": avg_launch_s * 1e3,
             "bytes_per_update": BYTES_PER_UPDATE[prec], "updates_per_launch": upl,
             "profile": None if prof is None else prof.get("source")}
 
@@ -229,10 +268,12 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "bf16sr", "fp8"])
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--schedule", default="stratum", choices=["hogwild", "stratum", "tile"],
-                    help="stratum: s x s conflict-free blocks, each on one XCD, balanced per diagonal "
-                         "(default); tile: the blocks of 8 x 8 strata split into row tiles, each trained by one "
-                         "workgroup with its P rows in LDS; hogwild: the reference's 1x1 mode")
+    ap.add_argument("--schedule", default="default", choices=["default", "hogwild", "stratum", "tile"],
+                    help="default: the library's choice for this problem and format (cumf_default_schedule: row "
+                         "tiles for FP32, FP16 and BF16, strata for BF16-SR and FP8); stratum: s x s "
+                         "conflict-free blocks, each on one XCD, balanced per diagonal; tile: the blocks of 8 x 8 "
+                         "strata split into row tiles, each trained by one workgroup with its P rows in LDS; "
+                         "hogwild: the reference's 1x1 mode")
     ap.add_argument("--strata", type=int, default=0,
                     help="s for the stratum and tile schedules (s x s blocks; 0 = 24 for strata, 8 for tiles)")
     ap.add_argument("--order", default="columns", choices=["columns", "rows", "input"],
@@ -274,10 +315,7 @@ def main() -> int:
         print(json.dumps({"error": "no GPU"}))
         return 2
     prec = cs.PRECISIONS[args.precision]
-    sched = cs.SCHEDULES[args.schedule]
-    if not args.strata:
-        args.strata = 8 if sched == cs.TILE else 24
-    sched_tag = {cs.STRATUM: f"stratum{args.strata}", cs.TILE: f"tile{args.strata}"}.get(sched, "hogwild")
+    sched = cs.SCHEDULES.get(args.schedule)   # None: the library's default
     t0 = time.perf_counter()
     # generated on the host (deterministic across machines), then made resident in HBM
     torch.set_num_threads(min(16, os.cpu_count() or 1))
@@ -295,13 +333,25 @@ def main() -> int:
 
     ctx = cs.Context(0)
     scale, b, lam = prepare(ctx, R, args.seed)
-    if sched == cs.STRATUM:
-        ctx.partition_strata(args.strata)
+    default_rounds = 0
+    if sched is None:
+        ctx.init_factors(K, prec, args.seed)   # the tile rows follow the factor format
+        sched, default_rounds = ctx.prepare_default(K, prec)
+        order_of = ctx.default_schedule(K, prec)[2]
+        args.order = {cs.ORDER_COLUMNS: "columns", cs.ORDER_ROWS: "rows"}.get(order_of, "input")
+        args.strata = 8 if sched == cs.TILE else 24
+    elif sched == cs.STRATUM:
+        ctx.partition_strata(args.strata or 24)
         if args.order != "input":
             ctx.order_blocks(cs.ORDER_COLUMNS if args.order == "columns" else cs.ORDER_ROWS)
     elif sched == cs.TILE:
         ctx.init_factors(K, prec, args.seed)   # the tile rows follow the factor format
-        ctx.partition_tiles(args.strata)
+        ctx.partition_tiles(args.strata or 8)
+    if not args.strata:
+        args.strata = 8 if sched == cs.TILE else 24
+    if not args.rounds:
+        args.rounds = default_rounds
+    sched_tag = {cs.STRATUM: f"stratum{args.strata}", cs.TILE: f"tile{args.strata}"}.get(sched, "hogwild")
     torch.cuda.synchronize()
     log(f"preprocessed: scale={scale:.5f} b={b:.5f}")
 
@@ -436,7 +486,7 @@ def main() -> int:
                    "lanes_per_sample": info.lanes_per_sample, "batch_len": info.batch_len,
                    "updates_per_epoch": info.updates_per_epoch},
         "roofline": roofline(prec, info, sched_tag, avg_launch_s, upd_per_launch,
-                             args.order if sched == cs.STRATUM else "input"),
+                             {cs.STRATUM: args.order, cs.TILE: "row tiles, columns within a tile"}.get(sched, "input")),
         "time_to_target": ttt,
         "cpu_baseline": cpu,
     }

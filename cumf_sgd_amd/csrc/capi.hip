@@ -739,10 +739,13 @@ int cumf_order_blocks(cumf_ctx* c, int32_t by)
 int32_t cumf_tile_rows(int32_t k, int32_t precision)
 {
     if (k < 1 || precision < CUMF_FP32 || precision > CUMF_BF16_SR) return 0;
-    // 128 rows at k = 128 (more rows per tile at smaller k), at most 64 KB:
-    // on the 99M workload 128-row tiles trained fastest for FP32 (64 KB) and
-    // FP16 (32 KB) alike; larger ones leave fewer workgroups per CU
-    const int64_t rows = 128 * 128 / k, cap = kSgdTileDefaultBytes / ((int64_t)k * prec_bytes(precision));
+    // At k = 128: 128 FP32 rows (64 KB, two 8-wave workgroups per CU), 96
+    // rows of the narrower formats (24 KB at FP16: four 4-wave workgroups per
+    // CU, which the registers of those waves allow); more rows at smaller k,
+    // at most 64 KB.  Measured fastest on the 99M workload (FP16: 64-128 rows
+    // within 4%, 96 best).
+    const int64_t base = precision == CUMF_FP32 ? 128 : 96;
+    const int64_t rows = base * 128 / k, cap = kSgdTileDefaultBytes / ((int64_t)k * prec_bytes(precision));
     return (int32_t)std::max<int64_t>(1, std::min(rows, cap));
 }
 
@@ -919,13 +922,17 @@ int cumf_default_schedule(cumf_ctx* c, int32_t k, int32_t precision, int32_t* sc
     // ones kept in registers
     if (*order && *schedule == CUMF_SCHED_STRATUM && (int64_t)c->m * k * prec_bytes(precision) > kHbmResidentP)
         *order = CUMF_ORDER_ROWS;
-    // FP32 rows make the strata L2-bound (a 512-byte P row read and written
-    // per update); a tile keeps its P rows in LDS instead, with two passes
-    // over the diagonals per epoch (one pass ends at the 0.005 bar on the
-    // Netflix shape, two 0.002 above the serialized oracle)
-    if (*schedule == CUMF_SCHED_STRATUM && *order == CUMF_ORDER_COLUMNS && precision == CUMF_FP32) {
+    // Where that is strata ordered by column, the round-to-nearest formats
+    // train row tiles instead: a tile keeps its P rows in LDS, so the L2 sees
+    // only the item rows and the records (99M Netflix shape: FP32 9.2e9 ->
+    // 19.1e9 updates/s, FP16 21.5e9 -> 25.3e9).  FP32 makes two passes over
+    // the diagonals per epoch (one pass ends at the 0.005 bar there, two
+    // 0.003 above the serialized oracle).  The stochastically rounded formats
+    // spend more VALU work per update and keep the strata (BF16-SR 15.8e9
+    // against 12.9e9 in tiles).
+    if (*schedule == CUMF_SCHED_STRATUM && *order == CUMF_ORDER_COLUMNS && !stochastic) {
         *schedule = CUMF_SCHED_TILE;
-        *stratum_rounds = kTileFp32Rounds;
+        *stratum_rounds = precision == CUMF_FP32 ? kTileFp32Rounds : 1;
         *order = 0;
     }
     return CUMF_OK;

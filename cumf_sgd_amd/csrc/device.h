@@ -248,12 +248,12 @@ __device__ inline __attribute__((address_space(1))) T* to_global(T* p)
     return (__attribute__((address_space(1))) T*)p;
 }
 
-template <int TB, bool COH>
+template <int TB, bool COH, int SPLIT = 16>
 __device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
 {
     Words<(TB + 3) / 4> r;
     if constexpr (TB == 32) {
-        const Words<4> a = load_raw<16, COH>(p), b = load_raw<16, COH>(p + 16);
+        const Words<4> a = load_raw<16, COH>(p), b = load_raw<16, COH>(p + SPLIT);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             r.w[i] = a.w[i];
@@ -290,7 +290,7 @@ __device__ inline Words<(TB + 3) / 4> load_raw(const gchar* p)
     return r;
 }
 
-template <int TB, bool COH>
+template <int TB, bool COH, int SPLIT = 16>
 __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
 {
     if constexpr (TB == 32) {
@@ -301,7 +301,7 @@ __device__ inline void store_raw(gchar* p, const Words<(TB + 3) / 4>& r)
             b.w[i] = r.w[4 + i];
         }
         store_raw<16, COH>(p, a);
-        store_raw<16, COH>(p + 16, b);
+        store_raw<16, COH>(p + SPLIT, b);
     } else if constexpr (TB == 2) {
         static_assert(!COH, "2-byte coherent stores are not supported");
         *(__attribute__((address_space(1))) uint16_t*)p = (uint16_t)r.w[0];
@@ -343,15 +343,18 @@ __device__ inline __amdgpu_buffer_rsrc_t factor_rsrc(void* base)
     return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)kBufRecords, kBufFlags);
 }
 
-template <int TB>
+// A 32-byte chunk is two 16-byte pieces SPLIT bytes apart (half a row: the
+// lane group's first pieces cover the row's first half contiguously, one
+// instruction per half); smaller chunks are one piece.
+template <int TB, int SPLIT = 16>
 __device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
     Words<(TB + 3) / 4> w;
     if constexpr (TB == 32) {
-        // two 16-byte halves; the second at the scalar offset 16 (a skipped
-        // lane's offset stays out of range with it)
+        // the second piece at the scalar offset SPLIT (a skipped lane's
+        // offset stays out of range with it)
         const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-        const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(r, off, 16, 0);
+        const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(r, off, SPLIT, 0);
         w.w[0] = x.x; w.w[1] = x.y; w.w[2] = x.z; w.w[3] = x.w;
         w.w[4] = y.x; w.w[5] = y.y; w.w[6] = y.z; w.w[7] = y.w;
     } else if constexpr (TB == 2) {
@@ -371,12 +374,12 @@ __device__ inline Words<(TB + 3) / 4> load_buf(__amdgpu_buffer_rsrc_t r, uint32_
     return w;
 }
 
-template <int TB>
+template <int TB, int SPLIT = 16>
 __device__ inline void store_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, const Words<(TB + 3) / 4>& w)
 {
     if constexpr (TB == 32) {
         __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[0], w.w[1], w.w[2], w.w[3]}, r, off, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[4], w.w[5], w.w[6], w.w[7]}, r, off, 16, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(u32x4{w.w[4], w.w[5], w.w[6], w.w[7]}, r, off, SPLIT, 0);
     } else if constexpr (TB == 2) {
         __builtin_amdgcn_raw_buffer_store_b16((unsigned short)w.w[0], r, off, 0, 0);
     } else if constexpr (TB == 4) {
@@ -392,13 +395,13 @@ __device__ inline void store_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, const W
 // The same row chunks in LDS (a tile of P rows held by one workgroup).
 typedef __attribute__((address_space(3))) char lchar;
 
-template <int TB>
+template <int TB, int SPLIT = 16>
 __device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
 {
     Words<(TB + 3) / 4> r;
     if constexpr (TB == 32) {
         const u32x4 x = *(const __attribute__((address_space(3))) u32x4*)p;
-        const u32x4 y = *(const __attribute__((address_space(3))) u32x4*)(p + 16);
+        const u32x4 y = *(const __attribute__((address_space(3))) u32x4*)(p + SPLIT);
         r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
         r.w[4] = y.x; r.w[5] = y.y; r.w[6] = y.z; r.w[7] = y.w;
     } else if constexpr (TB == 2) {
@@ -417,12 +420,12 @@ __device__ inline Words<(TB + 3) / 4> load_lds(const lchar* p)
     return r;
 }
 
-template <int TB>
+template <int TB, int SPLIT = 16>
 __device__ inline void store_lds(lchar* p, const Words<(TB + 3) / 4>& r)
 {
     if constexpr (TB == 32) {
         *(__attribute__((address_space(3))) u32x4*)p = u32x4{r.w[0], r.w[1], r.w[2], r.w[3]};
-        *(__attribute__((address_space(3))) u32x4*)(p + 16) = u32x4{r.w[4], r.w[5], r.w[6], r.w[7]};
+        *(__attribute__((address_space(3))) u32x4*)(p + SPLIT) = u32x4{r.w[4], r.w[5], r.w[6], r.w[7]};
     } else if constexpr (TB == 2) {
         *(__attribute__((address_space(3))) uint16_t*)p = (uint16_t)r.w[0];
     } else if constexpr (TB == 4) {

@@ -100,8 +100,8 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     const Key key = make_key(a.seed, T.problem);
     const int L = a.batch_len;   // host guarantees 1 <= L <= kMaxBatch and L % D == 0
 
-    gchar* Pb = to_global((char*)T.P) + (int64_t)ls * TB;
-    gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * TB;
+    gchar* Pb = to_global((char*)T.P) + (int64_t)ls * (TB == 32 ? 16 : TB);
+    gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * (TB == 32 ? 16 : TB);
     gchar* Pl = to_global((char*)FP);
     gchar* Ql = to_global((char*)FQ);
     gcint* Ru = to_global(T.u);
@@ -111,7 +111,14 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     gcfloat* Qs = to_global((const float*)(OFF32 ? FQs : T.qs));
     gcfloat* LR = to_global(T.lr);
     constexpr int64_t kRow = (int64_t)K * EB;
-    const uint32_t lane_off = (uint32_t)(ls * TB);
+    // Lane l's elements: [lE, lE + E) for chunks up to 16 bytes; a 32-byte
+    // chunk is two pieces of E/2 elements, [l E/2, (l+1) E/2) in each half of
+    // the row, so that every 16-byte load or store instruction of the lane
+    // group covers one half row contiguously (as many 64-byte L2 requests as
+    // the half row needs, not twice as many).  Element i of the lane is
+    // piece i / (E/2), element i % (E/2) of it.
+    constexpr int SPLIT = (int)(kRow / 2);
+    const uint32_t lane_off = (uint32_t)(ls * (TB == 32 ? 16 : TB));
     // the lane's part of row u of P / v of Q
     auto prow = [&](int u) -> gchar* {
         if constexpr (OFF32) return Pl + ((uint32_t)u * (uint32_t)kRow + lane_off);
@@ -171,11 +178,11 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         pd.u = su[at];
         pd.v = sv[at];
         pd.r = sr[at];
-        if constexpr (PT) pd.p = load_lds<TB>(prow_tile(pd.u));
-        else if constexpr (BUF) pd.p = load_buf<TB>(Pr, pd.u != prev_u ? boff(pd.u) : kBufSkip);
-        else if (pd.u != prev_u) pd.p = load_raw<TB, COH>(prow(pd.u));
-        if constexpr (BUF) pd.q = load_buf<TB>(Qr, pd.v != prev_v ? boff(pd.v) : kBufSkip);
-        else if (pd.v != prev_v) pd.q = load_raw<TB, COH>(qrow(pd.v));
+        if constexpr (PT) pd.p = load_lds<TB, SPLIT>(prow_tile(pd.u));
+        else if constexpr (BUF) pd.p = load_buf<TB, SPLIT>(Pr, pd.u != prev_u ? boff(pd.u) : kBufSkip);
+        else if (pd.u != prev_u) pd.p = load_raw<TB, COH, SPLIT>(prow(pd.u));
+        if constexpr (BUF) pd.q = load_buf<TB, SPLIT>(Qr, pd.v != prev_v ? boff(pd.v) : kBufSkip);
+        else if (pd.v != prev_v) pd.q = load_raw<TB, COH, SPLIT>(qrow(pd.v));
         if constexpr (FP8) {
             pd.ps = Ps[pd.u];
             pd.qs = Qs[pd.v];
@@ -411,11 +418,11 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 const bool has_next = decltype(more)::value || d + 1 < D;
                 const Pending<NW>& nx = pend[(d + 1) % D];
                 const bool keep_p = has_next && nx.u == uc, keep_q = has_next && nx.v == vc;
-                if constexpr (PT) store_lds<TB>(prow_tile(uc), po);
-                else if constexpr (BUF) store_buf<TB>(Pr, keep_p ? kBufSkip : boff(uc), po);
-                else if (!keep_p) store_raw<TB, COH>(prow(uc), po);
-                if constexpr (BUF) store_buf<TB>(Qr, keep_q ? kBufSkip : boff(vc), qo);
-                else if (!keep_q) store_raw<TB, COH>(qrow(vc), qo);
+                if constexpr (PT) store_lds<TB, SPLIT>(prow_tile(uc), po);
+                else if constexpr (BUF) store_buf<TB, SPLIT>(Pr, keep_p ? kBufSkip : boff(uc), po);
+                else if (!keep_p) store_raw<TB, COH, SPLIT>(prow(uc), po);
+                if constexpr (BUF) store_buf<TB, SPLIT>(Qr, keep_q ? kBufSkip : boff(vc), qo);
+                else if (!keep_q) store_raw<TB, COH, SPLIT>(qrow(vc), qo);
                 wp[d] = po;
                 wq[d] = qo;
                 wu[d] = uc;

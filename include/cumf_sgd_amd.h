@@ -181,17 +181,17 @@ int cumf_order_blocks(cumf_ctx* ctx, int32_t by);
  * column (stable).  A tile is a range of consecutive rows of one row band
  * (at most tile_rows rows, cut where the band's ratings would otherwise be
  * unevenly spread over its tiles); tile_rows = 0 takes cumf_tile_rows(k,
- * precision) for the factors the context holds, or k = 128 before they are
- * initialised.  No counterpart in the reference (its blocks are
+ * precision) for the factors the context holds (FP16 at k = 128 before
+ * they are initialised).  No counterpart in the reference (its blocks are
  * trained by all workers from memory, singleGPU/sgd_kernel.cu:249-372).  A
  * tile's P rows fit one workgroup's LDS, so the tile trains without sending
  * its P rows through the L2.  Up to 2^32 - 1 ratings; not for streamed
  * ratings.  offsets (s*s + 1 entries, may be NULL) receives the block
  * bounds; cumf_get_tiles returns the tiles. */
 int cumf_partition_tiles(cumf_ctx* ctx, int32_t s, int32_t tile_rows, int64_t* offsets);
-/* Default rows per tile for factors of length k in a format: 128 * 128 / k,
- * at most 64 KB of rows (measured fastest on a Netflix-shaped problem for
- * FP32 and FP16). */
+/* Default rows per tile for factors of length k in a format: 128 * 128 / k
+ * for FP32, 96 * 128 / k for the narrower formats, at most 64 KB of rows
+ * (measured fastest on a Netflix-shaped problem). */
 int32_t cumf_tile_rows(int32_t k, int32_t precision);
 /* The tiles of cumf_partition_tiles, block by block (block-major, then tile
  * order within the block): *count receives their number; when the arrays
@@ -261,9 +261,10 @@ int cumf_set_model(cumf_ctx* ctx, int32_t m, int32_t n, int32_t k, const float* 
  *    by column (cumf_order_blocks, CUMF_ORDER_COLUMNS), or by row when P
  *    takes more than 1 GB (its rows then come from HBM), when there are at
  *    most 2^32 - 1 ratings;
- *  - except, for FP32 factors where that would be strata ordered by column:
- *    CUMF_SCHED_TILE with two passes over the diagonals per epoch, on the
- *    partition of cumf_partition_tiles(ctx, 0, 0, ...) (*order 0).
+ *  - except, for the round-to-nearest formats (FP32, FP16, BF16) where that
+ *    would be strata ordered by column: CUMF_SCHED_TILE on the partition of
+ *    cumf_partition_tiles(ctx, 0, 0, ...) (*order 0), with two passes over
+ *    the diagonals per epoch for FP32, one for FP16 and BF16.
  * *order receives CUMF_ORDER_COLUMNS, CUMF_ORDER_ROWS or 0 (keep the
  * records' order).
  * It replaces the reference's fixed choice of Hogwild for a 1x1 grid

@@ -61,6 +61,8 @@ _sig = {
     "orc_grid_problem": ([P, I64, I32, I32, ctypes.c_int, ctypes.c_int, P, P], None),
     "orc_scale_problem": ([P, I64, F, ctypes.c_int, ctypes.c_int, I64, I64], None),
     "orc_tree_dot": ([P, P, ctypes.c_int, ctypes.c_int], F),
+    "orc_tree_dot_split": ([P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int], F),
+    "orc_lane_elem": ([ctypes.c_int] * 5, ctypes.c_int),
     "orc_sgd_serial": ([ctypes.POINTER(SgdCfg), P, P, P, I64, P, P, P, P, P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "orc_sgd_block_windows": ([ctypes.POINTER(SgdCfg), U32, P, P, P, I64, P, P, P, P, P, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int], ctypes.c_int),
@@ -178,6 +180,18 @@ def scale_problem(nodes, scale, ux, vy, u_seg, v_seg):
     nodes = np.ascontiguousarray(nodes, NODE_DTYPE).copy()
     _lib.orc_scale_problem(_p(nodes), len(nodes), scale, ux, vy, u_seg, v_seg)
     return nodes
+
+
+def tree_dot_split(p, q, G, half):
+    """The dot product with lanes of two pieces of `half` elements, one in
+    each half of the row (32-byte lanes)."""
+    p = np.ascontiguousarray(p, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return _lib.orc_tree_dot_split(_p(p), _p(q), len(p), G, half)
+
+
+def lane_elem(k, G, half, lane, i):
+    return _lib.orc_lane_elem(k, G, half, lane, i)
 
 
 def tree_dot(p, q, G):

@@ -487,7 +487,8 @@ EXPORT void orc_scale_problem(orc_node* R, int64_t nnz, float scale, int ux, int
 /* :85-99: e = r - p.q; p += lr(e q - lp p); q += lr(e p_old - lq q).        */
 /* Arithmetic order (this project's kernel contract):                        */
 /*  - the k elements are split over G lanes, lane l owning the E = k/G       */
-/*    consecutive elements [lE, lE+E);                                       */
+/*    consecutive elements [lE, lE+E) (when E elements take 32 bytes: the    */
+/*    E/2 elements [l E/2, (l+1) E/2) of each half of the row, lane_elem);   */
 /*  - lane partial: a = p0*q0, then a = fma(p_j, q_j, a) for j = 1..E-1;     */
 /*  - the G partials are summed by a pairwise butterfly (lane l adds lane    */
 /*    l^1, then l^2, l^4, ...), i.e. a balanced tree in lane order;          */
@@ -496,14 +497,34 @@ EXPORT void orc_scale_problem(orc_node* R, int64_t nnz, float scale, int ux, int
 /*    p' = fma(le, q, p*cp), q' = fma(le, p, q*cq) (the same step: p (1 -    */
 /*    lr lp) + lr e q).                                                      */
 /* ------------------------------------------------------------------------ */
-static float tree_dot(const float* p, const float* q, int k, int G, float* lanes)
+/* Element i of lane l.  A lane holding at most 16 bytes of a row owns
+ * [lE, lE+E); a lane holding 32 bytes (two 16-byte pieces, half = E/2
+ * elements each) owns [l*half, (l+1)*half) of each half of the row, piece
+ * c = i / half, so that each of the kernel's two 16-byte accesses covers one
+ * half row contiguously.  half = 0 selects the first layout. */
+static inline int lane_elem(int k, int G, int half, int l, int i)
+{
+    if (!half) return l * (k / G) + i;
+    return (i / half) * (k / 2) + l * half + i % half;
+}
+
+/* half for factors of length k in a storage format with G lanes. */
+static inline int lane_half(int k, int G, int prec)
+{
+    int E = k / G;
+    return E * (int)prec_bytes(prec) == 32 ? E / 2 : 0;
+}
+
+static float tree_dot_layout(const float* p, const float* q, int k, int G, int half, float* lanes)
 {
     int E = k / G;
     for (int l = 0; l < G; ++l) {
-        const float* pp = p + l * E;
-        const float* qq = q + l * E;
-        float a = pp[0] * qq[0];
-        for (int j = 1; j < E; ++j) a = fmaf(pp[j], qq[j], a);
+        int e0 = lane_elem(k, G, half, l, 0);
+        float a = p[e0] * q[e0];
+        for (int j = 1; j < E; ++j) {
+            int e = lane_elem(k, G, half, l, j);
+            a = fmaf(p[e], q[e], a);
+        }
         lanes[l] = a;
     }
     for (int d = 1; d < G; d <<= 1)
@@ -511,11 +532,27 @@ static float tree_dot(const float* p, const float* q, int k, int G, float* lanes
     return lanes[0];
 }
 
+static float tree_dot(const float* p, const float* q, int k, int G, float* lanes)
+{
+    return tree_dot_layout(p, q, k, G, 0, lanes);
+}
+
 EXPORT float orc_tree_dot(const float* p, const float* q, int k, int G)
 {
     float lanes[256];
     return tree_dot(p, q, k, G, lanes);
 }
+
+/* The dot product in the two-piece layout of 32-byte lanes (half elements
+ * per piece). */
+EXPORT float orc_tree_dot_split(const float* p, const float* q, int k, int G, int half)
+{
+    float lanes[256];
+    return tree_dot_layout(p, q, k, G, half, lanes);
+}
+
+/* Lane-local index of global element e in that layout (for the pins). */
+EXPORT int orc_lane_elem(int k, int G, int half, int l, int i) { return lane_elem(k, G, half, l, i); }
 
 typedef struct {
     int k;
@@ -555,10 +592,16 @@ static inline uint32_t sr_next(uint32_t x)
     return x + 0x6D2B79F5u;
 }
 
-/* The 16 random bits of element e (0..k-1) of row `which` (0 P, 1 Q). */
-static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e, int k, int G)
+/* The 16 random bits of element e (0..k-1) of row `which` (0 P, 1 Q); the
+ * lane and its element index i follow the layout (half, as lane_elem). */
+static inline uint32_t sr_bits(const uint32_t K[4], uint32_t j, int which, int e, int k, int G, int half)
 {
     int E = k / G, lane = e / E, i = e % E;
+    if (half) {
+        int r = e % (k / 2);
+        lane = r / half;
+        i = (e / (k / 2)) * half + r % half;
+    }
     uint32_t w = lowbias32(K[0] ^ (j * 0x9E3779B9u + (uint32_t)lane * 0x85EBCA77u));
     for (int s = 0; s < (i >> 1); ++s) w = sr_next(w);
     return ((i & 1) ^ (which != 0)) ? (w >> 16) : (w & 0xffffu);
@@ -585,7 +628,8 @@ static void sgd_update_one_e4m3(const orc_sgd_cfg* c, void* P, void* Q, float sp
         p[j] = orc_e4m3_to_f32(P8[bp + j]);
         q[j] = orc_e4m3_to_f32(Q8[bq + j]);
     }
-    float dot = tree_dot(p, q, k, c->G, lanes) * (sp * sq);
+    const int half = lane_half(k, c->G, c->prec);
+    float dot = tree_dot_layout(p, q, k, c->G, half, lanes) * (sp * sq);
     float e = r - dot;
     float cp = fmaf(-lr, c->lambda_p, 1.0f), cq = fmaf(-lr, c->lambda_q, 1.0f);
     float le = lr * e;
@@ -594,8 +638,8 @@ static void sgd_update_one_e4m3(const orc_sgd_cfg* c, void* P, void* Q, float sp
     for (int j = 0; j < k; ++j) {
         float pn = fmaf(lqp, q[j], p[j] * cp);
         float qn = fmaf(lpq, p[j], q[j] * cq);
-        P8[bp + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(pn, sr_bits(K, sj, 0, j, k, c->G));
-        Q8[bq + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(qn, sr_bits(K, sj, 1, j, k, c->G));
+        P8[bp + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(pn, sr_bits(K, sj, 0, j, k, c->G, half));
+        Q8[bq + j] = nan_rows ? 0xffu : orc_f32_to_e4m3_sr(qn, sr_bits(K, sj, 1, j, k, c->G, half));
     }
 }
 
@@ -615,7 +659,8 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
         p[j] = load_elem(P, c->prec, bp + j) * sp;
         q[j] = load_elem(Q, c->prec, bq + j) * sq;
     }
-    float dot = tree_dot(p, q, k, c->G, lanes);
+    const int half = lane_half(k, c->G, c->prec);
+    float dot = tree_dot_layout(p, q, k, c->G, half, lanes);
     float e = r - dot;
     /* p + lr (e q - lp p), evaluated as p (1 - lr lp) + (lr e) q */
     float cp = fmaf(-lr, c->lambda_p, 1.0f), cq = fmaf(-lr, c->lambda_q, 1.0f);
@@ -625,8 +670,8 @@ static void sgd_update_one(const orc_sgd_cfg* c, void* P, void* Q, float* ps, fl
         float qn = fmaf(le, p[j], q[j] * cq);
         uint32_t rp = 0, rq = 0;
         if (c->prec == PREC_E4M3 || c->prec == PREC_BF16_SR) {
-            rp = sr_bits(K, sj, 0, j, k, c->G);
-            rq = sr_bits(K, sj, 1, j, k, c->G);
+            rp = sr_bits(K, sj, 0, j, k, c->G, half);
+            rq = sr_bits(K, sj, 1, j, k, c->G, half);
         }
         store_elem(P, c->prec, bp + j, pn, ip, sp, rp);
         store_elem(Q, c->prec, bq + j, qn, iq, sq, rq);

@@ -6,8 +6,8 @@ from types import SimpleNamespace
 import bench
 
 
-def _info(lanes=16, depth=2, batch_len=32):
-    return SimpleNamespace(lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len)
+def _info(lanes=16, depth=2, batch_len=32, workers=516096):
+    return SimpleNamespace(lanes_per_sample=lanes, pipeline_depth=depth, batch_len=batch_len, num_workers=workers)
 
 
 def test_bytes_per_update_at_k128():
@@ -34,10 +34,11 @@ def test_traffic_reported_only_for_the_profiled_shape():
 
 
 def test_roofline_fractions_stay_below_one():
-    """Both levels of the bench's roofline at the profiled shape, from the
+    """Every level of the bench's roofline at the profiled shape, from the
     committed summary's own timing: the algorithmic bytes against the
-    aggregate L2 bandwidth and the measured HBM-side bytes against the HBM
-    peak are fractions of at most one, and the bound is the larger one."""
+    aggregate L2 bandwidth, the measured HBM-side bytes against the HBM peak
+    and the measured VALU instructions against the VALU issue peak are
+    fractions of at most one, and the bound is the largest."""
     import json
     import os
 
@@ -45,7 +46,29 @@ def test_roofline_fractions_stay_below_one():
     r = bench.roofline(1, _info(), "stratum24", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
                        d["block_order"])
     assert r["traffic"] == d["traffic_bytes_per_launch"]
-    assert set(r["levels"]) == {"l2", "hbm"}
+    assert set(r["levels"]) == {"l2", "hbm", "valu"}
     assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values())
     assert r["frac"] == max(lv["frac"] for lv in r["levels"].values())
     assert abs(bench.L2_PEAK_GBS - 39321.6) < 1e-6
+    assert abs(bench.VALU_PEAK_TLANE - 39.3216) < 1e-9
+
+
+def test_tile_roofline_uses_measured_l2_requests():
+    """A tile launch's P rows stay in LDS: its L2 level is the profile's
+    measured request bytes, not the algorithmic bytes; the profile is found
+    only for the tile kernel's workgroup size."""
+    import glob
+    import json
+
+    for path in glob.glob(bench.os.path.join(bench.ROOT, "profiles", "sgd_tile_*_medium_tile8.json")):
+        d = json.load(open(path))
+        prec = {"fp32": 0, "fp16": 1}[path.split("sgd_tile_")[1].split("_")[0]]
+        info = _info(lanes=d["lanes_per_sample"], workers=d["workers"])
+        r = bench.roofline(prec, info, "tile8", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
+                           d["block_order"])
+        assert r["kernel"] == "sgd_tile_kernel" and r["profile"] == d["source"]
+        assert abs(r["levels"]["l2"]["achieved"] - d["l2_request_bytes_per_launch"] /
+                   (d["trace_avg_launch_ms_timed_region"] / 1e3) / 1e9) < 1e-6
+        assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values())
+        assert bench.profile_summary(prec, _info(lanes=d["lanes_per_sample"], workers=4 * d["workers"]), "tile8",
+                                     d["updates_per_launch"], d["block_order"]) is None

@@ -169,15 +169,15 @@ def test_tall_skinny_3_1_billion_ratings(gpu):
 
 
 @pytest.mark.parametrize("prec,schedule", [(0, "hogwild"), (1, "hogwild"), (0, "default"), (1, "default"),
-                                           (1, "tile")])
+                                           (4, "default"), (1, "strata")])
 def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
     """Netflix-shaped 99M-rating workload, 20 epochs, under Hogwild (the
     reference's 1x1 mode, default launch shape), under the library's default
-    for it (cumf_default_schedule: for FP16 24 x 24 strata, one pass, each
-    block's records ordered by item; for FP32 row tiles of 8 x 8 strata with
-    the P rows in LDS, two passes) and under FP16 row tiles in one pass, with
-    the benchmark's 32-record windows and graph replay: the test RMSE is at
-    most 0.005 above the serialized FP32 CPU oracle's at 20 epochs
+    for it (cumf_default_schedule: row tiles of 8 x 8 strata with the P rows
+    in LDS, two passes for FP32 and one for FP16; 24 x 24 strata with each
+    block's records ordered by item for BF16-SR) and under FP16 24 x 24
+    strata, with the benchmark's 32-record windows and graph replay: the test
+    RMSE is at most 0.005 above the serialized FP32 CPU oracle's at 20 epochs
     (tests/golden/medium_oracle_rmse.json, produced on exactly this data and
     preprocessing)."""
     import json
@@ -197,12 +197,13 @@ def test_medium_reaches_serial_oracle_rmse(gpu, prec, schedule):
         scale, b, lam = bench.prepare(c, R, 2)
         if schedule == "hogwild":
             times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.HOGWILD)
-        elif schedule == "tile":
-            c.init_factors(128, prec, 2)
-            c.partition_tiles(8)
-            times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.TILE, batch_len=32, graph=1)
+        elif schedule == "strata":
+            c.partition_strata(24)
+            c.order_blocks(cs.ORDER_COLUMNS)
+            times, rmses = bench.run_epochs(c, prec, 20, scale, b, lam, 2, 0, 0, cs.STRATUM, batch_len=32, graph=1)
         else:
-            want = (cs.TILE, 2, 0) if prec == cs.FP32 else (cs.STRATUM, 1, cs.ORDER_COLUMNS)
+            want = ((cs.STRATUM, 1, cs.ORDER_COLUMNS) if prec == cs.BF16_SR
+                    else (cs.TILE, 2 if prec == cs.FP32 else 1, 0))
             assert c.default_schedule(128, prec) == want
             c.init_factors(128, prec, 2)
             sched, rounds = c.prepare_default(128, prec)

@@ -56,7 +56,7 @@ def test_tile_partition_is_strata_then_stable_tile_column_sort(gpu, s, tile_rows
         toff, row0, rows = c.get_tiles()
     assert np.array_equal(off, off_s)
     assert toff[0] == 0 and toff[-1] == len(u) and np.all(np.diff(toff) > 0)
-    cap = tile_rows or 128   # cumf_tile_rows(128, ...) before factors exist
+    cap = tile_rows or 96   # cumf_tile_rows(128, FP16) before factors exist
     assert rows.max() <= cap and rows.min() >= 1
     # every block is a whole number of tiles
     assert set(off.tolist()) <= set(toff.tolist())

@@ -271,6 +271,33 @@ def test_tree_dot_is_exact_on_integers_and_order_sensitive_otherwise():
         assert abs(oracle.tree_dot(p, q, G) - exact) < 1e-5
 
 
+@pytest.mark.parametrize("k,G", [(128, 8), (128, 16), (64, 8), (32, 4)])
+def test_split_lane_layout_covers_half_rows_and_reorders_the_dot(k, G):
+    """32-byte lanes: lane l holds elements [l h, (l+1) h) of each half of the
+    row (h = k / G / 2).  The layout is a bijection, each piece index covers
+    one half row contiguously across the lanes, and the dot product in it
+    equals the contiguous-layout dot product of the rows permuted into that
+    order (the permutation written out here from the definition)."""
+    E = k // G
+    h = E // 2
+    got = np.array([[oracle.lane_elem(k, G, h, l, i) for i in range(E)] for l in range(G)])
+    want = np.array([[(i // h) * (k // 2) + l * h + i % h for i in range(E)] for l in range(G)])
+    assert np.array_equal(got, want)
+    assert sorted(got.ravel().tolist()) == list(range(k))
+    for c in range(2):
+        piece = got[:, c * h:(c + 1) * h]
+        assert np.array_equal(np.sort(piece.ravel()), np.arange(c * k // 2, (c + 1) * k // 2))
+        assert np.all(np.diff(piece, axis=1) == 1)
+    rng = np.random.default_rng(k + G)
+    p = rng.normal(size=k).astype(np.float32)
+    q = rng.normal(size=k).astype(np.float32)
+    perm = want.ravel()
+    assert oracle.tree_dot_split(p, q, G, h) == oracle.tree_dot(p[perm], q[perm], G)
+    pi = rng.integers(-8, 8, k).astype(np.float32)
+    qi = rng.integers(-8, 8, k).astype(np.float32)
+    assert oracle.tree_dot_split(pi, qi, G, h) == float(np.dot(pi.astype(np.float64), qi.astype(np.float64)))
+
+
 def test_single_update_known_answer_and_gradient():
     """k=2 hand-computed update, then a finite-difference gradient check of the
     loss 1/2 e^2 + 1/2 lp|p|^2 + 1/2 lq|q|^2."""
