@@ -395,6 +395,7 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
         t.wave_count = (int32_t)waves;
         t.worker_count = (int32_t)workers;
         t.update_count = (int32_t)((p.nnz + workers * L - 1) / (workers * L));
+        t.batch_len = L;
         t.problem = (uint32_t)i;
         t.tag = 0;
         t.k = p.k;

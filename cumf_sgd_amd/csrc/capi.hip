@@ -1354,9 +1354,47 @@ SgdTask base_task(cumf_ctx* c, const cumf_train_opts* o, int64_t beg, int64_t en
     t.worker_count = W;
     // update_count = ceil(size / (W * L)) (singleGPU/sgd_kernel.cu:203)
     t.update_count = (int32_t)((t.nnz + (int64_t)W * L - 1) / ((int64_t)W * L));
+    t.batch_len = L;
     t.problem = 0;
     t.tag = tag;
     return t;
+}
+
+// Window plan of one tile of n records: window length len (a multiple of D,
+// at most Lmax) and windows = ceil(n / len) windows per epoch, pass r of R
+// running windows [windows r / R, windows (r + 1) / R) on w_r workers of u_r
+// windows each (u_r the fewest windows per worker that W workers allow, w_r
+// the fewest workers that keep them).  The tile's workgroup issues whole
+// waves of S slots, so a tile of, say, 26 workers of 8-slot waves would run
+// 6 idle slots; len is the length whose passes issue the fewest slot-steps,
+// a window costing kTileWindowSteps steps beyond its records, ties going to
+// fewer samples, then to the longer window.
+struct TileWindows {
+    int len;
+    int64_t windows;
+};
+
+TileWindows tile_windows(int64_t n, int W, int S, int Lmax, int D, int R)
+{
+    TileWindows best{Lmax, (n + Lmax - 1) / Lmax};
+    int64_t best_cost = -1, best_samples = 0;
+    for (int L = Lmax; L >= D; L -= D) {
+        const int64_t N = (n + L - 1) / L;
+        int64_t cost = 0, samples = 0;
+        for (int r = 0; r < R; ++r) {
+            const int64_t nr = N * (r + 1) / R - N * r / R;
+            if (nr == 0) continue;
+            const int64_t u = (nr + W - 1) / W, w = (nr + u - 1) / u;
+            cost += (w + S - 1) / S * S * u * (L + kTileWindowSteps);
+            samples += w * u * L;
+        }
+        if (best_cost < 0 || cost < best_cost || (cost == best_cost && samples < best_samples)) {
+            best = TileWindows{L, N};
+            best_cost = cost;
+            best_samples = samples;
+        }
+    }
+    return best;
 }
 
 // Waves of a tile workgroup holding `workers` workers of S per wave: 4, 8
@@ -1383,7 +1421,7 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
     auto push = [&](std::vector<SgdTask>& t, int nx, int32_t blocks = 0) {
         if (t.empty()) return;
         launches.push_back(LaunchRec{(int64_t)tasks.size(), (int32_t)t.size(), nx, blocks});
-        for (const SgdTask& x : t) *upd_epoch += (int64_t)x.worker_count * x.update_count * sh.L;
+        for (const SgdTask& x : t) *upd_epoch += (int64_t)x.worker_count * x.update_count * x.batch_len;
         tasks.insert(tasks.end(), t.begin(), t.end());
     };
     if (o->schedule == CUMF_SCHED_HOGWILD || o->schedule == CUMF_SCHED_SERIAL) {
@@ -1404,13 +1442,19 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
         // through its L2); the blocks of a diagonal go to XCDs largest first
         // onto the least-loaded one, and each XCD's tiles run largest first
         // (workgroups are dispatched to an XCD in order).  A tile gets one
-        // workgroup of per_tile workers, fewer when it has fewer windows.
+        // workgroup of at most per_tile workers and its own window length
+        // (tile_windows).
         const int s = c->ux;
         const int R = std::max(1, o->stratum_rounds);
         const int per_tile = sh.W;   // workers per tile (cumf_train resolved it)
         auto block_size = [&](int blk) { return c->offsets[(size_t)blk + 1] - c->offsets[(size_t)blk]; };
+        struct TileEntry {
+            SgdTask t;
+            int64_t windows;   // the tile's windows per epoch
+        };
         // diag[d][x]: the tiles XCD x trains in sub-epoch d, in launch order
-        std::vector<std::vector<std::vector<SgdTask>>> diag((size_t)s, std::vector<std::vector<SgdTask>>((size_t)num_xcd));
+        std::vector<std::vector<std::vector<TileEntry>>> diag((size_t)s,
+                                                              std::vector<std::vector<TileEntry>>((size_t)num_xcd));
         for (int d = 0; d < s; ++d) {
             std::vector<int> order((size_t)s);
             for (int i = 0; i < s; ++i) order[(size_t)i] = i * s + (i + d) % s;
@@ -1421,19 +1465,16 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                 const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
                 load[(size_t)x] += block_size(blk);
                 for (int32_t t = c->block_tile[(size_t)blk]; t < c->block_tile[(size_t)blk + 1]; ++t) {
-                    // the fewest windows per worker the tile's workers allow,
-                    // then the fewest workers that keep them (least over-sampling)
                     const int64_t beg = c->tile_rec[(size_t)t], end = c->tile_rec[(size_t)t + 1];
-                    const int64_t uc = (end - beg + (int64_t)per_tile * sh.L - 1) / ((int64_t)per_tile * sh.L);
-                    const int w = (int)((end - beg + uc * sh.L - 1) / (uc * sh.L));
-                    SgdTask tk = base_task(c, o, beg, end, w, sh.L, (uint32_t)t);
+                    const TileWindows tw = tile_windows(end - beg, per_tile, sh.S, sh.L, sh.D, R);
+                    SgdTask tk = base_task(c, o, beg, end, 1, tw.len, (uint32_t)t);
                     tk.row_begin = c->tile_row0[(size_t)t];
                     tk.row_count = c->tile_rows[(size_t)t];
-                    diag[(size_t)d][(size_t)x].push_back(tk);
+                    diag[(size_t)d][(size_t)x].push_back(TileEntry{tk, tw.windows});
                 }
             }
             for (auto& l : diag[(size_t)d])
-                std::stable_sort(l.begin(), l.end(), [](const SgdTask& a, const SgdTask& b) { return a.nnz > b.nnz; });
+                std::stable_sort(l.begin(), l.end(), [](const TileEntry& a, const TileEntry& b) { return a.t.nnz > b.t.nnz; });
         }
         // A launch holds up to F tiles per XCD (F = tiles_in_flight, 0: all):
         // task x * per + j is XCD x's j-th tile, the rest padding (nnz 0).
@@ -1453,7 +1494,10 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                 push(t, num_xcd, (int32_t)(num_xcd * per));
             }
         };
-        // pass rr runs N*(rr+1)/R - N*rr/R of a tile's N windows, as for strata
+        // pass rr runs windows [N*rr/R, N*(rr+1)/R) of a tile's N: the fewest
+        // windows per worker that per_tile workers allow, then the fewest
+        // workers that keep them; a worker's window indices continue from
+        // pass to pass
         std::vector<std::vector<std::vector<int32_t>>> next_window((size_t)s);
         for (int d = 0; d < s; ++d) {
             next_window[(size_t)d].resize((size_t)num_xcd);
@@ -1464,17 +1508,14 @@ int plan_epoch(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, int num_x
                 std::vector<std::vector<SgdTask>> lists((size_t)num_xcd);
                 for (int x = 0; x < num_xcd; ++x)
                     for (size_t j = 0; j < diag[(size_t)d][(size_t)x].size(); ++j) {
-                        SgdTask q = diag[(size_t)d][(size_t)x][j];
-                        if (R > 1) {
-                            const int64_t n_all = (int64_t)q.worker_count * q.update_count;
-                            const int64_t n_pass = n_all * (rr + 1) / R - n_all * rr / R;
-                            if (n_pass == 0) continue;
-                            const int32_t wc = q.worker_count;
-                            q.update_count = (int32_t)((n_pass + wc - 1) / wc);
-                            q.worker_count = (int32_t)((n_pass + q.update_count - 1) / q.update_count);
-                            q.window_begin = next_window[(size_t)d][(size_t)x][j];
-                            next_window[(size_t)d][(size_t)x][j] += q.update_count;
-                        }
+                        const TileEntry& te = diag[(size_t)d][(size_t)x][j];
+                        const int64_t n_pass = te.windows * (rr + 1) / R - te.windows * rr / R;
+                        if (n_pass == 0) continue;
+                        SgdTask q = te.t;
+                        q.update_count = (int32_t)((n_pass + per_tile - 1) / per_tile);
+                        q.worker_count = (int32_t)((n_pass + q.update_count - 1) / q.update_count);
+                        q.window_begin = next_window[(size_t)d][(size_t)x][j];
+                        next_window[(size_t)d][(size_t)x][j] += q.update_count;
                         lists[(size_t)x].push_back(q);
                     }
                 emit(lists);
@@ -1698,7 +1739,7 @@ int train_streamed(cumf_ctx* c, const cumf_train_opts* o, const Shape& sh, const
             const LaunchRec& l = ls[li];
             for (int32_t t = 0; t < l.task_cnt; ++t) {
                 const SgdTask& x = tasks[(size_t)(l.task_off + t)];
-                c->updates += (int64_t)x.worker_count * x.update_count * sh.L;
+                c->updates += (int64_t)x.worker_count * x.update_count * x.batch_len;
             }
         }
         if ((rc = queue_epoch_tail(c, nullptr))) return rc;
@@ -1912,7 +1953,7 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
             for (const LaunchRec& l : ls)
                 for (int32_t t = 0; t < l.task_cnt; ++t) {
                     const SgdTask& x = tasks[(size_t)(l.task_off + t)];
-                    c->updates += (int64_t)x.worker_count * x.update_count * sh.L;
+                    c->updates += (int64_t)x.worker_count * x.update_count * x.batch_len;
                 }
             c->launches += (int64_t)ls.size();
         }

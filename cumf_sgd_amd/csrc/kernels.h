@@ -31,6 +31,8 @@ struct SgdTask {
                               // are window_begin + [0, update_count))
     int32_t row_begin;        // tile launches: the task's P rows [row_begin, row_begin + row_count),
     int32_t row_count;        // held in the workgroup's LDS for the launch
+    int32_t batch_len;        // records per window; tile launches read it per task (<= the
+                              // launch's batch_len), other launches use the launch's
 };
 
 struct SgdLaunch {

@@ -117,6 +117,7 @@ int train_noalloc(int device, hipStream_t st, const cumf_train_opts* o, int32_t 
     t.worker_count = sh.W;
     // update_count = ceil(size / (W * L)) (singleGPU/sgd_kernel.cu:203)
     t.update_count = (int32_t)((nnz + (int64_t)sh.W * sh.L - 1) / ((int64_t)sh.W * sh.L));
+    t.batch_len = sh.L;
     static_assert(sizeof(SgdTask) % 4 == 0, "the task record is staged as words");
     if ((e = put_words((uint32_t*)base, &t, sizeof(SgdTask) / 4, st)) != hipSuccess ||
         (e = put_words((uint32_t*)(base + kTaskBytes), lr, (size_t)nlr, st)) != hipSuccess)

@@ -69,6 +69,7 @@ struct Pending {
     Words<NW> p, q;
     int u, v;
     float r, ps, qs;
+    uint32_t pa, qa;   // the lane's offset of row u in the tile (LDS) and of row v in Q (buffer)
 };
 
 // One worker (a slot of G lanes) running its windows of task T.  OFF32: the
@@ -98,7 +99,9 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
 
     if (T.nnz <= 0 || T.update_count <= 0) return;
     const Key key = make_key(a.seed, T.problem);
-    const int L = a.batch_len;   // host guarantees 1 <= L <= kMaxBatch and L % D == 0
+    // window length: a tile's own, else the launch's (host guarantees
+    // 1 <= L <= kMaxBatch, L % D == 0, and a tile's L at most the launch's)
+    const int L = PT ? T.batch_len : a.batch_len;
 
     gchar* Pb = to_global((char*)T.P) + (int64_t)ls * (TB == 32 ? 16 : TB);
     gchar* Qb = to_global((char*)T.Q) + (int64_t)ls * (TB == 32 ? 16 : TB);
@@ -129,7 +132,6 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         else return Qb + (int64_t)v * kRow;
     };
     const int row0 = T.row_begin;
-    auto prow_tile = [&](int u) -> lchar* { return Pt + ((uint32_t)(u - row0) * (uint32_t)kRow + lane_off); };
     // Factors under 4 GB (OFF32): rows go through buffer resources, and a
     // load or store that is skipped (a row kept in registers) is issued with
     // an out-of-range offset instead of being branched around, so that the
@@ -178,11 +180,23 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         pd.u = su[at];
         pd.v = sv[at];
         pd.r = sr[at];
-        if constexpr (PT) pd.p = load_lds<TB, SPLIT>(prow_tile(pd.u));
-        else if constexpr (BUF) pd.p = load_buf<TB, SPLIT>(Pr, pd.u != prev_u ? boff(pd.u) : kBufSkip);
-        else if (pd.u != prev_u) pd.p = load_raw<TB, COH, SPLIT>(prow(pd.u));
-        if constexpr (BUF) pd.q = load_buf<TB, SPLIT>(Qr, pd.v != prev_v ? boff(pd.v) : kBufSkip);
-        else if (pd.v != prev_v) pd.q = load_raw<TB, COH, SPLIT>(qrow(pd.v));
+        // tile launches keep the row offsets for the write-back
+        if constexpr (PT) {
+            pd.pa = (uint32_t)(pd.u - row0) * (uint32_t)kRow + lane_off;
+            pd.p = load_lds<TB, SPLIT>(Pt + pd.pa);
+        } else if constexpr (BUF) {
+            pd.p = load_buf<TB, SPLIT>(Pr, pd.u != prev_u ? boff(pd.u) : kBufSkip);
+        } else if (pd.u != prev_u) {
+            pd.p = load_raw<TB, COH, SPLIT>(prow(pd.u));
+        }
+        if constexpr (PT && BUF) {
+            pd.qa = boff(pd.v);
+            pd.q = load_buf<TB, SPLIT>(Qr, pd.v != prev_v ? pd.qa : kBufSkip);
+        } else if constexpr (BUF) {
+            pd.q = load_buf<TB, SPLIT>(Qr, pd.v != prev_v ? boff(pd.v) : kBufSkip);
+        } else if (pd.v != prev_v) {
+            pd.q = load_raw<TB, COH, SPLIT>(qrow(pd.v));
+        }
         if constexpr (FP8) {
             pd.ps = Ps[pd.u];
             pd.qs = Qs[pd.v];
@@ -244,19 +258,41 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 if constexpr (FP8) { psc = pend[d].ps; qsc = pend[d].qs; }
                 // forward rows written after this sample's rows were requested
                 // (samples j-D+1 .. j-1), oldest first so the newest wins; the
-                // wave skips this when none of its slots has such a row
-                bool hit = false;
-#pragma unroll
-                for (int back = D - 1; back >= 1; --back) {
-                    const int s = (d - back + D) % D;
-                    hit |= (wu[s] == uc) | (wv[s] == vc);
-                }
-                if (__builtin_amdgcn_ballot_w64(hit)) {
+                // wave skips this when none of its slots has such a row.  P
+                // rows in LDS (PT) are read again instead: the wave's LDS
+                // operations run in order, so the read returns every write
+                // before it (records ordered by column repeat P rows seldom:
+                // the read stays a rare branch, and no P ring is kept)
+                if constexpr (PT) {
+                    bool hit_p = false, hit_q = false;
 #pragma unroll
                     for (int back = D - 1; back >= 1; --back) {
                         const int s = (d - back + D) % D;
-                        if (wu[s] == uc) pend[d].p = wp[s];
-                        if (wv[s] == vc) pend[d].q = wq[s];
+                        hit_p |= wu[s] == uc;
+                        hit_q |= wv[s] == vc;
+                    }
+                    if (__builtin_amdgcn_ballot_w64(hit_p)) pend[d].p = load_lds<TB, SPLIT>(Pt + pend[d].pa);
+                    if (__builtin_amdgcn_ballot_w64(hit_q)) {
+#pragma unroll
+                        for (int back = D - 1; back >= 1; --back) {
+                            const int s = (d - back + D) % D;
+                            if (wv[s] == vc) pend[d].q = wq[s];
+                        }
+                    }
+                } else {
+                    bool hit = false;
+#pragma unroll
+                    for (int back = D - 1; back >= 1; --back) {
+                        const int s = (d - back + D) % D;
+                        hit |= (wu[s] == uc) | (wv[s] == vc);
+                    }
+                    if (__builtin_amdgcn_ballot_w64(hit)) {
+#pragma unroll
+                        for (int back = D - 1; back >= 1; --back) {
+                            const int s = (d - back + D) % D;
+                            if (wu[s] == uc) pend[d].p = wp[s];
+                            if (wv[s] == vc) pend[d].q = wq[s];
+                        }
                     }
                 }
                 const Words<NW> pw = pend[d].p, qw = pend[d].q;
@@ -418,12 +454,13 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                 const bool has_next = decltype(more)::value || d + 1 < D;
                 const Pending<NW>& nx = pend[(d + 1) % D];
                 const bool keep_p = has_next && nx.u == uc, keep_q = has_next && nx.v == vc;
-                if constexpr (PT) store_lds<TB, SPLIT>(prow_tile(uc), po);
+                if constexpr (PT) store_lds<TB, SPLIT>(Pt + pend[d].pa, po);
                 else if constexpr (BUF) store_buf<TB, SPLIT>(Pr, keep_p ? kBufSkip : boff(uc), po);
                 else if (!keep_p) store_raw<TB, COH, SPLIT>(prow(uc), po);
-                if constexpr (BUF) store_buf<TB, SPLIT>(Qr, keep_q ? kBufSkip : boff(vc), qo);
+                if constexpr (PT && BUF) store_buf<TB, SPLIT>(Qr, keep_q ? kBufSkip : pend[d].qa, qo);
+                else if constexpr (BUF) store_buf<TB, SPLIT>(Qr, keep_q ? kBufSkip : boff(vc), qo);
                 else if (!keep_q) store_raw<TB, COH, SPLIT>(qrow(vc), qo);
-                wp[d] = po;
+                if constexpr (!PT) wp[d] = po;
                 wq[d] = qo;
                 wu[d] = uc;
                 wv[d] = vc;

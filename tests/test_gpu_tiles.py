@@ -72,9 +72,46 @@ def test_tile_partition_is_strata_then_stable_tile_column_sort(gpu, s, tile_rows
     assert np.array_equal(gu, su[order]) and np.array_equal(gv, sv[order]) and np.array_equal(gr, sr[order])
 
 
-def _oracle_tiles(gu, gv, gr, off, toff, s, P, Q, ps, qs, epochs, rounds, prec, k, G, L, lp, lq, seed):
+def _tile_windows(n, W, S, Lmax, D, R):
+    """A tile's window length and window count, restated from the planner:
+    N = ceil(n / L) windows, pass r running windows [N r / R, N (r+1) / R)
+    on w workers of u windows (u = ceil(n_r / W), w = ceil(n_r / u)); L is
+    the multiple of D up to Lmax whose passes issue the fewest slot-steps
+    (whole waves of S slots, a window 16 steps longer than its records),
+    ties to fewer samples, then to the longer window."""
+    best = None
+    for L in range(Lmax, D - 1, -D):
+        N = -(-n // L)
+        cost = samples = 0
+        for r in range(R):
+            nr = N * (r + 1) // R - N * r // R
+            if nr == 0:
+                continue
+            u = -(-nr // W)
+            w = -(-nr // u)
+            cost += -(-w // S) * S * u * (L + 16)
+            samples += w * u * L
+        if best is None or (cost, samples) < best[:2]:
+            best = (cost, samples, L, N)
+    return best[2], best[3]
+
+
+def test_tile_windows_restatement_fills_waves():
+    """The restated rule on the shapes it is for: a tile of 2388 records on
+    32 workers of 8-slot waves takes 26-record windows (92 windows, 31
+    workers of 3 windows, 32 slots issued) instead of 32-record ones (25
+    workers of 3 windows: 7 idle slots); one worker per tile takes the
+    length with the least over-sampling per window step."""
+    assert _tile_windows(2388, 32, 8, 32, 2, 1) == (26, 92)
+    assert _tile_windows(20, 1, 8, 32, 2, 1) == (20, 1)
+    assert _tile_windows(64, 1, 8, 32, 2, 1) == (32, 2)
+    assert _tile_windows(0, 32, 8, 32, 2, 2) == (32, 0)
+
+
+def _oracle_tiles(gu, gv, gr, off, toff, s, P, Q, ps, qs, epochs, rounds, prec, k, G, Lmax, lp, lq, seed):
     """The deterministic tile epoch restated: passes, diagonals, each block's
-    tiles largest first (stable), a tile's windows of the pass in order."""
+    tiles largest first (stable), a tile's windows of the pass in order, one
+    worker per tile with the tile's own window length."""
     import oracle
 
     nt = len(toff) - 1
@@ -90,7 +127,7 @@ def _oracle_tiles(gu, gv, gr, off, toff, s, P, Q, ps, qs, epochs, rounds, prec, 
                     sizes = toff[ts + 1] - toff[ts]
                     for t in ts[np.argsort(-sizes, kind="stable")]:
                         lo, hi = int(toff[t]), int(toff[t + 1])
-                        n_all = (hi - lo + L - 1) // L   # one worker: one window per L records
+                        L, n_all = _tile_windows(hi - lo, 1, 64 // G, Lmax, 2, rounds)
                         n_pass = n_all * (rr + 1) // rounds - n_all * rr // rounds
                         if n_pass == 0:
                             continue
@@ -179,12 +216,16 @@ def test_tile_full_workgroups_train_every_rating_window(gpu):
         c.train(opts, 0, 5)
         info = c.last_launch()
         toff, _, _ = c.get_tiles()
-        L, W = info.batch_len, info.num_workers
-        sizes = np.diff(toff)
-        uc = (sizes + W * L - 1) // (W * L)     # fewest windows per worker
-        w = (sizes + uc * L - 1) // (uc * L)    # then fewest workers keeping them
+        Lmax, W = info.batch_len, info.num_workers
         assert W == 32 and info.lanes_per_sample == 8   # 32 bytes of an FP16 row per lane
-        assert info.updates_per_epoch == int(np.sum(w * uc * L))
+        planned = 0
+        for size in np.diff(toff):
+            L, N = _tile_windows(int(size), W, 8, Lmax, 2, 1)
+            uc = -(-N // W)        # fewest windows per worker
+            w = -(-N // uc)        # then fewest workers keeping them
+            assert w * uc * L >= size
+            planned += w * uc * L
+        assert info.updates_per_epoch == planned
         r1 = c.test_rmse(1.0, 0.0)
         P, _ = c.get_factors(0)
         assert np.all(np.isfinite(P.view(np.float16).astype(np.float32)))
