@@ -1792,10 +1792,14 @@ extern "C" int cumf_train(cumf_ctx* c, const cumf_train_opts* o, int32_t e0, int
         if (o->num_workers > most)
             return fail(CUMF_ERR_ARG, "tile schedule: num_workers=%d per tile exceeds %d", o->num_workers, most);
         sh.W = o->num_workers > 0 ? o->num_workers : std::min(kTileWorkers, most);   // workers per tile
-        const size_t lds = sgd_tile_lds_bytes(c->k, c->prec, sh.G, sh.L, c->tile_rows_max, tile_waves(sh.W, sh.S));
+        // windows of at most kSgdTileMaxBatch records (the longest by default)
+        if (o->batch_len == 0) sh.L = kSgdTileMaxBatch;
+        if (sh.L > kSgdTileMaxBatch)
+            return fail(CUMF_ERR_ARG, "tile schedule: batch_len=%d exceeds %d", sh.L, kSgdTileMaxBatch);
+        const size_t lds = sgd_tile_lds_bytes(c->k, c->prec, sh.G, c->tile_rows_max, tile_waves(sh.W, sh.S));
         if (lds > sgd_tile_lds_limit())
-            return fail(CUMF_ERR_ARG, "tiles of %d rows at k=%d precision=%d with %d-record windows need %zu bytes of LDS "
-                        "(at most %zu): partition with fewer tile rows", c->tile_rows_max, c->k, c->prec, sh.L, lds,
+            return fail(CUMF_ERR_ARG, "tiles of %d rows at k=%d precision=%d need %zu bytes of LDS with the staged "
+                        "records (at most %zu): partition with fewer tile rows", c->tile_rows_max, c->k, c->prec, lds,
                         sgd_tile_lds_limit());
     }
     if ((rc = upload_lr(c, o, e1 + 4))) return rc;
@@ -2007,6 +2011,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
         groups = {g0};
         if (sgd_shape_supported(c->k, c->prec, g0 / 2)) groups.push_back(g0 / 2);
         depths = {2};
+        lens = {kSgdTileMaxBatch};
         if (!base->num_workers) tile_workers = {4, 8, 16};
     }
     if (base->lanes_per_sample) groups = {base->lanes_per_sample};
@@ -2030,7 +2035,7 @@ extern "C" int cumf_autotune(cumf_ctx* c, const cumf_train_opts* base, cumf_trai
                     if (base->num_workers) o.num_workers = base->num_workers;
                     else if (tiled) o.num_workers = tile_workers[wi] * (64 / G);
                     else o.num_workers = default_workers(shape_input(c, base), 64 / G, L, waves_per_cu[wi]);
-                    if (tiled && sgd_tile_lds_bytes(c->k, c->prec, G, L, c->tile_rows_max,
+                    if (tiled && sgd_tile_lds_bytes(c->k, c->prec, G, c->tile_rows_max,
                                                     tile_waves(o.num_workers, 64 / G)) > sgd_tile_lds_limit())
                         continue;
                     o.use_graph = 0;

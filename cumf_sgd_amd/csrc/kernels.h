@@ -71,8 +71,11 @@ constexpr int kSgdTileMaxWaves = 16;
 // workgroups (16 waves) per CU with the record staging of 32-record windows.
 constexpr int kSgdTileMaxBytes = 128 * 1024;
 constexpr int kSgdTileDefaultBytes = 64 * 1024;
+// Longest window of a tile launch (records): each wave stages this many per
+// worker, in a fixed number of loads.
+constexpr int kSgdTileMaxBatch = 32;
 // LDS bytes of a tile launch: the tile, then each wave's staged records.
-size_t sgd_tile_lds_bytes(int k, int prec, int G, int L, int rows, int waves);
+size_t sgd_tile_lds_bytes(int k, int prec, int G, int rows, int waves);
 size_t sgd_tile_lds_limit();
 hipError_t launch_sgd_tile(const SgdLaunch& a, int blocks, int k, int prec, int G, int D, int rows, int waves,
                            hipStream_t s);

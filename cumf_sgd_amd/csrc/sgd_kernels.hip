@@ -54,10 +54,10 @@ hipError_t launch_sgd(const SgdLaunch& a, int blocks, int k, int prec, int G, bo
     return hipErrorInvalidValue;
 }
 
-size_t sgd_tile_lds_bytes(int k, int prec, int G, int L, int rows, int waves)
+size_t sgd_tile_lds_bytes(int k, int prec, int G, int rows, int waves)
 {
     const size_t eb = prec == kF32 ? 4 : (prec == kE4M3 ? 1 : 2);
-    const size_t stage = 64 * (size_t)((L + G - 1) / G);   // words per array and wave
+    const size_t stage = 64 * (size_t)((kSgdTileMaxBatch + G - 1) / G);   // words per array and wave
     return (size_t)rows * k * eb + (size_t)waves * 3 * stage * 4;
 }
 
@@ -67,15 +67,17 @@ size_t sgd_tile_lds_limit() { return 160 * 1024; }
 hipError_t launch_sgd_tile(const SgdLaunch& a, int blocks, int k, int prec, int G, int D, int rows, int waves,
                            hipStream_t s)
 {
-    const size_t lds = sgd_tile_lds_bytes(k, prec, G, a.batch_len, rows, waves);
-    if (lds > sgd_tile_lds_limit()) return hipErrorInvalidValue;
-    const int stage_words = 64 * ((a.batch_len + G - 1) / G);
+    if (a.batch_len > kSgdTileMaxBatch) return hipErrorInvalidValue;
+    if (sgd_tile_lds_bytes(k, prec, G, rows, waves) > sgd_tile_lds_limit()) return hipErrorInvalidValue;
+    // dynamic LDS: the tile (the kernel's record staging is static)
+    const size_t eb = prec == kF32 ? 4 : (prec == kE4M3 ? 1 : 2);
+    const size_t lds = (size_t)rows * k * eb;
     switch (prec) {
-    case kF32: return launch_tile_prec<kF32>(a, blocks, k, G, D, waves, lds, stage_words, s);
-    case kF16: return launch_tile_prec<kF16>(a, blocks, k, G, D, waves, lds, stage_words, s);
-    case kBF16: return launch_tile_prec<kBF16>(a, blocks, k, G, D, waves, lds, stage_words, s);
-    case kE4M3: return launch_tile_prec<kE4M3>(a, blocks, k, G, D, waves, lds, stage_words, s);
-    case kBF16SR: return launch_tile_prec<kBF16SR>(a, blocks, k, G, D, waves, lds, stage_words, s);
+    case kF32: return launch_tile_prec<kF32>(a, blocks, k, G, D, waves, lds, s);
+    case kF16: return launch_tile_prec<kF16>(a, blocks, k, G, D, waves, lds, s);
+    case kBF16: return launch_tile_prec<kBF16>(a, blocks, k, G, D, waves, lds, s);
+    case kE4M3: return launch_tile_prec<kE4M3>(a, blocks, k, G, D, waves, lds, s);
+    case kBF16SR: return launch_tile_prec<kBF16SR>(a, blocks, k, G, D, waves, lds, s);
     }
     return hipErrorInvalidValue;
 }

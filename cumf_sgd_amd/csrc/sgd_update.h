@@ -147,8 +147,30 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     // no registers): su/sv/sr are the wave's arrays, and load c of lane
     // (slot, ls) fetches record j = ls + G*c of its slot into word 64*c + lane,
     // so record j of a slot sits at (j / G) * 64 + slot * G + j % G.
+    // Tile launches load the records of a window into registers (kTileRows
+    // per array and lane, every lane active: a lane past the window's end
+    // loads the window's first record into a slot that is never read) and
+    // write them to LDS when the window starts.  Ordinary loads, unlike
+    // loads into LDS, let the compiler count the vector memory operations in
+    // flight across them, so the waits for a sample's rows stay partial
+    // while the next window's records load during the last round.
+    constexpr int kTileRows = PT ? (kSgdTileMaxBatch + G - 1) / G : 1;
+    int32_t nu[kTileRows], nv[kTileRows];
+    float nr[kTileRows];
     auto fetch_window = [&](int ep, int b) {
         const int64_t start = window_start(key, T.tag, lw, (uint32_t)ep, (uint32_t)(T.window_begin + b), nnz);
+        if constexpr (PT) {
+#pragma unroll
+            for (int c = 0; c < kTileRows; ++c) {
+                const int j = ls + G * c;
+                int64_t idx = start + (j < L ? j : 0);
+                while (idx >= nnz) idx -= nnz;
+                nu[c] = Ru[idx];
+                nv[c] = Rv[idx];
+                nr[c] = Rr[idx];
+            }
+            return;
+        }
         // reads of the previous window's records have returned before they
         // are overwritten
         __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
@@ -165,7 +187,19 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         }
     };
     auto stage_window = [&]() {
-        __builtin_amdgcn_s_waitcnt(0);   // the window's records are in LDS
+        if constexpr (PT) {
+            // the wave's LDS operations run in order: the previous window's
+            // reads of these words come before the writes
+            const int lane = slot * G + ls;
+#pragma unroll
+            for (int c = 0; c < kTileRows; ++c) {
+                su[64 * c + lane] = nu[c];
+                sv[64 * c + lane] = nv[c];
+                sr[64 * c + lane] = nr[c];
+            }
+        } else {
+            __builtin_amdgcn_s_waitcnt(0);   // the window's records are in LDS
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -210,15 +244,24 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
     // (else each guarded load below waits for them again)
     __builtin_amdgcn_s_waitcnt(0);
     fetch_window(ep0, 0);
+    // tile launches: the first window's records are in registers before the
+    // window loop, so that the wait for a later window's records counts only
+    // what the loop issued after them (the last round's stores stay in flight)
+    if constexpr (PT) __builtin_amdgcn_s_waitcnt(0);
 
+    float lr = 0.0f, cp = 0.0f, cq = 0.0f;
     for (int w = 0; w < nwin; ++w) {
         const int ep = ep0 + w / T.update_count;
         const int b = w % T.update_count;
-        const float lr = LR[ep];
-        // p + lr (e q - lp p) = p (1 - lr lp) + (lr e) q: one multiply and one
-        // fma per element
-        const float cp = __builtin_fmaf(-lr, T.lambda_p, 1.0f);
-        const float cq = __builtin_fmaf(-lr, T.lambda_q, 1.0f);
+        // tile launches read the rate when the epoch changes (no load that
+        // the window's first wait would drain)
+        if (!PT || b == 0) {
+            lr = LR[ep];
+            // p + lr (e q - lp p) = p (1 - lr lp) + (lr e) q: one multiply
+            // and one fma per element
+            cp = __builtin_fmaf(-lr, T.lambda_p, 1.0f);
+            cq = __builtin_fmaf(-lr, T.lambda_q, 1.0f);
+        }
         U4 srk = U4{0u, 0u, 0u, 0u};
         if constexpr (SR)
             srk = philox(U4{(uint32_t)(T.window_begin + b), (uint32_t)ep, lw, kStreamRound | (T.tag << 8)}, key.k0,
@@ -226,7 +269,14 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
         stage_window();
         const bool more_windows = w + 1 < nwin;
         auto fetch_next = [&]() {
-            if (more_windows) fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
+            if constexpr (PT) {
+                // every window issues the same loads: after the last one a
+                // valid window that is never read
+                const int wn = more_windows ? w + 1 : 0;
+                fetch_window(ep0 + wn / T.update_count, wn % T.update_count);
+            } else if (more_windows) {
+                fetch_window(ep0 + (w + 1) / T.update_count, (w + 1) % T.update_count);
+            }
         };
 
         Pending<NW> pend[D];
@@ -514,10 +564,11 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void sgd_update_kernel(SgdLaun
 // the rows go back to memory when every worker is done.  Per update, the L2
 // then sees the Q row (when the previous sample's differs) and the record,
 // instead of a P row read and written as well.
-// Dynamic LDS: the tile (row_count * K * bytes), then per wave the staged
-// records of a window (stage words per array: 64 * ceil(L / G)).
+// Dynamic LDS: the tile (row_count * K * bytes); static: per wave the
+// staged records of a window (64 * ceil(kSgdTileMaxBatch / G) words per
+// array).
 template <int K, int G, int PREC, int D, bool OFF32, int NWAVES>
-__global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a, int stage_words)
+__global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a)
 {
     extern __shared__ __attribute__((aligned(16))) char lds_dyn[];
     constexpr int S = 64 / G;
@@ -533,6 +584,10 @@ __global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a, int 
     const SgdTask T = a.tasks[ti];
     if (T.nnz <= 0 || T.row_count <= 0) return;
 
+    // the staged records: an array of their own, so that the compiler sees
+    // the tile's reads and writes apart from the record loads into LDS
+    constexpr int kStageWords = 64 * ((kSgdTileMaxBatch + G - 1) / G);
+    __shared__ int32_t s_stage[NWAVES][3][kStageWords];
     lchar* tile = (lchar*)lds_dyn;
     const int64_t bytes = (int64_t)T.row_count * kRow;
     const gchar* src = to_global((const char*)T.P) + (int64_t)T.row_begin * kRow;
@@ -545,10 +600,10 @@ __global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a, int 
     const int slot = lane / G;
     const int ls = lane % G;
     const int worker = wave * S + slot;
-    int32_t* stage = (int32_t*)(lds_dyn + bytes) + (int64_t)wave * 3 * stage_words;
     if (worker < T.worker_count)
-        sgd_worker<K, G, PREC, false, D, OFF32, true>(a, T, (uint32_t)worker, slot, ls, stage, stage + stage_words,
-                                                      (float*)(stage + 2 * stage_words), a.P, a.Q, a.ps, a.qs, tile);
+        sgd_worker<K, G, PREC, false, D, OFF32, true>(a, T, (uint32_t)worker, slot, ls, s_stage[wave][0],
+                                                      s_stage[wave][1], (float*)s_stage[wave][2], a.P, a.Q, a.ps,
+                                                      a.qs, tile);
     __syncthreads();
 
     gchar* dst = to_global((char*)T.P) + (int64_t)T.row_begin * kRow;
@@ -557,21 +612,20 @@ __global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a, int 
 }
 
 template <int K, int G, int PREC, int D, int NWAVES>
-inline hipError_t launch_tile_t(const SgdLaunch& a, int blocks, size_t lds, int stage_words, hipStream_t s)
+inline hipError_t launch_tile_t(const SgdLaunch& a, int blocks, size_t lds, hipStream_t s)
 {
     // dynamic LDS above the default limit must be allowed per kernel
     if (a.P) {
         hipError_t e = hipFuncSetAttribute((const void*)sgd_tile_kernel<K, G, PREC, D, true, NWAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((sgd_tile_kernel<K, G, PREC, D, true, NWAVES>), dim3(blocks), dim3(NWAVES * 64), lds, s, a,
-                           stage_words);
+        hipLaunchKernelGGL((sgd_tile_kernel<K, G, PREC, D, true, NWAVES>), dim3(blocks), dim3(NWAVES * 64), lds, s, a);
     } else {
         hipError_t e = hipFuncSetAttribute((const void*)sgd_tile_kernel<K, G, PREC, D, false, NWAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((sgd_tile_kernel<K, G, PREC, D, false, NWAVES>), dim3(blocks), dim3(NWAVES * 64), lds, s,
-                           a, stage_words);
+                           a);
     }
     return hipGetLastError();
 }
@@ -580,15 +634,15 @@ inline hipError_t launch_tile_t(const SgdLaunch& a, int blocks, size_t lds, int 
 // least 8 lanes) or half of it (32 bytes per lane), depth 2 and workgroups
 // of 4, 8 or 16 waves.
 template <int K, int PREC, int G>
-inline hipError_t launch_tile_w(const SgdLaunch& a, int blocks, int waves, size_t lds, int stage_words, hipStream_t s)
+inline hipError_t launch_tile_w(const SgdLaunch& a, int blocks, int waves, size_t lds, hipStream_t s)
 {
     constexpr int EB = PrecBytes<PREC>::value;
     if constexpr (G >= 8 && K % G == 0 && (K / G) * EB <= 32 && (K / G) * EB >= 2 &&
                   (PREC != kE4M3 || (K / G) % 4 == 0)) {
         switch (waves) {
-        case 4: return launch_tile_t<K, G, PREC, 2, 4>(a, blocks, lds, stage_words, s);
-        case 8: return launch_tile_t<K, G, PREC, 2, 8>(a, blocks, lds, stage_words, s);
-        case 16: return launch_tile_t<K, G, PREC, 2, 16>(a, blocks, lds, stage_words, s);
+        case 4: return launch_tile_t<K, G, PREC, 2, 4>(a, blocks, lds, s);
+        case 8: return launch_tile_t<K, G, PREC, 2, 8>(a, blocks, lds, s);
+        case 16: return launch_tile_t<K, G, PREC, 2, 16>(a, blocks, lds, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -596,14 +650,13 @@ inline hipError_t launch_tile_w(const SgdLaunch& a, int blocks, int waves, size_
 }
 
 template <int K, int PREC>
-inline hipError_t launch_tile_k(const SgdLaunch& a, int blocks, int G, int D, int waves, size_t lds, int stage_words,
-                                hipStream_t s)
+inline hipError_t launch_tile_k(const SgdLaunch& a, int blocks, int G, int D, int waves, size_t lds, hipStream_t s)
 {
     constexpr int EB = PrecBytes<PREC>::value;
     constexpr int G0 = (K * EB / 16) > 64 ? 64 : ((K * EB / 16) < 8 ? 8 : K * EB / 16);
     if (D != 2) return hipErrorInvalidValue;
-    if (G == G0) return launch_tile_w<K, PREC, G0>(a, blocks, waves, lds, stage_words, s);
-    if (G == G0 / 2) return launch_tile_w<K, PREC, G0 / 2>(a, blocks, waves, lds, stage_words, s);
+    if (G == G0) return launch_tile_w<K, PREC, G0>(a, blocks, waves, lds, s);
+    if (G == G0 / 2) return launch_tile_w<K, PREC, G0 / 2>(a, blocks, waves, lds, s);
     return hipErrorInvalidValue;
 }
 
@@ -759,6 +812,6 @@ template <int PREC>
 hipError_t launch_batched_for(const SgdLaunch& a, int blocks, int D, hipStream_t s);
 template <int PREC>
 hipError_t launch_tile_prec(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds,
-                            int stage_words, hipStream_t s);
+                            hipStream_t s);
 
 }  // namespace cumf

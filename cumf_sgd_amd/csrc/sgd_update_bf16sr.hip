@@ -21,13 +21,12 @@ hipError_t launch_batched_for<kBF16SR>(const SgdLaunch& a, int blocks, int D, hi
 }
 
 template <>
-hipError_t launch_tile_prec<kBF16SR>(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds,
-                                  int stage_words, hipStream_t s)
+hipError_t launch_tile_prec<kBF16SR>(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds, hipStream_t s)
 {
     switch (k) {
-    case 32: return launch_tile_k<32, kBF16SR>(a, blocks, G, D, waves, lds, stage_words, s);
-    case 64: return launch_tile_k<64, kBF16SR>(a, blocks, G, D, waves, lds, stage_words, s);
-    case 128: return launch_tile_k<128, kBF16SR>(a, blocks, G, D, waves, lds, stage_words, s);
+    case 32: return launch_tile_k<32, kBF16SR>(a, blocks, G, D, waves, lds, s);
+    case 64: return launch_tile_k<64, kBF16SR>(a, blocks, G, D, waves, lds, s);
+    case 128: return launch_tile_k<128, kBF16SR>(a, blocks, G, D, waves, lds, s);
     }
     return hipErrorInvalidValue;
 }

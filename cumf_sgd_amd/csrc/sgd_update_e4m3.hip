@@ -21,13 +21,12 @@ hipError_t launch_batched_for<kE4M3>(const SgdLaunch& a, int blocks, int D, hipS
 }
 
 template <>
-hipError_t launch_tile_prec<kE4M3>(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds,
-                                  int stage_words, hipStream_t s)
+hipError_t launch_tile_prec<kE4M3>(const SgdLaunch& a, int blocks, int k, int G, int D, int waves, size_t lds, hipStream_t s)
 {
     switch (k) {
-    case 32: return launch_tile_k<32, kE4M3>(a, blocks, G, D, waves, lds, stage_words, s);
-    case 64: return launch_tile_k<64, kE4M3>(a, blocks, G, D, waves, lds, stage_words, s);
-    case 128: return launch_tile_k<128, kE4M3>(a, blocks, G, D, waves, lds, stage_words, s);
+    case 32: return launch_tile_k<32, kE4M3>(a, blocks, G, D, waves, lds, s);
+    case 64: return launch_tile_k<64, kE4M3>(a, blocks, G, D, waves, lds, s);
+    case 128: return launch_tile_k<128, kE4M3>(a, blocks, G, D, waves, lds, s);
     }
     return hipErrorInvalidValue;
 }

@@ -73,8 +73,9 @@ typedef struct cumf_train_opts {
     int32_t k;                 /* factor length: 32, 64 or 128 */
     int32_t num_workers;       /* Hogwild workers (-s); 0 = 128 waves per CU (FP8 or stratum: 512), rounded for least over-sampling */
     int32_t batch_len;         /* records per window, even, <= 128: update_vector_size (sgd_kernel.cu:198); 0 = 128.
-                                  The tile schedule takes it as the longest window: each tile gets the
-                                  even length up to it whose windows fill its workgroup's waves best */
+                                  The tile schedule takes it as the longest window, at most 32 (0 = 32):
+                                  each tile gets the even length up to it whose windows fill its
+                                  workgroup's waves best */
     int32_t schedule;          /* enum cumf_schedule */
     int32_t lanes_per_sample;  /* lanes sharing one update: 8..64; 0 = 16 B of a row per lane (tile
                                   schedule: 32 B where k and the format allow it) */

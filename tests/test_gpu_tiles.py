@@ -234,8 +234,9 @@ def test_tile_full_workgroups_train_every_rating_window(gpu):
 
 def test_tile_refusals(gpu):
     """The tile schedule needs a tile partition, the default lane group,
-    depth 2, no coherent mode, at most 16 waves of workers per tile and tiles
-    that fit in LDS; a later partition drops the tiles."""
+    depth 2, no coherent mode, at most 16 waves of workers per tile, windows
+    of at most 32 records and tiles that fit in LDS; a later partition drops
+    the tiles."""
     import cumf_sgd_amd as cs
 
     u, v, r, _, _, _, m, n = _problem(seed=25, m=500, n=200, nnz=5_000)
@@ -247,11 +248,13 @@ def test_tile_refusals(gpu):
         assert e.value.code == 3
         c.partition_tiles(8)
         for bad in (dict(lanes_per_sample=64), dict(coherent=1), dict(pipeline_depth=4), dict(num_workers=1000),
-                    dict(tiles_in_flight=-1)):
+                    dict(tiles_in_flight=-1), dict(batch_len=64)):
             with pytest.raises(cs.CumfError) as e:
                 c.train(cs.make_opts(**{**base, **bad}), 0, 1)
             assert e.value.code == 1, bad
         c.train(cs.make_opts(**base), 0, 1)   # still usable after the refusals
+        c.train(cs.make_opts(**{**base, "batch_len": 0}), 0, 1)
+        assert c.last_launch().batch_len == 32   # the tile schedule's longest window by default
         c.partition_tiles(8, 700)   # 700 rows of 256 B exceed the LDS with the staged records
         with pytest.raises(cs.CumfError) as e:
             c.train(cs.make_opts(**base), 0, 1)
