@@ -163,8 +163,13 @@ int cumf_scale_ratings(cumf_ctx* ctx, float scale);
  * shape, block (u/u_seg)*vy + v/v_seg (grid_problem, sgd.cu:411-487), with a
  * stable counting sort.  offsets (ux*vy+1 entries) may be NULL. */
 int cumf_partition(cumf_ctx* ctx, int32_t ux, int32_t vy, int64_t* offsets);
-/* Partition into s x s blocks whose row and column ranges hold about equal
- * rating counts (for CUMF_SCHED_STRATUM), stable like cumf_partition. */
+/* Partition into s x s blocks of s contiguous row ranges and s contiguous
+ * column ranges (for CUMF_SCHED_STRATUM), stable like cumf_partition.  For
+ * s <= 8 (one block per XCD and diagonal) each axis is cut so that its
+ * largest range holds as few ratings as any contiguous cut allows, each range
+ * closed within that count nearest the remaining ratings' even share; for
+ * s > 8 a range ends at the first row reaching the next 1/s of the
+ * ratings. */
 int cumf_partition_strata(cumf_ctx* ctx, int32_t s, int64_t* offsets);
 /* Record order inside every block of the current partition: a stable sort
  * of each block's records by column id (CUMF_ORDER_COLUMNS) or row id
