@@ -164,11 +164,18 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
     return None
 
 
-# VALU issue peak: one wave64 vector instruction per CU per clock (four
-# SIMDs, each a wave64 instruction per 4 clocks), 256 CUs at 2.4 GHz, in
-# lane operations (64 per instruction): 39.3e12, the FP32 FMA peak of 78.6
-# TFLOP/s counted once per FMA.
-VALU_PEAK_TLANE = 256 * 2.4e9 * 64 / 1e12
+# VALU issue: 1024 SIMDs at 2.4 GHz, in SIMD-cycles per second.  A SIMD
+# issues a wave64 VALU instruction per quad-cycle, and gfx950 issues two in
+# one quad-cycle for some instruction pairs (SQ_ACTIVE_INST_VALU2 counts
+# those quad-cycles; measured: v_mul_f32, v_add_u32, v_and_b32 and v_mov_b32
+# at 1.6-1.8 wave64 instructions per CU per clock, v_fma_f32,
+# v_fma_mix_f32 and v_cvt_pk_f16_f32 at 0.95-1.08), so instruction counts
+# have no fixed peak; the level is the fraction of quad-cycles in which the
+# SIMDs were issuing VALU work, a dual-issued pair counted once.
+VALU_ISSUE_PEAK_G = 1024 * 2.4
+# L2 channels: 8 XCDs x 16 channels at 2.4 GHz, in channel-cycles per second;
+# TCC_BUSY counts the cycles a channel is busy.
+L2_CHANNEL_PEAK_G = 8 * 16 * 2.4
 
 
 def tile_waves(workers: int, lanes: int) -> int:
@@ -222,9 +229,12 @@ def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, or
         12-byte record + P and Q rows read and written); a tile launch keeps
         its P rows in LDS, and its level is the measured request bytes of the
         matching profile (TCP_TCC_READ_REQ x 128 + TCP_TCC_WRITE_REQ x 64);
+      - l2_busy: the profile's busy L2 channel-cycles (TCC_BUSY) against
+        the channels' cycles;
       - hbm: the profile's calibrated FETCH_SIZE + WRITE_SIZE against 8 TB/s;
-      - valu: the profile's VALU instructions per launch (SQ_INSTS_VALU) as
-        lane operations against the VALU issue peak.
+      - valu: the profile's VALU issue quad-cycles (SQ_ACTIVE_INST_VALU less
+        SQ_ACTIVE_INST_VALU2, a dual-issued pair counted once) against the
+        SIMDs' cycles.
     `bound` is the level with the largest fraction."""
     algo = BYTES_PER_UPDATE[prec] * upl / avg_launch_s / 1e9
     prof = profile_summary(prec, info, schedule, upl, order)
@@ -244,12 +254,19 @@ def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, or
         hbm = traffic / avg_launch_s / 1e9
         levels["hbm"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS, "unit": "GB/s",
                          "bytes": "measured: calibrated FETCH_SIZE + WRITE_SIZE per launch"}
-        if "valu_insts_per_launch" in prof:
-            v = prof["valu_insts_per_launch"] * 64 / avg_launch_s / 1e12
-            levels["valu"] = {"achieved": v, "peak": VALU_PEAK_TLANE, "frac": v / VALU_PEAK_TLANE,
-                              "unit": "Tlane-op/s",
-                              "ops": f"measured SQ_INSTS_VALU x 64 lanes "
-                                     f"({prof['valu_insts_per_launch'] / upl:.1f} wave-instructions per update)"}
+        busy = prof.get("counters_mean_per_launch", {}).get("TCC_BUSY")
+        if busy is not None:
+            b = busy / avg_launch_s / 1e9
+            levels["l2_busy"] = {"achieved": b, "peak": L2_CHANNEL_PEAK_G, "frac": b / L2_CHANNEL_PEAK_G,
+                                 "unit": "G channel-cycles/s", "cycles": "measured TCC_BUSY per launch"}
+        if "valu_dual_quad_cycles_per_launch" in prof:
+            q = prof["valu_active_quad_cycles_per_launch"] - prof["valu_dual_quad_cycles_per_launch"]
+            v = q * 4 / avg_launch_s / 1e9
+            levels["valu"] = {"achieved": v, "peak": VALU_ISSUE_PEAK_G, "frac": v / VALU_ISSUE_PEAK_G,
+                              "unit": "G SIMD-cycles/s",
+                              "cycles": f"measured VALU issue quad-cycles x 4 (SQ_ACTIVE_INST_VALU - "
+                                        f"SQ_ACTIVE_INST_VALU2; {prof['valu_insts_per_launch'] / upl:.1f} "
+                                        f"wave-instructions per update)"}
     bound = max(levels, key=lambda k: levels[k]["frac"]) if levels else None
     kernel = "sgd_tile_kernel" if tiled else "sgd_update_kernel"
     return {"bound": bound, "achieved": levels[bound]["achieved"] if bound else None,

@@ -46,11 +46,34 @@ def test_roofline_fractions_stay_below_one():
     r = bench.roofline(1, _info(), "stratum24", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
                        d["block_order"])
     assert r["traffic"] == d["traffic_bytes_per_launch"]
-    assert set(r["levels"]) == {"l2", "hbm", "valu"}
+    assert set(r["levels"]) == {"l2", "l2_busy", "hbm", "valu"}
     assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values())
     assert r["frac"] == max(lv["frac"] for lv in r["levels"].values())
     assert abs(bench.L2_PEAK_GBS - 39321.6) < 1e-6
-    assert abs(bench.VALU_PEAK_TLANE - 39.3216) < 1e-9
+    assert abs(bench.VALU_ISSUE_PEAK_G - 2457.6) < 1e-9
+    assert abs(bench.L2_CHANNEL_PEAK_G - 307.2) < 1e-9
+
+
+def test_every_profile_gives_levels_at_most_one():
+    """Every committed profile, over its own traced launch time: each level,
+    the VALU one counted as issue quad-cycles (a dual-issued pair once), is a
+    fraction of at most one, for every storage format and schedule."""
+    import glob
+    import json
+
+    precs = {"fp32": 0, "fp16": 1, "bf16": 2, "fp8": 3, "bf16sr": 4}
+    paths = sorted(glob.glob(bench.os.path.join(bench.ROOT, "profiles", "sgd_*_medium_*.json")))
+    assert len(paths) >= 6
+    for path in paths:
+        d = json.load(open(path))
+        base = bench.os.path.basename(path)[:-5].split("_")
+        prec, sched = precs[base[2]], base[4]
+        info = _info(lanes=d["lanes_per_sample"], workers=d["workers"])
+        r = bench.roofline(prec, info, sched, d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
+                           d["block_order"])
+        assert r["profile"] == d["source"], path
+        assert {"l2", "l2_busy", "hbm", "valu"} <= set(r["levels"]), path
+        assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values()), (path, r["levels"])
 
 
 def test_tile_roofline_uses_measured_l2_requests():
@@ -62,7 +85,7 @@ def test_tile_roofline_uses_measured_l2_requests():
 
     for path in glob.glob(bench.os.path.join(bench.ROOT, "profiles", "sgd_tile_*_medium_tile8.json")):
         d = json.load(open(path))
-        prec = {"fp32": 0, "fp16": 1}[path.split("sgd_tile_")[1].split("_")[0]]
+        prec = {"fp32": 0, "fp16": 1, "bf16": 2}[path.split("sgd_tile_")[1].split("_")[0]]
         info = _info(lanes=d["lanes_per_sample"], workers=d["workers"])
         r = bench.roofline(prec, info, "tile8", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
                            d["block_order"])
