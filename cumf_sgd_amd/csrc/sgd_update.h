@@ -14,9 +14,10 @@ namespace cumf {
 // A wave of 64 lanes holds S = 64/G workers ("slots"); the G lanes of a slot
 // share one rating update, lane l of the slot owning the E = K/G consecutive
 // elements [lE, lE+E) of the P and Q rows, so a row is one coalesced
-// K*bytes gather.  Per window, a slot's L records (u, v, r) go from memory
-// straight into LDS (global_load_lds; the next window's load during the
-// current window's last round), and the slot walks them in order as a D-deep
+// K*bytes gather.  Per window, a slot's L records (u, v, r) are staged in
+// LDS (loaded straight into LDS with global_load_lds, or in tile launches
+// through registers; the next window's records load during the current
+// window's last round), and the slot walks them in order as a D-deep
 // software pipeline: the rows of sample j + D are requested right after
 // sample j is written back, so D row pairs are in flight per slot.  A prefetched row may predate the write-back
 // of one of the D - 1 samples before it; those written rows are kept in a
