@@ -297,3 +297,84 @@ def test_epoch_per_call_reuses_the_plan_bit_exactly(gpu):
         (p3, q3), (p1, q1) = out
         assert np.array_equal(p3.view(np.uint8), p1.view(np.uint8)), sched
         assert np.array_equal(q3.view(np.uint8), q1.view(np.uint8)), sched
+
+
+def _first_crossing_ranges(deg, s):
+    """Range sizes when range j ends at the first row whose running count
+    reaches j/s of the ratings (the cut for s > 8)."""
+    nnz = int(deg.sum())
+    b = [0]
+    c = np.cumsum(deg)
+    for j in range(1, s):
+        hit = np.flatnonzero(c * s >= nnz * j)
+        b.append(int(hit[0]) + 1 if len(hit) else len(deg))
+    b.append(len(deg))
+    b = np.maximum.accumulate(np.array(b))
+    return np.diff(np.concatenate([[0], c])[b])
+
+
+def _least_largest_range(deg, s):
+    """The least count M for which ranges filled in id order, each closed
+    before the row that would take it past M, number at most s."""
+    def ranges(M):
+        n, run = 1, 0
+        for d in deg:
+            if run + d > M:
+                n, run = n + 1, 0
+            run += d
+        return n
+
+    lo, hi = max(int(deg.max()), -(-int(deg.sum()) // s)), max(int(deg.sum()), 1)
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if ranges(mid) <= s:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
+
+
+def _capped_nearest_ranges(deg, s):
+    """Range sizes of the cut for s <= 8: within the least largest count M,
+    each range closed at the row boundary nearer the remaining ratings' even
+    share."""
+    M = _least_largest_range(deg, s)
+    sizes, run, left = [], 0, int(deg.sum())
+    for d in deg:
+        d = int(d)
+        k = s - len(sizes)
+        if k > 1 and run > 0 and (run + d > M or 2 * run * k + d * k > 2 * left):
+            sizes.append(run)
+            left -= run
+            run = 0
+        run += d
+    sizes.append(run)
+    return np.array(sizes + [0] * (s - len(sizes)))
+
+
+def test_strata_ranges_balance_the_largest_when_each_xcd_trains_one_block(gpu):
+    """cumf_partition_strata's row and column ranges, restated from the
+    degree histograms of the device's ids: for s <= 8 (one block per XCD and
+    diagonal) the largest range holds exactly the least count any contiguous
+    cut allows, each range closed nearest the remaining even share; for
+    s > 8 a range ends at the first row reaching the next 1/s.  The columns
+    are Zipf-skewed, so one column holds more than 1/16 of the ratings."""
+    rng = np.random.default_rng(31)
+    m, n, nnz = 3000, 400, 80_000
+    p = 1.0 / np.arange(1, n + 1) ** 1.1
+    v = rng.choice(n, nnz, p=p / p.sum()).astype(np.int32)
+    u = rng.integers(0, m, nnz).astype(np.int32)
+    r = rng.uniform(1, 5, nnz).astype(np.float32)
+    with _ctx(u, v, r, m, n) as c:
+        for s in (2, 4, 8, 16, 24):
+            off = np.asarray(c.partition_strata(s))
+            gu, gv, _ = c.get_ratings()
+            blocks = np.diff(off).reshape(s, s)
+            for ids, size, bands in ((gu, m, blocks.sum(1)), (gv, n, blocks.sum(0))):
+                deg = np.bincount(np.asarray(ids), minlength=size).astype(np.int64)
+                want = _capped_nearest_ranges(deg, s) if s <= 8 else _first_crossing_ranges(deg, s)
+                assert np.array_equal(bands, want), (s, bands, want)
+                if s <= 8:
+                    assert bands.max() == _least_largest_range(deg, s)
+        hot = np.bincount(np.asarray(gv), minlength=n).max()
+        assert hot * 16 > nnz
