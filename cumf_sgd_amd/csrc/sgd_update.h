@@ -585,8 +585,8 @@ __global__ __launch_bounds__(NWAVES * 64) void sgd_tile_kernel(SgdLaunch a)
     const SgdTask T = a.tasks[ti];
     if (T.nnz <= 0 || T.row_count <= 0) return;
 
-    // the staged records: an array of their own, so that the compiler sees
-    // the tile's reads and writes apart from the record loads into LDS
+    // the staged records of each wave (a static array beside the dynamic
+    // tile; kSgdTileMaxBatch records per worker)
     constexpr int kStageWords = 64 * ((kSgdTileMaxBatch + G - 1) / G);
     __shared__ int32_t s_stage[NWAVES][3][kStageWords];
     lchar* tile = (lchar*)lds_dyn;
