@@ -173,9 +173,10 @@ def cpu_baseline(R, threads: int, sample: int, iters: int):
 # have no fixed peak; the level is the fraction of quad-cycles in which the
 # SIMDs were issuing VALU work, a dual-issued pair counted once.
 VALU_ISSUE_PEAK_G = 1024 * 2.4
-# L2 channels: 8 XCDs x 16 channels at 2.4 GHz, in channel-cycles per second;
-# TCC_BUSY counts the cycles a channel is busy.
-L2_CHANNEL_PEAK_G = 8 * 16 * 2.4
+# L2 requests: 8 XCDs x 16 channels, each accepting one request per clock
+# (its 128 B/clock path serves one 128-byte read per clock), at 2.4 GHz, in
+# requests per second.
+L2_REQUEST_PEAK_G = 8 * 16 * 2.4
 
 
 def tile_waves(workers: int, lanes: int) -> int:
@@ -229,8 +230,9 @@ def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, or
         12-byte record + P and Q rows read and written); a tile launch keeps
         its P rows in LDS, and its level is the measured request bytes of the
         matching profile (TCP_TCC_READ_REQ x 128 + TCP_TCC_WRITE_REQ x 64);
-      - l2_busy: the profile's busy L2 channel-cycles (TCC_BUSY) against
-        the channels' cycles;
+      - l2_requests: the profile's L2 requests (TCP_TCC_READ_REQ +
+        TCP_TCC_WRITE_REQ, a 64-byte write counting as one) against one
+        request per channel per clock;
       - hbm: the profile's calibrated FETCH_SIZE + WRITE_SIZE against 8 TB/s;
       - valu: the profile's VALU issue quad-cycles (SQ_ACTIVE_INST_VALU less
         SQ_ACTIVE_INST_VALU2, a dual-issued pair counted once) against the
@@ -254,11 +256,12 @@ def roofline(prec: int, info, schedule: str, avg_launch_s: float, upl: float, or
         hbm = traffic / avg_launch_s / 1e9
         levels["hbm"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS, "unit": "GB/s",
                          "bytes": "measured: calibrated FETCH_SIZE + WRITE_SIZE per launch"}
-        busy = prof.get("counters_mean_per_launch", {}).get("TCC_BUSY")
-        if busy is not None:
-            b = busy / avg_launch_s / 1e9
-            levels["l2_busy"] = {"achieved": b, "peak": L2_CHANNEL_PEAK_G, "frac": b / L2_CHANNEL_PEAK_G,
-                                 "unit": "G channel-cycles/s", "cycles": "measured TCC_BUSY per launch"}
+        nreq = prof["tcp_tcc_read_req_per_launch"] + prof["tcp_tcc_write_req_per_launch"]
+        rq = nreq / avg_launch_s / 1e9
+        levels["l2_requests"] = {"achieved": rq, "peak": L2_REQUEST_PEAK_G, "frac": rq / L2_REQUEST_PEAK_G,
+                                 "unit": "G requests/s",
+                                 "requests": f"measured TCP_TCC_READ_REQ + TCP_TCC_WRITE_REQ "
+                                             f"({nreq / upl:.2f} per update)"}
         if "valu_dual_quad_cycles_per_launch" in prof:
             q = prof["valu_active_quad_cycles_per_launch"] - prof["valu_dual_quad_cycles_per_launch"]
             v = q * 4 / avg_launch_s / 1e9

@@ -46,12 +46,12 @@ def test_roofline_fractions_stay_below_one():
     r = bench.roofline(1, _info(), "stratum24", d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
                        d["block_order"])
     assert r["traffic"] == d["traffic_bytes_per_launch"]
-    assert set(r["levels"]) == {"l2", "l2_busy", "hbm", "valu"}
+    assert set(r["levels"]) == {"l2", "l2_requests", "hbm", "valu"}
     assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values())
     assert r["frac"] == max(lv["frac"] for lv in r["levels"].values())
     assert abs(bench.L2_PEAK_GBS - 39321.6) < 1e-6
     assert abs(bench.VALU_ISSUE_PEAK_G - 2457.6) < 1e-9
-    assert abs(bench.L2_CHANNEL_PEAK_G - 307.2) < 1e-9
+    assert abs(bench.L2_REQUEST_PEAK_G - 307.2) < 1e-9
 
 
 def test_every_profile_gives_levels_at_most_one():
@@ -72,7 +72,7 @@ def test_every_profile_gives_levels_at_most_one():
         r = bench.roofline(prec, info, sched, d["trace_avg_launch_ms_timed_region"] / 1e3, d["updates_per_launch"],
                            d["block_order"])
         assert r["profile"] == d["source"], path
-        assert {"l2", "l2_busy", "hbm", "valu"} <= set(r["levels"]), path
+        assert {"l2", "l2_requests", "hbm", "valu"} <= set(r["levels"]), path
         assert all(0 < lv["frac"] <= 1 for lv in r["levels"].values()), (path, r["levels"])
 
 
