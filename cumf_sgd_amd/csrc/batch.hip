@@ -379,13 +379,16 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     for (size_t i = 0; i < b->probs.size(); ++i) {
         const Problem& p = b->probs[i];
         if (p.nnz == 0) continue;
+        // workers from the wave budget at k/2 lanes per worker (k/4 for
+        // FP8), packed 16 bytes of a row per lane into the waves that run
         const int G = sgd_batch_group(p.k, b->prec), S = 64 / G;
+        const int Sb = 64 / sgd_batch_budget_group(p.k, b->prec);
         const double cost = (double)p.nnz * (12.0 + 4.0 * p.k * eb_of(b->prec));
         int64_t waves = serial ? 1 : std::max<int64_t>(1, (int64_t)std::llround(total_waves * cost / cost_sum));
         // at least 16 windows per worker and epoch: a small problem runs on
         // few workers (near-serial, like its serial solve) instead of many
         // concurrent ones on a handful of rows
-        int64_t workers = serial ? 1 : std::max<int64_t>(1, std::min<int64_t>(waves * S, p.nnz / (16LL * L)));
+        int64_t workers = serial ? 1 : std::max<int64_t>(1, std::min<int64_t>(waves * Sb, p.nnz / (16LL * L)));
         waves = (workers + S - 1) / S;
         SgdTask t{};
         t.u = p.u; t.v = p.v; t.r = p.r; t.nnz = p.nnz;

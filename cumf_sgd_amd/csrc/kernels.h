@@ -79,9 +79,11 @@ size_t sgd_tile_lds_bytes(int k, int prec, int G, int rows, int waves);
 size_t sgd_tile_lds_limit();
 hipError_t launch_sgd_tile(const SgdLaunch& a, int blocks, int k, int prec, int G, int D, int rows, int waves,
                            hipStream_t s);
-// Batched launch: tasks of mixed k (32, 64, 128), lane group k/2 (k/4 for FP8).
+// Batched launch: tasks of mixed k (32, 64, 128), 16 bytes of a row per lane (8 to 64 lanes).
 hipError_t launch_sgd_batched(const SgdLaunch& a, int blocks, int prec, int D, hipStream_t s);
 int sgd_batch_group(int k, int prec);
+// Lanes per worker in which a batch's wave budget is counted (k/2; FP8 k/4).
+int sgd_batch_budget_group(int k, int prec);
 
 // *p = value, and *p += 1 (device-side epoch counter of a captured epoch loop)
 hipError_t launch_set_int(int32_t* p, int32_t value, hipStream_t st);

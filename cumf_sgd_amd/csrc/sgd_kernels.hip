@@ -105,6 +105,11 @@ int sgd_batch_group(int k, int prec)
     }
 }
 
+int sgd_batch_budget_group(int k, int prec)
+{
+    return prec == kE4M3 ? batch_budget_group<kE4M3>(k) : batch_budget_group<kF16>(k);
+}
+
 // ---------------------------------------------------------------------------
 // Device-side epoch counter.
 // ---------------------------------------------------------------------------

@@ -673,9 +673,19 @@ __device__ inline T* wave_uniform(T* p)
     return (T*)(((uint64_t)hi << 32) | lo);
 }
 
-// Lane group of a factor length in batched launches (one per k).
+// Lane group of a factor length in batched launches (one per k): 16 bytes
+// of a row per lane, 8 to 64 lanes, as in single-problem launches.
 template <int PREC>
 __host__ __device__ constexpr int batch_group(int k)
+{
+    return k * PrecBytes<PREC>::value / 16 > 64 ? 64 : (k * PrecBytes<PREC>::value / 16 < 8 ? 8 : k * PrecBytes<PREC>::value / 16);
+}
+// The batch's worker budget is counted in waves of k/2 lanes per worker
+// (k/4 for FP8), the unit its waves-per-CU defaults and autotuner were
+// measured in: a problem keeps that many workers, packed 16 bytes of a row
+// per lane into fewer waves.
+template <int PREC>
+__host__ __device__ constexpr int batch_budget_group(int k)
 {
     return PREC == kE4M3 ? k / 4 : (k / 2 > 64 ? 64 : k / 2);
 }

@@ -13,7 +13,10 @@ def _problems(count, seed=11, scale=0.05):
 
 
 def _oracle_group(k, prec):
-    return k // 4 if prec == 3 else min(64, k // 2)
+    """Lanes per sample in batched launches: 16 bytes of a row per lane,
+    8 to 64 lanes."""
+    eb = {0: 4, 1: 2, 2: 2, 3: 1, 4: 2}[prec]
+    return max(8, min(64, k * eb // 16))
 
 
 @pytest.mark.parametrize("prec", [0, 1, 2, 3, 4])
