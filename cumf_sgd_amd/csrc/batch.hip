@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstring>
 
 #include <algorithm>
 #include <cmath>
@@ -71,6 +72,9 @@ struct cumf_batch {
     std::vector<Problem> probs;
     float* lr = nullptr;
     int lr_n = 0;
+    // rates the table holds: its first lr_valid entries, for (alpha, beta, lrate) = lr_key
+    int lr_valid = 0;
+    float lr_key[3] = {0, 0, 0};
     SgdTask* tasks = nullptr;
     size_t tasks_n = 0;
     double* partials = nullptr;
@@ -355,17 +359,25 @@ int cumf_batch_train(cumf_batch* b, const cumf_train_opts* o, int32_t e0, int32_
     while (L % D) D >>= 1;
     if (b->probs.empty() || e1 == e0) return CUMF_OK;
 
-    // learning-rate table, singleGPU/sgd_kernel.cu:137-154
-    std::vector<float> lr((size_t)e1 + 4);
-    for (size_t i = 0; i < lr.size(); ++i) lr[i] = (float)(o->alpha / (1 + o->beta * std::pow((double)i, 1.5)) + o->lrate);
-    if ((int)lr.size() > b->lr_n) {
-        dev_free(b->lr);
-        b->lr = nullptr;
-        b->lr_n = 0;
-        if ((rc = dev_alloc((void**)&b->lr, sizeof(float) * lr.size()))) return rc;
-        b->lr_n = (int)lr.size();
+    // learning-rate table, singleGPU/sgd_kernel.cu:137-154, grown in steps of
+    // kLrChunk epochs and written only when the call needs more epochs or
+    // other rates (its address is in the tasks and in the graph key)
+    const float lr_key[3] = {o->alpha, o->beta, o->lrate};
+    if (!b->lr || b->lr_valid < e1 + 4 || std::memcmp(lr_key, b->lr_key, sizeof(lr_key)) != 0) {
+        std::vector<float> lr((size_t)(e1 + 4 + kLrChunk - 1) / kLrChunk * kLrChunk);
+        lr_table(o, (int)lr.size(), lr.data());
+        b->lr_valid = 0;
+        if ((int)lr.size() > b->lr_n) {
+            dev_free(b->lr);
+            b->lr = nullptr;
+            b->lr_n = 0;
+            if ((rc = dev_alloc((void**)&b->lr, sizeof(float) * lr.size()))) return rc;
+            b->lr_n = (int)lr.size();
+        }
+        BHIP_TRY(hipMemcpyAsync(b->lr, lr.data(), sizeof(float) * lr.size(), hipMemcpyHostToDevice, b->stream));
+        b->lr_valid = (int)lr.size();
+        std::memcpy(b->lr_key, lr_key, sizeof(lr_key));
     }
-    BHIP_TRY(hipMemcpyAsync(b->lr, lr.data(), sizeof(float) * lr.size(), hipMemcpyHostToDevice, b->stream));
 
     // waves per problem in proportion to its bytes of work; no more workers
     // than windows

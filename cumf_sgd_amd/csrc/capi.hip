@@ -137,6 +137,9 @@ struct cumf_ctx {
     bool have_factors = false;
 
     float* lr = nullptr; size_t lr_cap = 0;
+    // rates the table holds: its first lr_valid entries, for (alpha, beta, lrate) = lr_key
+    size_t lr_valid = 0;
+    float lr_key[3] = {0, 0, 0};
     SgdTask* tasks = nullptr; size_t tasks_cap = 0;
     int32_t* epoch_dev = nullptr; size_t epoch_dev_cap = 0;
     hipGraphExec_t graph_exec = nullptr;
@@ -1383,13 +1386,24 @@ ShapeInput shape_input(const cumf_ctx* c, const cumf_train_opts* o)
     return ShapeInput{c->k, c->prec, one ? std::max(1, c->cu_count / kNumXcd) : c->cu_count, c->nnz};
 }
 
+// The table is written only when the call needs more epochs than it holds or
+// other rates.  Its address is part of the plan key and of every captured
+// task, and growing it by one epoch per call would free device memory (a
+// device-wide synchronisation), re-plan, re-upload the tasks and re-capture
+// the epoch graph on every call of a one-epoch-per-call loop.
 int upload_lr(cumf_ctx* c, const cumf_train_opts* o, int n)
 {
-    std::vector<float> h((size_t)n);
-    lr_table(o, n, h.data());
-    int rc = ensure(&c->lr, &c->lr_cap, (size_t)n);
+    const float key[3] = {o->alpha, o->beta, o->lrate};
+    if (c->lr && c->lr_valid >= (size_t)n && std::memcmp(key, c->lr_key, sizeof(key)) == 0) return CUMF_OK;
+    const size_t want = ((size_t)n + kLrChunk - 1) / kLrChunk * kLrChunk;
+    std::vector<float> h(want);
+    lr_table(o, (int)want, h.data());
+    c->lr_valid = 0;
+    int rc = ensure(&c->lr, &c->lr_cap, want);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->lr, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->lr, h.data(), sizeof(float) * want, hipMemcpyHostToDevice, c->stream));
+    c->lr_valid = want;
+    std::memcpy(c->lr_key, key, sizeof(key));
     return CUMF_OK;
 }
 

@@ -66,6 +66,9 @@ constexpr int kTileWorkers = 32;
 // FP16 tiles, bound by VALU issue, the same from 1 to 64.
 constexpr int kTileWindowSteps = 16;
 constexpr int kTileStrata = 8;
+// The learning-rate table grows in steps of this many epochs, so that an
+// epoch loop driven one epoch per call keeps the table where it is.
+constexpr int kLrChunk = 1024;
 constexpr int kTileFp32Rounds = 2;
 
 // lr[i] = float(alpha / (1 + beta * i^1.5) + lrate) in double for i < n
