@@ -633,8 +633,10 @@ int partition_with(cumf_ctx* c, const std::vector<int64_t>& bu, const std::vecto
 //   schedule's 8 x 8): the largest range sets every launch.  M is the least
 //   capacity for which ranges filled in id order, one closed before the row
 //   that would take it past M, fit in s ranges (a binary search; M is at
-//   least the largest row and nnz / s); each range is then closed, within M,
-//   at the row boundary nearer to the remaining ratings' even share.  A cut at
+//   least the largest row and nnz / s); each range is then closed at the
+//   row boundary nearer to the remaining ratings' even share, unless the
+//   rows after it would then not fit in the ranges left at M, and always
+//   before the row that would take it past M, so no range exceeds M.  A cut at
 //   the first row reaching the next 1/s of the ratings left the 99M
 //   workload's item range holding its most popular item (5.8% of all
 //   ratings) at 1.2 times the mean, and every launch that much longer.
@@ -678,16 +680,31 @@ std::vector<int64_t> balanced_bounds(const std::vector<int32_t>& deg, int s, int
         if (ranges(mid) <= s) hi = mid;
         else lo = mid + 1;
     }
+    // need[i]: the ranges of at most M that rows [i, rows) take when filled
+    // in id order (the fewest any cut of them allows), through end[i], the
+    // row after the first such range (two pointers)
+    std::vector<int32_t> need((size_t)rows + 1, 0), end((size_t)rows, 0);
+    {
+        int64_t j = 0, sum = 0;
+        for (int64_t i = 0; i < rows; ++i) {
+            if (j < i) j = i, sum = 0;
+            while (j < rows && sum + deg[(size_t)j] <= lo) sum += deg[(size_t)j++];
+            end[(size_t)i] = (int32_t)j;
+            sum -= deg[(size_t)i];
+        }
+        for (int64_t i = rows - 1; i >= 0; --i) need[(size_t)i] = 1 + need[(size_t)end[(size_t)i]];
+    }
     int next = 1;
     int64_t run = 0, left = nnz;
     for (int64_t i = 0; i < rows; ++i) {
         const int64_t d = deg[(size_t)i];
         if (next < s && run > 0) {
             // close before row i when it would pass M, or when the range is
-            // nearer the even share without it: (left / k - run) < (run + d -
-            // left / k), in integers
+            // nearer the even share without it, (left / k - run) < (run + d -
+            // left / k) in integers, and rows [i, rows) still fit in the
+            // k - 1 ranges left (else a later range would pass M)
             const int64_t k = s - next + 1;
-            if (run + d > lo || 2 * run * k + d * k > 2 * left) {
+            if (run + d > lo || (2 * run * k + d * k > 2 * left && need[(size_t)i] <= k - 1)) {
                 b[next++] = i;
                 left -= run;
                 run = 0;

@@ -12,6 +12,7 @@ index in the tile list as tag.
 """
 import numpy as np
 import pytest
+from ranges_rule import capped_nearest_ranges, first_crossing_ranges, least_largest_range
 
 pytestmark = pytest.mark.gpu
 
@@ -299,59 +300,6 @@ def test_epoch_per_call_reuses_the_plan_bit_exactly(gpu):
         assert np.array_equal(q3.view(np.uint8), q1.view(np.uint8)), sched
 
 
-def _first_crossing_ranges(deg, s):
-    """Range sizes when range j ends at the first row whose running count
-    reaches j/s of the ratings (the cut for s > 8)."""
-    nnz = int(deg.sum())
-    b = [0]
-    c = np.cumsum(deg)
-    for j in range(1, s):
-        hit = np.flatnonzero(c * s >= nnz * j)
-        b.append(int(hit[0]) + 1 if len(hit) else len(deg))
-    b.append(len(deg))
-    b = np.maximum.accumulate(np.array(b))
-    return np.diff(np.concatenate([[0], c])[b])
-
-
-def _least_largest_range(deg, s):
-    """The least count M for which ranges filled in id order, each closed
-    before the row that would take it past M, number at most s."""
-    def ranges(M):
-        n, run = 1, 0
-        for d in deg:
-            if run + d > M:
-                n, run = n + 1, 0
-            run += d
-        return n
-
-    lo, hi = max(int(deg.max()), -(-int(deg.sum()) // s)), max(int(deg.sum()), 1)
-    while lo < hi:
-        mid = (lo + hi) // 2
-        if ranges(mid) <= s:
-            hi = mid
-        else:
-            lo = mid + 1
-    return lo
-
-
-def _capped_nearest_ranges(deg, s):
-    """Range sizes of the cut for s <= 8: within the least largest count M,
-    each range closed at the row boundary nearer the remaining ratings' even
-    share."""
-    M = _least_largest_range(deg, s)
-    sizes, run, left = [], 0, int(deg.sum())
-    for d in deg:
-        d = int(d)
-        k = s - len(sizes)
-        if k > 1 and run > 0 and (run + d > M or 2 * run * k + d * k > 2 * left):
-            sizes.append(run)
-            left -= run
-            run = 0
-        run += d
-    sizes.append(run)
-    return np.array(sizes + [0] * (s - len(sizes)))
-
-
 def test_strata_ranges_balance_the_largest_when_each_xcd_trains_one_block(gpu):
     """cumf_partition_strata's row and column ranges, restated from the
     degree histograms of the device's ids: for s <= 8 (one block per XCD and
@@ -372,9 +320,30 @@ def test_strata_ranges_balance_the_largest_when_each_xcd_trains_one_block(gpu):
             blocks = np.diff(off).reshape(s, s)
             for ids, size, bands in ((gu, m, blocks.sum(1)), (gv, n, blocks.sum(0))):
                 deg = np.bincount(np.asarray(ids), minlength=size).astype(np.int64)
-                want = _capped_nearest_ranges(deg, s) if s <= 8 else _first_crossing_ranges(deg, s)
+                want = capped_nearest_ranges(deg, s) if s <= 8 else first_crossing_ranges(deg, s)
                 assert np.array_equal(bands, want), (s, bands, want)
                 if s <= 8:
-                    assert bands.max() == _least_largest_range(deg, s)
+                    assert bands.max() == least_largest_range(deg, s)
         hot = np.bincount(np.asarray(gv), minlength=n).max()
         assert hot * 16 > nnz
+
+
+@pytest.mark.parametrize("deg,s", [([1, 3, 1, 0, 5, 1], 3), ([0, 0, 5, 1, 1, 1, 3, 3, 8, 8, 1], 5),
+                                   ([30, 5, 0, 1, 13, 13, 5, 3, 30, 8, 5], 4), ([3, 13, 3, 1, 30, 13, 1, 30, 1], 5)])
+def test_strata_ranges_stay_within_the_least_largest_range(gpu, deg, s):
+    """Degree histograms where closing every range at the boundary nearer
+    the even share would leave a later range above the least largest count:
+    the library's row and column ranges (both sides given the histogram)
+    equal the restated cut and their largest is that least count."""
+    deg = np.array(deg, dtype=np.int64)
+    m = len(deg)
+    rng = np.random.default_rng(5)
+    u = np.repeat(np.arange(m, dtype=np.int32), deg)
+    v = rng.permutation(u).astype(np.int32)
+    r = rng.uniform(1, 5, len(u)).astype(np.float32)
+    with _ctx(u, v, r, m, m) as c:
+        off = np.asarray(c.partition_strata(s))
+        blocks = np.diff(off).reshape(s, s)
+    want = capped_nearest_ranges(deg, s)
+    assert np.array_equal(blocks.sum(1), want) and np.array_equal(blocks.sum(0), want)
+    assert want.max() == least_largest_range(deg, s)
