@@ -474,8 +474,10 @@ __device__ __forceinline__ void sgd_worker(const SgdLaunch& a, const SgdTask& T,
                         // its Q row the other half
                         if (e >= 2 && (e & 1) == 0) wr = sr_next(wr);
                         if constexpr (PREC == kBF16SR && E % 2 == 0) {
-                            // the instruction writes half (e & 1) of the word directly
-                            const uint32_t lo = wr << 16, hi = wr & 0xffff0000u;
+                            // the instruction writes half (e & 1) of the word directly,
+                            // and reads only the top half of its random word, so the
+                            // word itself stands for its own top half
+                            const uint32_t lo = wr << 16, hi = wr;
                             const uint32_t rp = (e & 1) ? hi : lo;
                             const uint32_t rq = (e & 1) ? lo : hi;
                             const uint32_t op = (e & 1) ? po.w[e >> 1] : 0u, oq = (e & 1) ? qo.w[e >> 1] : 0u;
