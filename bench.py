@@ -284,7 +284,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed epochs first: the GPU reaches its steady epoch time after about 20")
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "bf16sr", "fp8"])
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
